@@ -1,0 +1,230 @@
+/*
+ * meshtrex_amd C ABI: isosurface extraction (block min/max -> active-block
+ * filter -> PMB marching cubes) on one AMD MI355X (gfx950) GPU.
+ *
+ * Plain C: POD structs, device pointers as `void*`/typed pointers, sizes as
+ * fixed-width integers, streams as `void*` (a hipStream_t).  Every entry point
+ * returns an mtx_status and never asserts; mtx_context_last_error() gives the
+ * message of the last failure on a context.
+ *
+ * The three stages replace the reference's host managers:
+ *   mtx_compute_min_max      <- computeMinMaxMip          (src/minMaxManager.h:20-21,
+ *                                                          src/minMaxManager.cpp:63-225)
+ *   mtx_filter_active_blocks <- filterActiveBlocks        (src/filteringManager.h:9-10,
+ *                                                          src/filteringManager.cpp:20-141)
+ *   mtx_read_active_count    <- readActiveBlockCount      (src/filteringManager.h:13,
+ *                                                          src/filteringManager.cpp:144-164)
+ *   mtx_extract              <- extractMeshletDescriptors (src/extractionManager.h:10-11,
+ *                                                          src/extractionManager.cpp:250-717)
+ * and the output structs mirror MinMaxOutput (src/minMaxOutput.h:10-64),
+ * FilteringOutput (src/filteringOutput.h:11-74) and ExtractionOutput
+ * (src/extractionOutput.h:10-134).  The C++ wrapper with the reference's
+ * names and RAII outputs is include/meshtrex_amd/managers.hpp.
+ *
+ * Streams: a NULL stream means "use the context's own stream and synchronise
+ * before returning" (the reference's externalCmd == VK_NULL_HANDLE case);
+ * with a caller stream every call is asynchronous and stream-ordered.  A
+ * context's scratch memory is shared by its calls: use one context per
+ * concurrently running stream.
+ */
+#ifndef MESHTREX_AMD_MTX_H
+#define MESHTREX_AMD_MTX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MTX_ABI_VERSION 1
+#define MTX_MAX_LEVELS 24
+
+typedef enum mtx_status {
+    MTX_OK = 0,
+    MTX_EINVAL = 1,     /* bad argument (dims, block edge, dtype, null pointer) */
+    MTX_ENOMEM = 2,     /* device allocation failed */
+    MTX_EDEVICE = 3,    /* HIP runtime error */
+    MTX_ECAPACITY = 4,  /* output buffers too small (reuse mode) */
+    MTX_ESTATE = 5      /* inputs do not belong together (e.g. filter of another min-max) */
+} mtx_status;
+
+typedef enum mtx_dtype { MTX_U8 = 0, MTX_F32 = 1 } mtx_dtype;
+
+/* Block activity rule.
+ * MTX_RULE_CPU: min != max and round(iso) in [min, max] (u8, integer compare)
+ *               or iso in [min, max] (f32): src/blockFilteringTestUtils.h:456-468.
+ * MTX_RULE_GPU: iso >= min and iso <= max in fp32: the reference's live shader
+ *               src/shaders/occupiedBlockPrefixSum.comp.glsl:84. */
+typedef enum mtx_rule { MTX_RULE_CPU = 0, MTX_RULE_GPU = 1 } mtx_rule;
+
+/* Byte-compatible with the reference's PushConstants (src/common.h:29-34):
+ * uvec4 volumeDim, uvec4 blockDim, uvec4 blockGridDim, float isovalue; 64 B.
+ * Only .x of blockDim is used (cubic blocks, edge 1..16 cells). */
+typedef struct mtx_push_constants {
+    uint32_t volumeDim[4];
+    uint32_t blockDim[4];
+    uint32_t blockGridDim[4];
+    float isovalue;
+    uint32_t _pad[3];
+} mtx_push_constants;
+
+/* The volume: x-fastest, linear index x + X*(y + Y*z) (src/testMinMax.h:97-101).
+ * on_device = 1: `data` is device memory and stays owned by the caller.
+ * on_device = 0: `data` is host memory; mtx_compute_min_max uploads it into a
+ * device buffer owned by the min-max output (the reference's volumeImage). */
+typedef struct mtx_volume {
+    const void* data;
+    uint32_t dims[3];
+    uint32_t dtype;     /* mtx_dtype */
+    uint32_t on_device;
+    uint32_t _pad;
+} mtx_volume;
+
+/* Options; pass NULL for the defaults shown. */
+typedef struct mtx_options {
+    uint32_t overlap;        /* 1: block window [B*b, B*b+B] (live GPU shader, default);
+                                0: [B*b, B*b+B-1] (reference CPU oracle; min-max only) */
+    uint32_t rule;           /* mtx_rule, default MTX_RULE_CPU */
+    uint32_t build_pyramid;  /* 1 (default): all mip levels of the min/max pyramid */
+    uint32_t fuse_filter;    /* 1 (default): min-max also writes the activity mask of
+                                pc->isovalue, so the filter does not re-read the pairs */
+    uint32_t normals;        /* 1 (default): vertex normals; 0: normal = (0,0,0,0) */
+    uint32_t edge_ids;       /* 1: also write each vertex's owning edge ID (default 0) */
+    uint32_t case_indices;   /* 1: also write the case index of every cell of every active
+                                block (default 0) */
+    uint32_t voxel_positions;/* 1: also write float3 positions in voxel units (default 0) */
+    uint32_t reuse;          /* 1: mtx_extract reuses the output's existing buffers: no
+                                allocation, no host synchronisation; overflow is reported
+                                by mtx_read_extraction_counts (default 0) */
+    uint32_t _reserved[7];
+} mtx_options;
+
+/* Min/max of every block (level 0) and the min/max pyramid, as uint32 key
+ * pairs: u8 values as-is, f32 through the order-preserving map
+ * key = bits ^ (sign ? 0xFFFFFFFF : 0x80000000) (mtx_key_f32). */
+typedef struct mtx_minmax_output {
+    uint32_t* minmax;                          /* device, 2 * level_offset[num_levels] */
+    uint64_t level_offset[MTX_MAX_LEVELS + 1]; /* in pairs */
+    uint32_t level_extent[MTX_MAX_LEVELS][3];
+    uint32_t num_levels;
+    uint32_t dtype;
+    uint32_t dims[3];
+    uint32_t block_dim;
+    uint32_t grid[3];
+    uint32_t overlap;
+    const void* volume;                        /* device volume the pairs describe */
+    void* owned_volume;                        /* set when uploaded from host memory */
+    uint32_t* active_mask;                     /* fused activity bits, or NULL */
+    uint32_t mask_words_per_row;
+    float mask_isovalue;
+    uint32_t mask_rule;
+    uint32_t mask_valid;
+} mtx_minmax_output;
+
+typedef struct mtx_filtering_output {
+    uint32_t* compacted_block_ids;  /* device, ascending, capacity = number of blocks */
+    uint32_t* active_block_count;   /* device, one uint32 */
+    uint32_t activeBlockCount;      /* host mirror, filled by mtx_read_active_count */
+    uint32_t capacity;
+    float isovalue;
+    uint32_t rule;
+} mtx_filtering_output;
+
+/* VertexData (src/extractionOutput.h:131-134): NDC position (p/dim)*2-1, w = 1,
+ * and unit normal, w = 0. */
+typedef struct mtx_vertex {
+    float position[4];
+    float normal[4];
+} mtx_vertex;
+
+/* MeshletDescriptor (src/extractionOutput.h:124-129). */
+typedef struct mtx_meshlet {
+    uint32_t vertexOffset;
+    uint32_t indexOffset;
+    uint32_t vertexCount;
+    uint32_t primitiveCount;
+} mtx_meshlet;
+
+typedef struct mtx_extraction_output {
+    mtx_vertex* vertices;        /* device */
+    uint32_t* counts;            /* device uint32[3]: vertexCount, indexCount, meshletCount */
+    uint32_t* indices;           /* device, 3 per triangle, global vertex indices */
+    mtx_meshlet* meshlets;       /* device, one per active block (may be NULL) */
+    uint64_t* vertex_edge_ids;   /* optional: 3*linear(owner corner) + axis */
+    float* vertex_voxel_pos;     /* optional: float3 per vertex, voxel units */
+    uint8_t* case_indices;       /* optional: B^3 bytes per active block */
+    void* workspace;             /* per-block topology kept for the adjoint */
+    uint64_t workspace_bytes;
+    uint64_t vertex_capacity;    /* in vertices */
+    uint64_t index_capacity;     /* in uint32 indices */
+    uint64_t case_capacity;      /* in bytes */
+    uint32_t meshletCount;       /* host mirrors (mtx_read_extraction_counts) */
+    uint32_t vertexCount;
+    uint32_t indexCount;
+    uint32_t block_dim;
+    uint32_t max_blocks;
+    uint32_t flags;              /* options the buffers were made for */
+} mtx_extraction_output;
+
+/* ---- context --------------------------------------------------------------- */
+
+typedef struct mtx_context mtx_context;
+
+int mtx_abi_version(void);
+const char* mtx_status_string(mtx_status s);
+mtx_status mtx_context_create(int device, mtx_context** out);
+void mtx_context_destroy(mtx_context* ctx);
+const char* mtx_context_last_error(const mtx_context* ctx);
+void mtx_options_default(mtx_options* o);
+
+/* Fill a push-constant block for dims / block edge / isovalue the way the
+ * reference's main does (src/meshtrex.cpp:76-84): blockGridDim = ceil(dims/B). */
+void mtx_push_constants_init(mtx_push_constants* pc, const uint32_t dims[3], uint32_t block_dim,
+                             float isovalue);
+
+/* ---- the path ------------------------------------------------------------- */
+
+mtx_status mtx_compute_min_max(mtx_context* ctx, const mtx_volume* volume,
+                               const mtx_push_constants* pc, const mtx_options* opt, void* stream,
+                               mtx_minmax_output* out);
+void mtx_minmax_release(mtx_context* ctx, mtx_minmax_output* out);
+
+mtx_status mtx_filter_active_blocks(mtx_context* ctx, const mtx_minmax_output* mm,
+                                    const mtx_push_constants* pc, const mtx_options* opt,
+                                    void* stream, mtx_filtering_output* out);
+/* Blocking 4-byte readback of the active count into out->activeBlockCount. */
+mtx_status mtx_read_active_count(mtx_context* ctx, mtx_filtering_output* out);
+void mtx_filtering_release(mtx_context* ctx, mtx_filtering_output* out);
+
+/* Marching cubes over the active blocks.  Needs overlap-1 min-max.  Does not
+ * need a host-side active count (the count stays on the device). */
+mtx_status mtx_extract(mtx_context* ctx, const mtx_minmax_output* mm,
+                       const mtx_filtering_output* filt, const mtx_push_constants* pc,
+                       const mtx_options* opt, void* stream, mtx_extraction_output* out);
+/* Blocking readback of the three counters into the host mirrors; returns
+ * MTX_ECAPACITY if the last reuse-mode extraction overflowed its buffers and
+ * MTX_ESTATE if the active list did not match the volume. */
+mtx_status mtx_read_extraction_counts(mtx_context* ctx, mtx_extraction_output* out);
+void mtx_extraction_release(mtx_context* ctx, mtx_extraction_output* out);
+
+/* ---- device memory plumbing (for bindings without a device allocator) --- */
+
+mtx_status mtx_device_malloc(void** ptr, uint64_t bytes);
+mtx_status mtx_device_free(void* ptr);
+mtx_status mtx_memcpy_h2d(void* dst, const void* src, uint64_t bytes);
+mtx_status mtx_memcpy_d2h(void* dst, const void* src, uint64_t bytes);
+mtx_status mtx_memcpy_d2d(void* dst, const void* src, uint64_t bytes);
+mtx_status mtx_stream_synchronize(void* stream);
+mtx_status mtx_device_synchronize(void);
+uint32_t mtx_key_f32(float f);
+float mtx_unkey_f32(uint32_t k);
+/* sizeof of a public struct by name ("mtx_volume", ...), 0 if unknown: lets a
+ * binding check its mirror of the layouts. */
+uint64_t mtx_struct_size(const char* name);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MESHTREX_AMD_MTX_H */

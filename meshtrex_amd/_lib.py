@@ -1,0 +1,136 @@
+"""ctypes view of the C ABI in include/meshtrex_amd/mtx.h.
+
+Loads the in-tree libmeshtrex_amd.so built by meshtrex_amd.build; there is no
+fallback: if the library is missing, importing a function raises.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmeshtrex_amd.so")
+SYNTH_PATH = os.path.join(HERE, "libmtx_synth.so")
+
+u32 = ctypes.c_uint32
+u64 = ctypes.c_uint64
+f32 = ctypes.c_float
+vp = ctypes.c_void_p
+
+MTX_MAX_LEVELS = 24
+MTX_OK, MTX_EINVAL, MTX_ENOMEM, MTX_EDEVICE, MTX_ECAPACITY, MTX_ESTATE = range(6)
+MTX_U8, MTX_F32 = 0, 1
+MTX_RULE_CPU, MTX_RULE_GPU = 0, 1
+
+
+class PushConstants(ctypes.Structure):
+    """PushConstants of src/common.h:29-34 (64 bytes)."""
+    _fields_ = [("volumeDim", u32 * 4), ("blockDim", u32 * 4), ("blockGridDim", u32 * 4),
+                ("isovalue", f32), ("_pad", u32 * 3)]
+
+
+class CVolume(ctypes.Structure):
+    _fields_ = [("data", vp), ("dims", u32 * 3), ("dtype", u32), ("on_device", u32), ("_pad", u32)]
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("overlap", u32), ("rule", u32), ("build_pyramid", u32), ("fuse_filter", u32),
+                ("normals", u32), ("edge_ids", u32), ("case_indices", u32),
+                ("voxel_positions", u32), ("reuse", u32), ("_reserved", u32 * 7)]
+
+
+class CMinMaxOutput(ctypes.Structure):
+    _fields_ = [("minmax", vp), ("level_offset", u64 * (MTX_MAX_LEVELS + 1)),
+                ("level_extent", (u32 * 3) * MTX_MAX_LEVELS), ("num_levels", u32), ("dtype", u32),
+                ("dims", u32 * 3), ("block_dim", u32), ("grid", u32 * 3), ("overlap", u32),
+                ("volume", vp), ("owned_volume", vp), ("active_mask", vp),
+                ("mask_words_per_row", u32), ("mask_isovalue", f32), ("mask_rule", u32),
+                ("mask_valid", u32)]
+
+
+class CFilteringOutput(ctypes.Structure):
+    _fields_ = [("compacted_block_ids", vp), ("active_block_count", vp),
+                ("activeBlockCount", u32), ("capacity", u32), ("isovalue", f32), ("rule", u32)]
+
+
+class CExtractionOutput(ctypes.Structure):
+    _fields_ = [("vertices", vp), ("counts", vp), ("indices", vp), ("meshlets", vp),
+                ("vertex_edge_ids", vp), ("vertex_voxel_pos", vp), ("case_indices", vp),
+                ("workspace", vp), ("workspace_bytes", u64), ("vertex_capacity", u64),
+                ("index_capacity", u64), ("case_capacity", u64), ("meshletCount", u32),
+                ("vertexCount", u32), ("indexCount", u32), ("block_dim", u32),
+                ("max_blocks", u32), ("flags", u32)]
+
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "mtx_abi_version": (ctypes.c_int, []),
+    "mtx_status_string": (ctypes.c_char_p, [ctypes.c_int]),
+    "mtx_context_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(vp)]),
+    "mtx_context_destroy": (None, [vp]),
+    "mtx_context_last_error": (ctypes.c_char_p, [vp]),
+    "mtx_options_default": (None, [ctypes.POINTER(Options)]),
+    "mtx_push_constants_init": (None, [ctypes.POINTER(PushConstants), ctypes.POINTER(u32), u32, f32]),
+    "mtx_compute_min_max": (ctypes.c_int, [vp, ctypes.POINTER(CVolume), ctypes.POINTER(PushConstants),
+                                           ctypes.POINTER(Options), vp, ctypes.POINTER(CMinMaxOutput)]),
+    "mtx_minmax_release": (None, [vp, ctypes.POINTER(CMinMaxOutput)]),
+    "mtx_filter_active_blocks": (ctypes.c_int, [vp, ctypes.POINTER(CMinMaxOutput),
+                                                ctypes.POINTER(PushConstants), ctypes.POINTER(Options),
+                                                vp, ctypes.POINTER(CFilteringOutput)]),
+    "mtx_read_active_count": (ctypes.c_int, [vp, ctypes.POINTER(CFilteringOutput)]),
+    "mtx_filtering_release": (None, [vp, ctypes.POINTER(CFilteringOutput)]),
+    "mtx_extract": (ctypes.c_int, [vp, ctypes.POINTER(CMinMaxOutput), ctypes.POINTER(CFilteringOutput),
+                                   ctypes.POINTER(PushConstants), ctypes.POINTER(Options), vp,
+                                   ctypes.POINTER(CExtractionOutput)]),
+    "mtx_read_extraction_counts": (ctypes.c_int, [vp, ctypes.POINTER(CExtractionOutput)]),
+    "mtx_extraction_release": (None, [vp, ctypes.POINTER(CExtractionOutput)]),
+    "mtx_device_malloc": (ctypes.c_int, [ctypes.POINTER(vp), u64]),
+    "mtx_device_free": (ctypes.c_int, [vp]),
+    "mtx_memcpy_h2d": (ctypes.c_int, [vp, vp, u64]),
+    "mtx_memcpy_d2h": (ctypes.c_int, [vp, vp, u64]),
+    "mtx_memcpy_d2d": (ctypes.c_int, [vp, vp, u64]),
+    "mtx_stream_synchronize": (ctypes.c_int, [vp]),
+    "mtx_device_synchronize": (ctypes.c_int, []),
+    "mtx_key_f32": (u32, [f32]),
+    "mtx_unkey_f32": (f32, [u32]),
+    "mtx_struct_size": (u64, [ctypes.c_char_p]),
+}
+
+SYNTH_SIGS = {
+    "mtx_synth_blobs_u8": (ctypes.c_int, [vp, ctypes.POINTER(u32), u64, ctypes.c_int, vp]),
+    "mtx_synth_blobs_f32": (ctypes.c_int, [vp, ctypes.POINTER(u32), u64, ctypes.c_int, vp]),
+    "mtx_synth_vessels_u8": (ctypes.c_int, [vp, ctypes.POINTER(u32), u64, ctypes.c_int, ctypes.c_int, vp]),
+    "mtx_synth_sdf_f32": (ctypes.c_int, [vp, ctypes.POINTER(u32), u64, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_double, ctypes.c_double, vp]),
+}
+
+_lib = None
+_synth = None
+
+
+def _load(path, sigs):
+    if not os.path.exists(path):
+        raise ImportError("%s is not built; run `python -m meshtrex_amd.build`" % path)
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def lib():
+    """The extraction library (raises ImportError if it is not built)."""
+    global _lib
+    if _lib is None:
+        _lib = _load(LIB_PATH, _SIGS)
+    return _lib
+
+
+def synth():
+    global _synth
+    if _synth is None:
+        _synth = _load(SYNTH_PATH, SYNTH_SIGS)
+    return _synth
+
+
+def exported_symbols():
+    return sorted(_SIGS)

@@ -1,0 +1,639 @@
+// C-ABI host layer: argument checking, device memory, launch sequencing.
+#include "../../include/meshtrex_amd/mtx.h"
+#include "mtx_internal.h"
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+struct mtx_context {
+    int device = 0;
+    hipStream_t own = nullptr;
+    std::string last_error;
+    std::mutex mu;
+};
+
+namespace {
+
+using namespace mtx;
+
+mtx_status fail(mtx_context* ctx, mtx_status st, const char* fmt, ...)
+{
+    if (ctx) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        ctx->last_error = buf;
+    }
+    return st;
+}
+
+#define MTX_HIP(ctx, call)                                                                 \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail((ctx), e_ == hipErrorOutOfMemory ? MTX_ENOMEM : MTX_EDEVICE,       \
+                        "%s: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__); \
+    } while (0)
+
+mtx_options defaults()
+{
+    mtx_options o;
+    mtx_options_default(&o);
+    return o;
+}
+
+struct Bound {
+    mtx_context* ctx;
+    hipStream_t s;
+    bool sync;
+};
+
+mtx_status bind(mtx_context* ctx, void* stream, Bound& b)
+{
+    b.ctx = ctx;
+    b.s = stream ? static_cast<hipStream_t>(stream) : ctx->own;
+    b.sync = stream == nullptr;
+    if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, MTX_EDEVICE, "hipSetDevice failed");
+    return MTX_OK;
+}
+
+mtx_status finish(const Bound& b)
+{
+    if (b.sync) MTX_HIP(b.ctx, hipStreamSynchronize(b.s));
+    return MTX_OK;
+}
+
+ActParams make_act(int dtype, float iso, int rule)
+{
+    ActParams a{};
+    a.iso = iso;
+    a.dtype = dtype;
+    a.rule = rule;
+    float r = std::roundf(iso);
+    a.iso_r_valid = (r >= 0.0f && r < 4294967296.0f) ? 1 : 0;
+    a.iso_r = a.iso_r_valid ? (uint32_t)r : 0u;
+    return a;
+}
+
+uint64_t align256(uint64_t x) { return (x + 255) & ~255ull; }
+
+// Pyramid layout, src/minMaxManager.cpp:85-92,147-151.
+uint32_t pyramid_layout(const uint32_t G[3], uint32_t (*ext)[3], uint64_t* off)
+{
+    uint32_t m = G[0] > G[1] ? G[0] : G[1];
+    m = m > G[2] ? m : G[2];
+    uint32_t L = 1;
+    while ((m >> L) > 0) ++L;
+    uint64_t o = 0;
+    uint32_t e[3] = { G[0], G[1], G[2] };
+    for (uint32_t l = 0; l < L; ++l) {
+        ext[l][0] = e[0];
+        ext[l][1] = e[1];
+        ext[l][2] = e[2];
+        off[l] = o;
+        o += (uint64_t)e[0] * e[1] * e[2];
+        for (int a = 0; a < 3; ++a) e[a] = (e[a] + 1) >> 1;
+    }
+    off[L] = o;
+    return L;
+}
+
+// Extraction workspace layout (offsets in bytes).
+struct WsLayout {
+    uint64_t counts, vbase, tbase, table, totals, scan, bytes;
+    uint32_t scan_tiles;
+};
+
+WsLayout ws_layout(uint64_t max_blocks, uint32_t B)
+{
+    WsLayout w{};
+    uint64_t o = 0;
+    w.counts = o; o = align256(o + 8 * max_blocks);
+    w.vbase = o;  o = align256(o + 8 * max_blocks);
+    w.tbase = o;  o = align256(o + 8 * max_blocks);
+    w.table = o;  o = align256(o + corner_entry_bytes(B) * (uint64_t)B * B * B * max_blocks);
+    w.totals = o; o = align256(o + 8 * 4);
+    uint64_t tiles = (max_blocks + scan_tile_items() - 1) / scan_tile_items();
+    if (tiles == 0) tiles = 1;
+    w.scan_tiles = (uint32_t)tiles;
+    w.scan = o;   o = align256(o + 8 * (2 * tiles + 1));
+    w.bytes = o;
+    return w;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+int mtx_abi_version(void) { return MTX_ABI_VERSION; }
+
+const char* mtx_status_string(mtx_status s)
+{
+    switch (s) {
+    case MTX_OK: return "MTX_OK";
+    case MTX_EINVAL: return "MTX_EINVAL";
+    case MTX_ENOMEM: return "MTX_ENOMEM";
+    case MTX_EDEVICE: return "MTX_EDEVICE";
+    case MTX_ECAPACITY: return "MTX_ECAPACITY";
+    case MTX_ESTATE: return "MTX_ESTATE";
+    }
+    return "MTX_UNKNOWN";
+}
+
+void mtx_options_default(mtx_options* o)
+{
+    if (!o) return;
+    std::memset(o, 0, sizeof *o);
+    o->overlap = 1;
+    o->rule = MTX_RULE_CPU;
+    o->build_pyramid = 1;
+    o->fuse_filter = 1;
+    o->normals = 1;
+}
+
+void mtx_push_constants_init(mtx_push_constants* pc, const uint32_t dims[3], uint32_t block_dim,
+                             float isovalue)
+{
+    if (!pc) return;
+    std::memset(pc, 0, sizeof *pc);
+    for (int a = 0; a < 3; ++a) {
+        pc->volumeDim[a] = dims[a];
+        pc->blockDim[a] = block_dim;
+        pc->blockGridDim[a] = block_dim ? (dims[a] + block_dim - 1) / block_dim : 0;
+    }
+    pc->volumeDim[3] = 1;
+    pc->blockDim[3] = 1;
+    pc->blockGridDim[3] = 1;
+    pc->isovalue = isovalue;
+}
+
+mtx_status mtx_context_create(int device, mtx_context** out)
+{
+    if (!out) return MTX_EINVAL;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return MTX_EDEVICE;
+    if (device < 0 || device >= n) return MTX_EINVAL;
+    if (hipSetDevice(device) != hipSuccess) return MTX_EDEVICE;
+    mtx_context* c = new (std::nothrow) mtx_context;
+    if (!c) return MTX_ENOMEM;
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return MTX_EDEVICE;
+    }
+    *out = c;
+    return MTX_OK;
+}
+
+void mtx_context_destroy(mtx_context* ctx)
+{
+    if (!ctx) return;
+    hipSetDevice(ctx->device);
+    if (ctx->own) {
+        hipStreamSynchronize(ctx->own);
+        hipStreamDestroy(ctx->own);
+    }
+    delete ctx;
+}
+
+const char* mtx_context_last_error(const mtx_context* ctx)
+{
+    return ctx ? ctx->last_error.c_str() : "no context";
+}
+
+uint32_t mtx_key_f32(float f)
+{
+    uint32_t b;
+    std::memcpy(&b, &f, 4);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+float mtx_unkey_f32(uint32_t k)
+{
+    uint32_t b = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+    float f;
+    std::memcpy(&f, &b, 4);
+    return f;
+}
+
+// ---- min/max -------------------------------------------------------------------
+
+mtx_status mtx_compute_min_max(mtx_context* ctx, const mtx_volume* volume,
+                               const mtx_push_constants* pc, const mtx_options* opt, void* stream,
+                               mtx_minmax_output* out)
+{
+    if (!ctx) return MTX_EINVAL;
+    if (!volume || !pc || !out) return fail(ctx, MTX_EINVAL, "null argument");
+    if (!volume->data) return fail(ctx, MTX_EINVAL, "volume data is null");
+    const mtx_options o = opt ? *opt : defaults();
+    std::memset(out, 0, sizeof *out);
+    const uint32_t X = volume->dims[0], Y = volume->dims[1], Z = volume->dims[2];
+    const uint32_t B = pc->blockDim[0];
+    if (volume->dtype != MTX_U8 && volume->dtype != MTX_F32)
+        return fail(ctx, MTX_EINVAL, "unsupported dtype %u", volume->dtype);
+    if (X == 0 || Y == 0 || Z == 0) return fail(ctx, MTX_EINVAL, "empty volume %ux%ux%u", X, Y, Z);
+    if ((uint64_t)X * Y > 0xFFFFFFFFull || (uint64_t)X * Y * Z > (1ull << 40))
+        return fail(ctx, MTX_EINVAL, "volume %ux%ux%u too large", X, Y, Z);
+    if (B < 1 || B > 16) return fail(ctx, MTX_EINVAL, "block edge %u outside 1..16", B);
+    if (pc->blockDim[1] != B || pc->blockDim[2] != B)
+        return fail(ctx, MTX_EINVAL, "blocks must be cubic (blockDim %u,%u,%u)", B, pc->blockDim[1],
+                    pc->blockDim[2]);
+    if (pc->volumeDim[0] != X || pc->volumeDim[1] != Y || pc->volumeDim[2] != Z)
+        return fail(ctx, MTX_EINVAL, "push constants volumeDim does not match the volume");
+    if (o.overlap > 1) return fail(ctx, MTX_EINVAL, "overlap must be 0 or 1");
+    Grid g{ B, (X + B - 1) / B, (Y + B - 1) / B, (Z + B - 1) / B };
+    if (pc->blockGridDim[0] != g.gx || pc->blockGridDim[1] != g.gy || pc->blockGridDim[2] != g.gz)
+        return fail(ctx, MTX_EINVAL, "blockGridDim must be ceil(volumeDim / blockDim)");
+    if (g.nb() > 0xFFFFFFFFull) return fail(ctx, MTX_EINVAL, "too many blocks");
+
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    Bound bd;
+    if (mtx_status st = bind(ctx, stream, bd)) return st;
+
+    out->dtype = volume->dtype;
+    out->dims[0] = X; out->dims[1] = Y; out->dims[2] = Z;
+    out->block_dim = B;
+    out->grid[0] = g.gx; out->grid[1] = g.gy; out->grid[2] = g.gz;
+    out->overlap = o.overlap;
+    out->num_levels = pyramid_layout(out->grid, out->level_extent, out->level_offset);
+    if (out->num_levels > MTX_MAX_LEVELS) return fail(ctx, MTX_EINVAL, "pyramid too deep");
+    const uint64_t esz = volume->dtype == MTX_F32 ? 4 : 1;
+    const uint64_t vbytes = (uint64_t)X * Y * Z * esz;
+
+    if (volume->on_device) {
+        out->volume = volume->data;
+    } else {
+        MTX_HIP(ctx, hipMalloc(&out->owned_volume, vbytes));
+        out->volume = out->owned_volume;
+        hipError_t e = hipMemcpyAsync(out->owned_volume, volume->data, vbytes,
+                                      hipMemcpyHostToDevice, bd.s);
+        if (e != hipSuccess) {
+            mtx_minmax_release(ctx, out);
+            return fail(ctx, MTX_EDEVICE, "volume upload: %s", hipGetErrorString(e));
+        }
+    }
+    const uint64_t pairs = o.build_pyramid ? out->level_offset[out->num_levels] : g.nb();
+    if (!o.build_pyramid) out->num_levels = 1;
+    hipError_t e = hipMalloc(&out->minmax, 8 * pairs);
+    if (e != hipSuccess) {
+        mtx_minmax_release(ctx, out);
+        return fail(ctx, MTX_ENOMEM, "min/max buffer (%llu pairs)", (unsigned long long)pairs);
+    }
+    VolDesc v{ out->volume, X, Y, Z, (int)volume->dtype };
+    const bool fast = B == 8 && o.overlap == 1;
+    const bool fuse = fast && o.fuse_filter;
+    ActParams act = make_act((int)volume->dtype, pc->isovalue, (int)o.rule);
+    if (fuse) {
+        out->mask_words_per_row = (g.gx + 31) / 32;
+        uint64_t words = (uint64_t)out->mask_words_per_row * g.gy * g.gz;
+        e = hipMalloc(&out->active_mask, 4 * words);
+        if (e != hipSuccess) {
+            mtx_minmax_release(ctx, out);
+            return fail(ctx, MTX_ENOMEM, "activity mask");
+        }
+        out->mask_isovalue = pc->isovalue;
+        out->mask_rule = o.rule;
+        out->mask_valid = 1;
+    }
+    if (fast)
+        e = launch_minmax_b8(v, g, out->minmax, fuse ? out->active_mask : nullptr, &act, nullptr, 0,
+                             bd.s);
+    else
+        e = launch_minmax_generic(v, g, (int)o.overlap, out->minmax, bd.s);
+    if (e == hipSuccess && o.build_pyramid && out->num_levels > 1)
+        e = launch_pyramid(out->minmax, out->num_levels, out->level_extent, out->level_offset, bd.s);
+    if (e != hipSuccess) {
+        mtx_minmax_release(ctx, out);
+        return fail(ctx, MTX_EDEVICE, "min/max launch: %s", hipGetErrorString(e));
+    }
+    return finish(bd);
+}
+
+void mtx_minmax_release(mtx_context* ctx, mtx_minmax_output* out)
+{
+    if (!out) return;
+    if (ctx) hipSetDevice(ctx->device);
+    if (out->minmax) hipFree(out->minmax);
+    if (out->active_mask) hipFree(out->active_mask);
+    if (out->owned_volume) hipFree(out->owned_volume);
+    std::memset(out, 0, sizeof *out);
+}
+
+// ---- filter ------------------------------------------------------------------
+
+mtx_status mtx_filter_active_blocks(mtx_context* ctx, const mtx_minmax_output* mm,
+                                    const mtx_push_constants* pc, const mtx_options* opt,
+                                    void* stream, mtx_filtering_output* out)
+{
+    if (!ctx) return MTX_EINVAL;
+    if (!mm || !pc || !out) return fail(ctx, MTX_EINVAL, "null argument");
+    if (!mm->minmax) return fail(ctx, MTX_ESTATE, "min/max output is empty");
+    const mtx_options o = opt ? *opt : defaults();
+    if (o.rule > 1) return fail(ctx, MTX_EINVAL, "unknown activity rule %u", o.rule);
+    const Grid g{ mm->block_dim, mm->grid[0], mm->grid[1], mm->grid[2] };
+    if (pc->blockGridDim[0] != g.gx || pc->blockGridDim[1] != g.gy || pc->blockGridDim[2] != g.gz ||
+        pc->blockDim[0] != g.B)
+        return fail(ctx, MTX_ESTATE, "push constants do not match the min/max output");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    Bound bd;
+    if (mtx_status st = bind(ctx, stream, bd)) return st;
+
+    const bool reuse = out->compacted_block_ids && out->capacity >= g.nb();
+    if (!reuse) {
+        std::memset(out, 0, sizeof *out);
+        MTX_HIP(ctx, hipMalloc(&out->compacted_block_ids, 4 * (g.nb() ? g.nb() : 1)));
+        hipError_t e = hipMalloc(&out->active_block_count, 256 + 8ull * 65536);
+        if (e != hipSuccess) {
+            mtx_filtering_release(ctx, out);
+            return fail(ctx, MTX_ENOMEM, "filter buffers");
+        }
+        out->capacity = (uint32_t)g.nb();
+    }
+    out->activeBlockCount = 0;
+    out->isovalue = pc->isovalue;
+    out->rule = o.rule;
+    const bool use_mask = mm->active_mask && mm->mask_valid && mm->mask_rule == o.rule &&
+                          (mm->mask_isovalue == pc->isovalue ||
+                           (std::isnan(mm->mask_isovalue) && std::isnan(pc->isovalue)));
+    const uint32_t words = filter_scan_words(g.nb(), use_mask, mm->mask_words_per_row, g);
+    if (words > 65536 - 1) {
+        // scan state lives behind the count word; grow it for very large grids
+        uint32_t* big = nullptr;
+        if (hipMalloc(&big, 256 + 8ull * (words + 1)) != hipSuccess)
+            return fail(ctx, MTX_ENOMEM, "filter scan state");
+        hipFree(out->active_block_count);
+        out->active_block_count = big;
+    }
+    uint64_t* scan = reinterpret_cast<uint64_t*>(reinterpret_cast<char*>(out->active_block_count) + 256);
+    ScanState st{ scan, reinterpret_cast<uint32_t*>(scan + words - 1) };
+    hipError_t e = launch_zero64(scan, words, bd.s);
+    if (e == hipSuccess) {
+        if (use_mask)
+            e = launch_compact_mask(mm->active_mask, g, mm->mask_words_per_row,
+                                    out->compacted_block_ids, out->active_block_count, st, bd.s);
+        else
+            e = launch_filter_pairs(mm->minmax, g, make_act((int)mm->dtype, pc->isovalue, (int)o.rule),
+                                    out->compacted_block_ids, out->active_block_count, st, bd.s);
+    }
+    if (e != hipSuccess) return fail(ctx, MTX_EDEVICE, "filter launch: %s", hipGetErrorString(e));
+    return finish(bd);
+}
+
+mtx_status mtx_read_active_count(mtx_context* ctx, mtx_filtering_output* out)
+{
+    if (!ctx) return MTX_EINVAL;
+    if (!out || !out->active_block_count) return fail(ctx, MTX_ESTATE, "filtering output is empty");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    MTX_HIP(ctx, hipSetDevice(ctx->device));
+    MTX_HIP(ctx, hipDeviceSynchronize());
+    uint32_t n = 0;
+    MTX_HIP(ctx, hipMemcpy(&n, out->active_block_count, 4, hipMemcpyDeviceToHost));
+    out->activeBlockCount = n;
+    return MTX_OK;
+}
+
+void mtx_filtering_release(mtx_context* ctx, mtx_filtering_output* out)
+{
+    if (!out) return;
+    if (ctx) hipSetDevice(ctx->device);
+    if (out->compacted_block_ids) hipFree(out->compacted_block_ids);
+    if (out->active_block_count) hipFree(out->active_block_count);
+    std::memset(out, 0, sizeof *out);
+}
+
+// ---- extraction ------------------------------------------------------------------
+
+static ExtractArgs make_extract_args(const mtx_minmax_output* mm, const mtx_filtering_output* filt,
+                                     const mtx_push_constants* pc, mtx_extraction_output* out,
+                                     const WsLayout& w)
+{
+    ExtractArgs a{};
+    a.vol = VolDesc{ mm->volume, mm->dims[0], mm->dims[1], mm->dims[2], (int)mm->dtype };
+    a.grid = Grid{ mm->block_dim, mm->grid[0], mm->grid[1], mm->grid[2] };
+    a.iso = pc->isovalue;
+    a.ids = filt->compacted_block_ids;
+    a.count = filt->active_block_count;
+    char* ws = static_cast<char*>(out->workspace);
+    a.blk_counts = reinterpret_cast<uint32_t*>(ws + w.counts);
+    a.blk_vbase = reinterpret_cast<uint64_t*>(ws + w.vbase);
+    a.blk_tbase = reinterpret_cast<uint64_t*>(ws + w.tbase);
+    a.corner_table = ws + w.table;
+    a.totals = reinterpret_cast<uint64_t*>(ws + w.totals);
+    a.case_out = out->case_indices;
+    a.vertices = reinterpret_cast<float*>(out->vertices);
+    a.indices = out->indices;
+    a.edge_ids = out->vertex_edge_ids;
+    a.vox_pos = out->vertex_voxel_pos;
+    a.vcap = out->vertex_capacity;
+    a.icap = out->index_capacity;
+    a.out_counts = out->counts;
+    a.max_blocks = out->max_blocks;
+    return a;
+}
+
+mtx_status mtx_extract(mtx_context* ctx, const mtx_minmax_output* mm,
+                       const mtx_filtering_output* filt, const mtx_push_constants* pc,
+                       const mtx_options* opt, void* stream, mtx_extraction_output* out)
+{
+    if (!ctx) return MTX_EINVAL;
+    if (!mm || !filt || !pc || !out) return fail(ctx, MTX_EINVAL, "null argument");
+    if (!mm->minmax || !mm->volume) return fail(ctx, MTX_ESTATE, "min/max output is empty");
+    if (!filt->compacted_block_ids) return fail(ctx, MTX_ESTATE, "filtering output is empty");
+    if (mm->overlap != 1)
+        return fail(ctx, MTX_ESTATE, "extraction needs overlap-1 block windows");
+    if (!extract_supported_block(mm->block_dim))
+        return fail(ctx, MTX_EINVAL, "block edge %u not supported", mm->block_dim);
+    if (pc->blockDim[0] != mm->block_dim || pc->blockGridDim[0] != mm->grid[0] ||
+        pc->blockGridDim[1] != mm->grid[1] || pc->blockGridDim[2] != mm->grid[2])
+        return fail(ctx, MTX_ESTATE, "push constants do not match the min/max output");
+    if (filt->isovalue != pc->isovalue && !(std::isnan(filt->isovalue) && std::isnan(pc->isovalue)))
+        return fail(ctx, MTX_ESTATE, "filter was run for isovalue %g, extraction asks %g",
+                    (double)filt->isovalue, (double)pc->isovalue);
+    const mtx_options o = opt ? *opt : defaults();
+    const uint32_t B = mm->block_dim;
+    const uint64_t nb = (uint64_t)mm->grid[0] * mm->grid[1] * mm->grid[2];
+    const uint32_t want = (o.normals ? 1u : 0u) | (o.edge_ids ? 2u : 0u) |
+                          (o.case_indices ? 4u : 0u) | (o.voxel_positions ? 8u : 0u);
+    const WsLayout w = ws_layout(nb, B);
+
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    Bound bd;
+    if (mtx_status st = bind(ctx, stream, bd)) return st;
+
+    if (o.reuse) {
+        if (!out->workspace || out->workspace_bytes < w.bytes || out->block_dim != B ||
+            out->max_blocks < nb || (out->flags & want) != want || !out->vertices || !out->indices)
+            return fail(ctx, MTX_ESTATE, "reuse: output buffers were made for another problem");
+        ExtractArgs a = make_extract_args(mm, filt, pc, out, w);
+        a.normals = o.normals ? 1 : 0;
+        if (!o.edge_ids) a.edge_ids = nullptr;
+        if (!o.voxel_positions) a.vox_pos = nullptr;
+        if (!o.case_indices) a.case_out = nullptr;
+        if (a.case_out && out->case_capacity < nb * B * B * B) a.case_out = nullptr;
+        uint64_t* scan = reinterpret_cast<uint64_t*>(static_cast<char*>(out->workspace) + w.scan);
+        ScanState st{ scan, reinterpret_cast<uint32_t*>(scan + 2 * w.scan_tiles) };
+        hipError_t e = launch_zero64(reinterpret_cast<uint64_t*>(static_cast<char*>(out->workspace) + w.totals),
+                                     (w.scan - w.totals) / 8 + 2 * w.scan_tiles + 1, bd.s);
+        if (e == hipSuccess) e = launch_extract_count(a, bd.s);
+        if (e == hipSuccess) e = launch_extract_scan(a, st, bd.s);
+        if (e == hipSuccess) e = launch_extract_emit(a, bd.s);
+        if (e != hipSuccess) return fail(ctx, MTX_EDEVICE, "extract launch: %s", hipGetErrorString(e));
+        return finish(bd);
+    }
+
+    // Allocating form: count, scan, read the two totals, allocate exactly, emit.
+    std::memset(out, 0, sizeof *out);
+    out->block_dim = B;
+    out->max_blocks = (uint32_t)nb;
+    out->flags = want;
+    out->workspace_bytes = w.bytes;
+    hipError_t e = hipMalloc(&out->workspace, w.bytes);
+    if (e == hipSuccess) e = hipMalloc(&out->counts, 256);
+    if (e == hipSuccess && o.case_indices) {
+        out->case_capacity = nb * B * B * B;
+        e = hipMalloc(&out->case_indices, out->case_capacity ? out->case_capacity : 1);
+    }
+    if (e != hipSuccess) {
+        mtx_extraction_release(ctx, out);
+        return fail(ctx, MTX_ENOMEM, "extraction workspace (%llu B)", (unsigned long long)w.bytes);
+    }
+    ExtractArgs a = make_extract_args(mm, filt, pc, out, w);
+    a.normals = o.normals ? 1 : 0;
+    uint64_t* scan = reinterpret_cast<uint64_t*>(static_cast<char*>(out->workspace) + w.scan);
+    ScanState st{ scan, reinterpret_cast<uint32_t*>(scan + 2 * w.scan_tiles) };
+    e = launch_zero64(reinterpret_cast<uint64_t*>(static_cast<char*>(out->workspace) + w.totals),
+                      (w.scan - w.totals) / 8 + 2 * w.scan_tiles + 1, bd.s);
+    if (e == hipSuccess) e = hipMemsetAsync(out->counts, 0, 256, bd.s);
+    if (e == hipSuccess) e = launch_extract_count(a, bd.s);
+    if (e == hipSuccess) e = launch_extract_scan(a, st, bd.s);
+    uint64_t tot[2] = { 0, 0 };
+    if (e == hipSuccess) e = hipMemcpyAsync(tot, a.totals, 16, hipMemcpyDeviceToHost, bd.s);
+    if (e == hipSuccess) e = hipStreamSynchronize(bd.s);
+    if (e != hipSuccess) {
+        mtx_extraction_release(ctx, out);
+        return fail(ctx, MTX_EDEVICE, "extract count/scan: %s", hipGetErrorString(e));
+    }
+    if (tot[0] > 0xFFFFFFFFull || 3 * tot[1] > 0xFFFFFFFFull) {
+        mtx_extraction_release(ctx, out);
+        return fail(ctx, MTX_ECAPACITY, "mesh too large for 32-bit indices (%llu vertices, %llu triangles)",
+                    (unsigned long long)tot[0], (unsigned long long)tot[1]);
+    }
+    out->vertex_capacity = tot[0];
+    out->index_capacity = 3 * tot[1];
+    e = hipMalloc(&out->vertices, sizeof(mtx_vertex) * (tot[0] ? tot[0] : 1));
+    if (e == hipSuccess) e = hipMalloc(&out->indices, 4 * (tot[1] ? 3 * tot[1] : 1));
+    if (e == hipSuccess && o.edge_ids) e = hipMalloc(&out->vertex_edge_ids, 8 * (tot[0] ? tot[0] : 1));
+    if (e == hipSuccess && o.voxel_positions) e = hipMalloc(&out->vertex_voxel_pos, 12 * (tot[0] ? tot[0] : 1));
+    if (e != hipSuccess) {
+        mtx_extraction_release(ctx, out);
+        return fail(ctx, MTX_ENOMEM, "extraction outputs (%llu vertices, %llu triangles)",
+                    (unsigned long long)tot[0], (unsigned long long)tot[1]);
+    }
+    a = make_extract_args(mm, filt, pc, out, w);
+    a.normals = o.normals ? 1 : 0;
+    e = launch_extract_emit(a, bd.s);
+    if (e != hipSuccess) return fail(ctx, MTX_EDEVICE, "extract emit: %s", hipGetErrorString(e));
+    out->vertexCount = (uint32_t)tot[0];
+    out->indexCount = (uint32_t)(3 * tot[1]);
+    return finish(bd);
+}
+
+mtx_status mtx_read_extraction_counts(mtx_context* ctx, mtx_extraction_output* out)
+{
+    if (!ctx) return MTX_EINVAL;
+    if (!out || !out->counts || !out->workspace) return fail(ctx, MTX_ESTATE, "extraction output is empty");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    MTX_HIP(ctx, hipSetDevice(ctx->device));
+    MTX_HIP(ctx, hipDeviceSynchronize());
+    uint32_t c[3] = { 0, 0, 0 };
+    MTX_HIP(ctx, hipMemcpy(c, out->counts, 12, hipMemcpyDeviceToHost));
+    const WsLayout w = ws_layout(out->max_blocks, out->block_dim);
+    uint64_t tot[3] = { 0, 0, 0 };
+    MTX_HIP(ctx, hipMemcpy(tot, static_cast<char*>(out->workspace) + w.totals, 24, hipMemcpyDeviceToHost));
+    out->vertexCount = c[0];
+    out->indexCount = c[1];
+    out->meshletCount = c[2];
+    if (tot[2] & 2) return fail(ctx, MTX_ESTATE, "active block list does not match the volume");
+    if (tot[2] & 1) return fail(ctx, MTX_ECAPACITY, "output buffers too small: need %llu vertices, %llu indices",
+                                (unsigned long long)tot[0], (unsigned long long)(3 * tot[1]));
+    return MTX_OK;
+}
+
+void mtx_extraction_release(mtx_context* ctx, mtx_extraction_output* out)
+{
+    if (!out) return;
+    if (ctx) hipSetDevice(ctx->device);
+    void* ptrs[] = { out->vertices, out->counts, out->indices, out->meshlets, out->vertex_edge_ids,
+                     out->vertex_voxel_pos, out->case_indices, out->workspace };
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    std::memset(out, 0, sizeof *out);
+}
+
+// ---- plumbing ---------------------------------------------------------------
+
+mtx_status mtx_device_malloc(void** ptr, uint64_t bytes)
+{
+    if (!ptr) return MTX_EINVAL;
+    *ptr = nullptr;
+    hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);
+    return e == hipSuccess ? MTX_OK : (e == hipErrorOutOfMemory ? MTX_ENOMEM : MTX_EDEVICE);
+}
+
+mtx_status mtx_device_free(void* ptr)
+{
+    return hipFree(ptr) == hipSuccess ? MTX_OK : MTX_EDEVICE;
+}
+
+mtx_status mtx_memcpy_h2d(void* dst, const void* src, uint64_t bytes)
+{
+    return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? MTX_OK : MTX_EDEVICE;
+}
+
+mtx_status mtx_memcpy_d2h(void* dst, const void* src, uint64_t bytes)
+{
+    return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? MTX_OK : MTX_EDEVICE;
+}
+
+mtx_status mtx_memcpy_d2d(void* dst, const void* src, uint64_t bytes)
+{
+    return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice) == hipSuccess ? MTX_OK : MTX_EDEVICE;
+}
+
+mtx_status mtx_stream_synchronize(void* stream)
+{
+    return hipStreamSynchronize(static_cast<hipStream_t>(stream)) == hipSuccess ? MTX_OK : MTX_EDEVICE;
+}
+
+uint64_t mtx_struct_size(const char* name)
+{
+    if (!name) return 0;
+#define MTX_SZ(T) \
+    if (std::strcmp(name, #T) == 0) return sizeof(T);
+    MTX_SZ(mtx_push_constants)
+    MTX_SZ(mtx_volume)
+    MTX_SZ(mtx_options)
+    MTX_SZ(mtx_minmax_output)
+    MTX_SZ(mtx_filtering_output)
+    MTX_SZ(mtx_vertex)
+    MTX_SZ(mtx_meshlet)
+    MTX_SZ(mtx_extraction_output)
+#undef MTX_SZ
+    return 0;
+}
+
+mtx_status mtx_device_synchronize(void)
+{
+    return hipDeviceSynchronize() == hipSuccess ? MTX_OK : MTX_EDEVICE;
+}
+
+}  // extern "C"

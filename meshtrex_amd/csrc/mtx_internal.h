@@ -1,0 +1,120 @@
+// Internal interface between the C-ABI host layer (mtx_api.hip) and the
+// kernel launchers (mtx_kernels.hip).  Not installed; not part of the ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mtx {
+
+enum Dtype : int { kU8 = 0, kF32 = 1 };
+enum Rule : int { kRuleCpu = 0, kRuleGpu = 1 };
+
+// Activity test parameters, prepared once on the host.
+struct ActParams {
+    float iso;        // isovalue
+    uint32_t iso_r;   // round(iso) for the u8 CPU rule
+    int iso_r_valid;  // round(iso) representable as uint32
+    int dtype;
+    int rule;
+};
+
+struct VolDesc {
+    const void* data;
+    uint32_t X, Y, Z;
+    int dtype;
+};
+
+struct Grid {
+    uint32_t B;
+    uint32_t gx, gy, gz;
+    uint64_t nb() const { return (uint64_t)gx * gy * gz; }
+};
+
+// Lookback scan state: one 64-bit status word per tile plus a ticket counter.
+struct ScanState {
+    uint64_t* status;   // zeroed before use
+    uint32_t* ticket;   // zeroed before use
+};
+
+// ---- launchers (all asynchronous on `s`) ----------------------------------
+
+// Block min/max, B=8, overlap 1, coalesced wave64 kernel.  Optionally writes
+// the activity bit mask (one 32-bit word per 32 x-blocks) for `act`.
+// zero/zero_words: a buffer the kernel clears on the way (scan state of the
+// following compaction), may be null.
+hipError_t launch_minmax_b8(const VolDesc& v, const Grid& g, uint32_t* pairs, uint32_t* mask,
+                            const ActParams* act, uint64_t* zero, uint32_t zero_words,
+                            hipStream_t s);
+
+// Block min/max for any B, overlap 0 or 1 (one wave per block).
+hipError_t launch_minmax_generic(const VolDesc& v, const Grid& g, int overlap, uint32_t* pairs,
+                                 hipStream_t s);
+
+// Pyramid levels 1..L-1 from level 0.  ext/off describe every level.
+hipError_t launch_pyramid(uint32_t* pairs, uint32_t levels, const uint32_t (*ext)[3],
+                          const uint64_t* off, hipStream_t s);
+
+// Number of 64-bit words a compaction's scan state needs for nb blocks.
+uint32_t filter_scan_words(uint64_t nb, bool from_mask, uint32_t words_per_row, const Grid& g);
+
+// Ordered compaction of active block IDs from the fused bit mask.
+hipError_t launch_compact_mask(const uint32_t* mask, const Grid& g, uint32_t words_per_row,
+                               uint32_t* ids, uint32_t* count, ScanState st, hipStream_t s);
+
+// Ordered compaction of active block IDs from leaf min/max pairs.
+hipError_t launch_filter_pairs(const uint32_t* pairs, const Grid& g, const ActParams& act,
+                               uint32_t* ids, uint32_t* count, ScanState st, hipStream_t s);
+
+// Clear n 64-bit words (a small fill kernel, safe under graph capture).
+hipError_t launch_zero64(uint64_t* p, uint64_t n, hipStream_t s);
+
+// ---- extraction --------------------------------------------------------------
+
+struct ExtractArgs {
+    VolDesc vol;
+    Grid grid;
+    float iso;
+    const uint32_t* ids;        // active block IDs (ascending)
+    const uint32_t* count;      // device active count
+    uint32_t* blk_counts;       // per active block: {nv, nt}
+    void* corner_table;         // per active block: B^3 entries (prefix<<3 | flags)
+    uint64_t* blk_vbase;        // per active block vertex base
+    uint64_t* blk_tbase;        // per active block triangle base
+    uint64_t* totals;           // [0]=vertices [1]=triangles [2]=overflow flag
+    uint8_t* case_out;          // optional: B^3 case bytes per active block
+    // outputs
+    float* vertices;            // 8 floats per vertex (position xyzw, normal xyzw)
+    uint32_t* indices;          // 3 per triangle
+    uint64_t* edge_ids;         // optional
+    float* vox_pos;             // optional: 3 floats per vertex, voxel units
+    uint64_t vcap, icap;        // capacities (vertices, indices)
+    uint32_t* out_counts;       // optional device uint32[3]: vertexCount, indexCount, meshletCount
+    int normals;
+    uint32_t max_blocks;        // capacity of the per-block arrays
+};
+
+bool extract_supported_block(uint32_t B);
+size_t corner_entry_bytes(uint32_t B);
+uint32_t scan_tile_items();
+
+hipError_t launch_extract_count(const ExtractArgs& a, hipStream_t s);
+hipError_t launch_extract_scan(const ExtractArgs& a, ScanState st, hipStream_t s);
+hipError_t launch_extract_emit(const ExtractArgs& a, hipStream_t s);
+
+// Adjoint: dL/dvoxel (dense, overwritten) and dL/diso (one float, overwritten)
+// from dL/d(vertex position in voxel units), topology of the forward pass.
+struct AdjointArgs {
+    ExtractArgs fwd;            // forward topology (ids, count, corner tables, bases)
+    const int32_t* blk_to_active; // dense map block -> active index or -1
+    const float* dl_dpos;       // 3 per vertex
+    float* dl_dvol;             // one per voxel
+    float* dl_diso;             // one float
+    double* iso_partials;       // scratch: one double per adjoint workgroup
+    uint32_t n_wg;
+};
+hipError_t launch_block_to_active(const uint32_t* ids, const uint32_t* count, const Grid& g,
+                                  int32_t* map, hipStream_t s);
+hipError_t launch_adjoint(const AdjointArgs& a, hipStream_t s);
+
+}  // namespace mtx

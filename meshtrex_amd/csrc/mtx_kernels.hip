@@ -1,0 +1,1122 @@
+// gfx950 kernels of the isosurface-extraction path: block min/max, active-block
+// compaction, and PMB marching cubes (count, scan, emit).  Written for CDNA4:
+// wave64, LDS tiles, XCD-aware work order; no matrix cores (nothing here is a
+// dense contraction).
+#include "mtx_internal.h"
+#include "mc_cases.h"
+
+#include <type_traits>
+
+namespace mtx {
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t vkey(float f)
+{
+    // order-preserving map of IEEE-754 binary32 onto uint32
+    uint32_t b = __float_as_uint(f);
+    uint32_t m = (uint32_t)((int32_t)b >> 31) | 0x80000000u;
+    return b ^ m;
+}
+__device__ __forceinline__ uint32_t vkey(uint8_t v) { return v; }
+
+__device__ __forceinline__ float unkey(uint32_t k)
+{
+    uint32_t b = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+    return __uint_as_float(b);
+}
+
+__device__ __forceinline__ float vfloat(float f) { return f; }
+__device__ __forceinline__ float vfloat(uint8_t v) { return (float)v; }
+
+// Activity of a block from its key pair (CPU rule of src/blockFilteringTestUtils.h:456-468,
+// or the live shader's rule of src/shaders/occupiedBlockPrefixSum.comp.glsl:84).
+__device__ __forceinline__ bool block_active(uint32_t lo, uint32_t hi, const ActParams& a)
+{
+    float flo = a.dtype == kU8 ? (float)lo : unkey(lo);
+    float fhi = a.dtype == kU8 ? (float)hi : unkey(hi);
+    if (a.rule == kRuleGpu) return a.iso >= flo && a.iso <= fhi;
+    if (lo == hi) return false;
+    if (a.dtype == kU8) return a.iso_r_valid && a.iso_r >= lo && a.iso_r <= hi;
+    return a.iso >= flo && a.iso <= fhi;
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
+
+template <typename V>
+__device__ __forceinline__ V wave_incl_scan(V v)
+{
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        V t = __shfl_up(v, d, 64);
+        if (lane >= (uint32_t)d) v += t;
+    }
+    return v;
+}
+
+template <typename V>
+__device__ __forceinline__ V wave_sum(V v)
+{
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+// XCD-aware remap: the dispatcher deals consecutive workgroup IDs round-robin
+// over the 8 XCDs; give each XCD a contiguous range of logical work so that
+// neighbouring tiles share one L2.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t wg, uint32_t nwg)
+{
+    if (nwg % 8u) return wg;
+    return (wg % 8u) * (nwg / 8u) + wg / 8u;
+}
+
+// ---------------------------------------------------------------------------
+// decoupled look-back (single-pass scan across workgroups)
+// status word: bits 63:62 = 1 aggregate ready, 2 inclusive prefix ready
+// ---------------------------------------------------------------------------
+
+constexpr uint64_t kFlagAgg = 1ull << 62;
+constexpr uint64_t kFlagInc = 2ull << 62;
+constexpr uint64_t kValMask = (1ull << 62) - 1;
+
+__device__ __forceinline__ uint64_t ld_acq(const uint64_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_rel(uint64_t* p, uint64_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Called by one whole wave; returns the exclusive prefix of `tile` (same in all lanes).
+__device__ uint64_t lookback(uint64_t* status, uint32_t tile, uint64_t aggregate)
+{
+    const uint32_t lane = lane_id();
+    if (tile == 0) {
+        if (lane == 0) st_rel(&status[0], kFlagInc | aggregate);
+        return 0;
+    }
+    if (lane == 0) st_rel(&status[tile], kFlagAgg | aggregate);
+    uint64_t excl = 0;
+    int64_t end = (int64_t)tile - 1;
+    for (;;) {
+        int64_t idx = end - (int64_t)lane;
+        uint64_t s = idx >= 0 ? ld_acq(&status[idx]) : kFlagInc;
+        uint64_t flag = s >> 62;
+        if (__any(flag == 0)) {
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        uint64_t inc = __ballot(flag == 2);
+        uint64_t v = s & kValMask;
+        if (inc) {
+            uint32_t first = __builtin_ctzll(inc);
+            excl += wave_sum<uint64_t>(lane <= first ? v : 0ull);
+            break;
+        }
+        excl += wave_sum<uint64_t>(v);
+        end -= 64;
+    }
+    if (lane == 0) st_rel(&status[tile], kFlagInc | (excl + aggregate));
+    return excl;
+}
+
+__global__ void k_zero64(uint64_t* __restrict__ p, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+hipError_t launch_zero64(uint64_t* p, uint64_t n, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    uint64_t g = (n + 255) / 256;
+    if (g > 1024) g = 1024;
+    hipLaunchKernelGGL(k_zero64, dim3((uint32_t)g), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// block min/max, B = 8, overlap 1: the roofline kernel
+//
+// A workgroup (4 waves) owns 32 x-blocks x 4 y-blocks x zc z-blocks.  Lane l of
+// wave w reads 4 consecutive voxels (16 B for f32, 4 B for u8) of every row of
+// y-block 4*gy+w, so a wave issues one fully coalesced row segment per load.
+// Lanes 2k, 2k+1 cover x-block k; its overlap column x = 8k+8 is lane 2k+2's
+// first voxel (the last block of the tile takes it from one wave-uniform
+// load).  The z sweep carries the shared plane 8bz+8 from block bz into bz+1,
+// so every plane is read once; the y overlap row is the next wave's first row
+// and hits L2.  Out-of-window rows/planes are clamped onto the last in-window
+// row/plane: re-reading it cannot change a min/max, and it keeps every load
+// unconditional so all 9 rows of a plane are in flight together.
+// ---------------------------------------------------------------------------
+
+struct MMAcc {
+    uint32_t amin, amax;  // all 4 voxels of this lane
+    uint32_t emin, emax;  // first voxel of this lane (overlap column for lane-2)
+    uint32_t xmin, xmax;  // wave-uniform extra column
+};
+
+__device__ __forceinline__ void mm_neutral(MMAcc& a)
+{
+    a.amin = a.emin = a.xmin = 0xFFFFFFFFu;
+    a.amax = a.emax = a.xmax = 0u;
+}
+
+__device__ __forceinline__ void mm_merge(MMAcc& a, const MMAcc& b)
+{
+    a.amin = min(a.amin, b.amin); a.amax = max(a.amax, b.amax);
+    a.emin = min(a.emin, b.emin); a.emax = max(a.emax, b.emax);
+    a.xmin = min(a.xmin, b.xmin); a.xmax = max(a.xmax, b.xmax);
+}
+
+template <typename T> struct Vec4;
+template <> struct Vec4<float> { using type = float4; };
+template <> struct Vec4<uint8_t> { using type = uchar4; };
+
+template <typename T, bool ALIGNED>
+__device__ __forceinline__ void mm_plane(MMAcc& acc, const T* __restrict__ vol, uint64_t zoff,
+                                         uint32_t X, uint32_t y0, uint32_t ylast, uint32_t x0,
+                                         bool lane_ok, uint32_t xe, bool xe_ok)
+{
+    using V4 = typename Vec4<T>::type;
+    uint32_t k[9][4];
+    uint32_t kx[9];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        uint32_t y = min(y0 + (uint32_t)r, ylast);
+        const T* row = vol + zoff + (uint64_t)y * X;
+        if (ALIGNED) {
+            V4 v = *reinterpret_cast<const V4*>(row + (lane_ok ? x0 : 0u));
+            k[r][0] = vkey(v.x); k[r][1] = vkey(v.y); k[r][2] = vkey(v.z); k[r][3] = vkey(v.w);
+            if (!lane_ok) { k[r][0] = k[r][1] = k[r][2] = k[r][3] = 0xFFFFFFFFu; }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                bool ok = x0 + j < X;
+                k[r][j] = vkey(row[ok ? x0 + j : 0u]);
+                if (!ok) k[r][j] = 0xFFFFFFFFu;
+            }
+        }
+        kx[r] = vkey(row[xe_ok ? xe : 0u]);
+    }
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        // neutral (all-ones) entries must not raise the max: mask them out
+        bool ok0 = ALIGNED ? lane_ok : (x0 < X);
+        bool ok1 = ALIGNED ? lane_ok : (x0 + 1 < X);
+        bool ok2 = ALIGNED ? lane_ok : (x0 + 2 < X);
+        bool ok3 = ALIGNED ? lane_ok : (x0 + 3 < X);
+        acc.amin = min(acc.amin, min(min(k[r][0], k[r][1]), min(k[r][2], k[r][3])));
+        uint32_t m0 = ok0 ? k[r][0] : 0u, m1 = ok1 ? k[r][1] : 0u;
+        uint32_t m2 = ok2 ? k[r][2] : 0u, m3 = ok3 ? k[r][3] : 0u;
+        acc.amax = max(acc.amax, max(max(m0, m1), max(m2, m3)));
+        acc.emin = min(acc.emin, k[r][0]);
+        acc.emax = max(acc.emax, m0);
+        if (xe_ok) {
+            acc.xmin = min(acc.xmin, kx[r]);
+            acc.xmax = max(acc.xmax, kx[r]);
+        }
+    }
+}
+
+template <typename T, bool ALIGNED, bool MASK>
+__global__ __launch_bounds__(256) void k_minmax_b8(
+    const T* __restrict__ vol, uint32_t X, uint32_t Y, uint32_t Z, uint32_t gx, uint32_t gy,
+    uint32_t gz, uint32_t ntx, uint32_t ngy, uint32_t zc, uint2* __restrict__ pairs,
+    uint32_t* __restrict__ mask, uint32_t wpr, ActParams act, uint64_t* __restrict__ zero,
+    uint32_t zero_words)
+{
+    if (zero) {
+        for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < zero_words; i += gridDim.x * 256u)
+            zero[i] = 0;
+    }
+    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t tgy = tile % ngy;
+    const uint32_t rest = tile / ngy;
+    const uint32_t tx = rest % ntx;
+    const uint32_t cz = rest / ntx;
+    const uint32_t lane = lane_id();
+    const uint32_t w = threadIdx.x >> 6;
+    const uint32_t by = tgy * 4u + w;
+    if (by >= gy) return;
+    const uint32_t bz0 = cz * zc;
+    if (bz0 >= gz) return;
+    const uint32_t bz1 = min(bz0 + zc, gz);
+
+    const uint32_t x0 = tx * 256u + lane * 4u;
+    const bool lane_ok = x0 < X;
+    const uint32_t xe = tx * 256u + 256u;
+    const bool xe_ok = xe < X;
+    const uint32_t y0 = by * 8u;
+    const uint32_t ylast = min(y0 + 8u, Y - 1u);
+    const uint64_t XY = (uint64_t)X * Y;
+
+    MMAcc first;
+    mm_neutral(first);
+    mm_plane<T, ALIGNED>(first, vol, (uint64_t)(bz0 * 8u) * XY, X, y0, ylast, x0, lane_ok, xe,
+                         xe_ok);
+
+    const uint32_t bx = tx * 32u + (lane >> 1);
+    for (uint32_t bz = bz0; bz < bz1; ++bz) {
+        MMAcc acc = first;
+        const uint32_t zl = min(bz * 8u + 8u, Z - 1u);
+#pragma unroll 2
+        for (uint32_t dz = 1; dz <= 8u; ++dz) {
+            uint32_t z = min(bz * 8u + dz, zl);
+            MMAcc p;
+            mm_neutral(p);
+            mm_plane<T, ALIGNED>(p, vol, (uint64_t)z * XY, X, y0, ylast, x0, lane_ok, xe, xe_ok);
+            mm_merge(acc, p);
+            if (dz == 8u) first = p;
+        }
+        // x combine: pair (2k, 2k+1) plus lane 2k+2's first voxel
+        uint32_t bmin = min(acc.amin, __shfl_xor(acc.amin, 1, 64));
+        uint32_t bmax = max(acc.amax, __shfl_xor(acc.amax, 1, 64));
+        uint32_t src = min(lane + 2u, 63u);
+        uint32_t nmin = __shfl(acc.emin, (int)src, 64);
+        uint32_t nmax = __shfl(acc.emax, (int)src, 64);
+        if (lane >= 62u) { nmin = acc.xmin; nmax = acc.xmax; }
+        bmin = min(bmin, nmin);
+        bmax = max(bmax, nmax);
+        const bool writer = !(lane & 1u) && bx < gx;
+        const uint64_t id = bx + (uint64_t)gx * (by + (uint64_t)gy * bz);
+        if (writer) pairs[id] = make_uint2(bmin, bmax);
+        if (MASK) {
+            uint64_t bal = __ballot(writer && block_active(bmin, bmax, act));
+            // compress the even bits (one per x-block) into 32 bits
+            uint64_t x = bal & 0x5555555555555555ull;
+            x = (x | (x >> 1)) & 0x3333333333333333ull;
+            x = (x | (x >> 2)) & 0x0F0F0F0F0F0F0F0Full;
+            x = (x | (x >> 4)) & 0x00FF00FF00FF00FFull;
+            x = (x | (x >> 8)) & 0x0000FFFF0000FFFFull;
+            x = (x | (x >> 16)) & 0x00000000FFFFFFFFull;
+            if (lane == 0) mask[((uint64_t)bz * gy + by) * wpr + tx] = (uint32_t)x;
+        }
+    }
+}
+
+template <typename T, bool MASK>
+static hipError_t launch_mm_b8_t(const VolDesc& v, const Grid& g, uint32_t* pairs, uint32_t* mask,
+                                 const ActParams* act, uint64_t* zero, uint32_t zero_words,
+                                 hipStream_t s)
+{
+    const uint32_t ntx = (g.gx + 31u) / 32u;
+    const uint32_t ngy = (g.gy + 3u) / 4u;
+    const uint64_t cols = (uint64_t)ntx * ngy;
+    // z-chunk: as long as possible while keeping >= 1024 workgroups (4 per CU)
+    uint64_t zc = (cols * g.gz) / 1024u;
+    if (zc < 1) zc = 1;
+    if (zc > g.gz) zc = g.gz;
+    const uint32_t ncz = (uint32_t)((g.gz + zc - 1) / zc);
+    const uint64_t nwg = cols * ncz;
+    if (nwg > 0x7FFFFFFFull) return hipErrorInvalidValue;
+    const uint32_t wpr = (g.gx + 31u) / 32u;
+    const bool aligned = (v.X % 4u) == 0 && (((uintptr_t)v.data) % (4 * sizeof(T))) == 0;
+    ActParams a{};
+    if (act) a = *act;
+    const T* p = static_cast<const T*>(v.data);
+    uint2* pr = reinterpret_cast<uint2*>(pairs);
+    if (aligned)
+        hipLaunchKernelGGL((k_minmax_b8<T, true, MASK>), dim3((uint32_t)nwg), dim3(256), 0, s, p,
+                           v.X, v.Y, v.Z, g.gx, g.gy, g.gz, ntx, ngy, (uint32_t)zc, pr, mask, wpr,
+                           a, zero, zero_words);
+    else
+        hipLaunchKernelGGL((k_minmax_b8<T, false, MASK>), dim3((uint32_t)nwg), dim3(256), 0, s, p,
+                           v.X, v.Y, v.Z, g.gx, g.gy, g.gz, ntx, ngy, (uint32_t)zc, pr, mask, wpr,
+                           a, zero, zero_words);
+    return hipGetLastError();
+}
+
+hipError_t launch_minmax_b8(const VolDesc& v, const Grid& g, uint32_t* pairs, uint32_t* mask,
+                            const ActParams* act, uint64_t* zero, uint32_t zero_words,
+                            hipStream_t s)
+{
+    const bool m = mask != nullptr;
+    if (v.dtype == kF32)
+        return m ? launch_mm_b8_t<float, true>(v, g, pairs, mask, act, zero, zero_words, s)
+                 : launch_mm_b8_t<float, false>(v, g, pairs, mask, act, zero, zero_words, s);
+    return m ? launch_mm_b8_t<uint8_t, true>(v, g, pairs, mask, act, zero, zero_words, s)
+             : launch_mm_b8_t<uint8_t, false>(v, g, pairs, mask, act, zero, zero_words, s);
+}
+
+// ---------------------------------------------------------------------------
+// block min/max, any B, overlap 0 or 1: one wave per block
+// ---------------------------------------------------------------------------
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_minmax_generic(const T* __restrict__ vol, uint32_t X,
+                                                        uint32_t Y, uint32_t Z, uint32_t B,
+                                                        uint32_t gx, uint32_t gy, uint64_t nb,
+                                                        uint32_t overlap, uint2* __restrict__ pairs)
+{
+    const uint64_t blk = (uint64_t)blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (blk >= nb) return;
+    const uint32_t lane = lane_id();
+    const uint32_t bx = (uint32_t)(blk % gx);
+    const uint32_t by = (uint32_t)((blk / gx) % gy);
+    const uint32_t bz = (uint32_t)(blk / ((uint64_t)gx * gy));
+    const uint32_t span = overlap ? B : B - 1u;
+    const uint32_t xa = bx * B, ya = by * B, za = bz * B;
+    const uint32_t xb = min(xa + span, X - 1u), yb = min(ya + span, Y - 1u), zb = min(za + span, Z - 1u);
+    const uint32_t wx = xb - xa + 1u, wy = yb - ya + 1u, wz = zb - za + 1u;
+    const uint32_t n = wx * wy * wz;
+    uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+    for (uint32_t i = lane; i < n; i += 64u) {
+        uint32_t x = xa + i % wx;
+        uint32_t y = ya + (i / wx) % wy;
+        uint32_t z = za + i / (wx * wy);
+        uint32_t k = vkey(vol[x + (uint64_t)X * (y + (uint64_t)Y * z)]);
+        lo = min(lo, k);
+        hi = max(hi, k);
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        lo = min(lo, (uint32_t)__shfl_xor(lo, d, 64));
+        hi = max(hi, (uint32_t)__shfl_xor(hi, d, 64));
+    }
+    if (lane == 0) pairs[blk] = make_uint2(lo, hi);
+}
+
+hipError_t launch_minmax_generic(const VolDesc& v, const Grid& g, int overlap, uint32_t* pairs,
+                                 hipStream_t s)
+{
+    const uint64_t nb = g.nb();
+    const uint64_t nwg = (nb + 3) / 4;
+    if (nwg > 0x7FFFFFFFull) return hipErrorInvalidValue;
+    uint2* pr = reinterpret_cast<uint2*>(pairs);
+    if (v.dtype == kF32)
+        hipLaunchKernelGGL(k_minmax_generic<float>, dim3((uint32_t)nwg), dim3(256), 0, s,
+                           static_cast<const float*>(v.data), v.X, v.Y, v.Z, g.B, g.gx, g.gy, nb,
+                           (uint32_t)overlap, pr);
+    else
+        hipLaunchKernelGGL(k_minmax_generic<uint8_t>, dim3((uint32_t)nwg), dim3(256), 0, s,
+                           static_cast<const uint8_t*>(v.data), v.X, v.Y, v.Z, g.B, g.gx, g.gy,
+                           nb, (uint32_t)overlap, pr);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// min/max pyramid: each workgroup reduces an 8x8x8 tile of one level into up
+// to three coarser levels (4^3, 2^3, 1) through LDS
+// ---------------------------------------------------------------------------
+
+struct PyrLevels {
+    uint64_t off[4];
+    uint32_t ext[4][3];
+    uint32_t n;  // number of destination levels (1..3)
+};
+
+__global__ __launch_bounds__(512) void k_pyramid(uint2* __restrict__ pairs, PyrLevels L)
+{
+    __shared__ uint2 s[512];
+    const uint32_t t = threadIdx.x;
+    const uint32_t lx = t & 7u, ly = (t >> 3) & 7u, lz = t >> 6;
+    const uint32_t ntx = (L.ext[0][0] + 7u) / 8u, nty = (L.ext[0][1] + 7u) / 8u;
+    const uint32_t tx = blockIdx.x % ntx, ty = (blockIdx.x / ntx) % nty, tz = blockIdx.x / (ntx * nty);
+    {
+        uint32_t x = tx * 8u + lx, y = ty * 8u + ly, z = tz * 8u + lz;
+        uint2 v = make_uint2(0xFFFFFFFFu, 0u);
+        if (x < L.ext[0][0] && y < L.ext[0][1] && z < L.ext[0][2])
+            v = pairs[L.off[0] + x + (uint64_t)L.ext[0][0] * (y + (uint64_t)L.ext[0][1] * z)];
+        s[t] = v;
+    }
+    for (uint32_t l = 1; l <= L.n; ++l) {
+        const uint32_t size = 8u >> l;
+        __syncthreads();
+        uint2 r = make_uint2(0xFFFFFFFFu, 0u);
+        const bool act = lx < size && ly < size && lz < size;
+        if (act) {
+#pragma unroll
+            for (uint32_t c = 0; c < 8u; ++c) {
+                uint32_t cx = 2u * lx + (c & 1u), cy = 2u * ly + ((c >> 1) & 1u), cz = 2u * lz + (c >> 2);
+                uint2 v = s[cx + 8u * (cy + 8u * cz)];
+                r.x = min(r.x, v.x);
+                r.y = max(r.y, v.y);
+            }
+        }
+        __syncthreads();
+        if (act) {
+            s[lx + 8u * (ly + 8u * lz)] = r;
+            uint32_t x = ((tx * 8u) >> l) + lx, y = ((ty * 8u) >> l) + ly, z = ((tz * 8u) >> l) + lz;
+            if (x < L.ext[l][0] && y < L.ext[l][1] && z < L.ext[l][2])
+                pairs[L.off[l] + x + (uint64_t)L.ext[l][0] * (y + (uint64_t)L.ext[l][1] * z)] = r;
+        }
+    }
+}
+
+hipError_t launch_pyramid(uint32_t* pairs, uint32_t levels, const uint32_t (*ext)[3],
+                          const uint64_t* off, hipStream_t s)
+{
+    uint2* pr = reinterpret_cast<uint2*>(pairs);
+    for (uint32_t src = 0; src + 1 < levels; src += 3) {
+        PyrLevels L{};
+        L.n = min(3u, levels - 1 - src);
+        for (uint32_t j = 0; j <= L.n; ++j) {
+            L.off[j] = off[src + j];
+            L.ext[j][0] = ext[src + j][0];
+            L.ext[j][1] = ext[src + j][1];
+            L.ext[j][2] = ext[src + j][2];
+        }
+        uint64_t nwg = (uint64_t)((L.ext[0][0] + 7) / 8) * ((L.ext[0][1] + 7) / 8) * ((L.ext[0][2] + 7) / 8);
+        hipLaunchKernelGGL(k_pyramid, dim3((uint32_t)nwg), dim3(512), 0, s, pr, L);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// ---------------------------------------------------------------------------
+// ordered compaction of active block IDs
+// ---------------------------------------------------------------------------
+
+constexpr uint32_t kFilterItems = 8;  // blocks per thread (pairs path)
+
+uint32_t filter_scan_words(uint64_t nb, bool from_mask, uint32_t words_per_row, const Grid& g)
+{
+    uint64_t tiles;
+    if (from_mask) {
+        uint64_t words = (uint64_t)g.gy * g.gz * words_per_row;
+        tiles = (words + 255) / 256;
+    } else {
+        tiles = (nb + 256 * kFilterItems - 1) / (256 * kFilterItems);
+    }
+    if (tiles == 0) tiles = 1;
+    return (uint32_t)tiles + 1;  // +1: the ticket counter lives in the last word
+}
+
+// From leaf pairs: 8 blocks per thread, in block-ID order (k, wave, lane).
+__global__ __launch_bounds__(256) void k_filter_pairs(const uint2* __restrict__ pairs, uint64_t nb,
+                                                      ActParams act, uint32_t* __restrict__ ids,
+                                                      uint32_t* __restrict__ count,
+                                                      uint64_t* status, uint32_t* ticket,
+                                                      uint32_t ntiles)
+{
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_cnt[kFilterItems * 4];
+    __shared__ uint64_t s_excl;
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint64_t base = (uint64_t)tile * 256u * kFilterItems;
+    uint64_t bal[kFilterItems];
+#pragma unroll
+    for (uint32_t k = 0; k < kFilterItems; ++k) {
+        uint64_t b = base + k * 256u + threadIdx.x;
+        bool a = false;
+        if (b < nb) {
+            uint2 p = pairs[b];
+            a = block_active(p.x, p.y, act);
+        }
+        bal[k] = __ballot(a);
+        if (lane == 0) s_cnt[k * 4 + w] = (uint32_t)__popcll(bal[k]);
+    }
+    __syncthreads();
+    if (w == 0) {
+        uint32_t v = lane < kFilterItems * 4 ? s_cnt[lane] : 0u;
+        uint32_t inc = wave_incl_scan<uint32_t>(v);
+        uint32_t agg = __shfl(inc, 63, 64);
+        if (lane < kFilterItems * 4) s_cnt[lane] = inc - v;
+        uint64_t ex = lookback(status, tile, agg);
+        if (lane == 0) {
+            s_excl = ex;
+            if (tile == ntiles - 1) *count = (uint32_t)(ex + agg);
+        }
+    }
+    __syncthreads();
+    const uint64_t lt = (1ull << lane) - 1ull;
+#pragma unroll
+    for (uint32_t k = 0; k < kFilterItems; ++k) {
+        if ((bal[k] >> lane) & 1ull) {
+            uint64_t pos = s_excl + s_cnt[k * 4 + w] + (uint32_t)__popcll(bal[k] & lt);
+            ids[pos] = (uint32_t)(base + k * 256u + threadIdx.x);
+        }
+    }
+}
+
+hipError_t launch_filter_pairs(const uint32_t* pairs, const Grid& g, const ActParams& act,
+                               uint32_t* ids, uint32_t* count, ScanState st, hipStream_t s)
+{
+    const uint64_t nb = g.nb();
+    uint64_t ntiles = (nb + 256 * kFilterItems - 1) / (256 * kFilterItems);
+    if (ntiles == 0) ntiles = 1;
+    hipLaunchKernelGGL(k_filter_pairs, dim3((uint32_t)ntiles), dim3(256), 0, s,
+                       reinterpret_cast<const uint2*>(pairs), nb, act, ids, count, st.status,
+                       st.ticket, (uint32_t)ntiles);
+    return hipGetLastError();
+}
+
+// From the fused bit mask: one 32-bit word (32 x-blocks of one row) per thread.
+__global__ __launch_bounds__(256) void k_compact_mask(const uint32_t* __restrict__ mask,
+                                                      uint64_t nwords, uint32_t gx, uint32_t wpr,
+                                                      uint32_t* __restrict__ ids,
+                                                      uint32_t* __restrict__ count,
+                                                      uint64_t* status, uint32_t* ticket,
+                                                      uint32_t ntiles)
+{
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_w[4];
+    __shared__ uint64_t s_excl;
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint64_t wi = (uint64_t)tile * 256u + threadIdx.x;
+    const uint32_t word = wi < nwords ? mask[wi] : 0u;
+    const uint32_t c = (uint32_t)__popc(word);
+    const uint32_t inc = wave_incl_scan<uint32_t>(c);
+    if (lane == 63) s_w[w] = inc;
+    __syncthreads();
+    uint32_t woff = 0, agg = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < 4; ++j) {
+        woff += j < w ? s_w[j] : 0u;
+        agg += s_w[j];
+    }
+    if (w == 0) {
+        uint64_t ex = lookback(status, tile, agg);
+        if (lane == 0) {
+            s_excl = ex;
+            if (tile == ntiles - 1) *count = (uint32_t)(ex + agg);
+        }
+    }
+    __syncthreads();
+    uint64_t pos = s_excl + woff + inc - c;
+    const uint64_t row = wi / wpr;
+    const uint32_t base = (uint32_t)(row * gx + (wi % wpr) * 32u);
+    uint32_t m = word;
+    while (m) {
+        uint32_t bit = __builtin_ctz(m);
+        m &= m - 1u;
+        ids[pos++] = base + bit;
+    }
+}
+
+hipError_t launch_compact_mask(const uint32_t* mask, const Grid& g, uint32_t words_per_row,
+                               uint32_t* ids, uint32_t* count, ScanState st, hipStream_t s)
+{
+    const uint64_t nwords = (uint64_t)g.gy * g.gz * words_per_row;
+    uint64_t ntiles = (nwords + 255) / 256;
+    if (ntiles == 0) ntiles = 1;
+    hipLaunchKernelGGL(k_compact_mask, dim3((uint32_t)ntiles), dim3(256), 0, s, mask, nwords, g.gx,
+                       words_per_row, ids, count, st.status, st.ticket, (uint32_t)ntiles);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// extraction
+//
+// Vertices are numbered block by block (ascending active block ID), within a
+// block by owner corner in x-fastest order, then axis x, y, z; each active
+// edge gets exactly one vertex, owned by the corner at its lower end.  Per
+// active block the count pass stores one entry per corner:
+// (vertex prefix within the block << 3) | (3 edge flags), which lets the emit
+// pass resolve edges owned by the +x/+y/+z neighbour blocks without
+// recounting them.
+// ---------------------------------------------------------------------------
+
+template <int B> struct EB {
+    static constexpr int NC = B * B * B;
+    static constexpr int CPT = (NC + 255) / 256;
+    using Entry = typename std::conditional<(B <= 13), uint16_t, uint32_t>::type;
+};
+
+bool extract_supported_block(uint32_t B) { return B >= 1 && B <= 16; }
+size_t corner_entry_bytes(uint32_t B) { return B <= 13 ? 2 : 4; }
+
+__device__ __forceinline__ uint32_t item_index(uint32_t k, uint32_t na)
+{
+    // XCD-aware: XCD x walks a contiguous eighth of the active list, so blocks
+    // that are x-neighbours (consecutive IDs) share an L2
+    const uint32_t nwg = gridDim.x;
+    if (nwg % 8u) return blockIdx.x + k * nwg;
+    const uint32_t per = nwg / 8u;
+    const uint32_t chunk = (na + 7u) / 8u;
+    const uint32_t xcd = blockIdx.x % 8u, slot = blockIdx.x / 8u;
+    const uint32_t j = slot + k * per;
+    return j < chunk ? xcd * chunk + j : 0xFFFFFFFFu;
+}
+
+__device__ __forceinline__ bool item_done(uint32_t k, uint32_t na)
+{
+    const uint32_t nwg = gridDim.x;
+    if (nwg % 8u) return blockIdx.x + (uint64_t)k * nwg >= na;
+    const uint32_t per = nwg / 8u;
+    const uint32_t chunk = (na + 7u) / 8u;
+    return blockIdx.x / 8u + (uint64_t)k * per >= chunk;
+}
+
+// Workgroup exclusive scan of two counters; returns exclusive values and totals.
+__device__ __forceinline__ void wg_scan2(uint32_t a, uint32_t b, uint32_t& ea, uint32_t& eb,
+                                         uint32_t& ta, uint32_t& tb, uint32_t* s_scan /*[8]*/)
+{
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    uint32_t ia = wave_incl_scan<uint32_t>(a);
+    uint32_t ib = wave_incl_scan<uint32_t>(b);
+    if (lane == 63) {
+        s_scan[w] = ia;
+        s_scan[4 + w] = ib;
+    }
+    __syncthreads();
+    uint32_t oa = 0, ob = 0, sa = 0, sb = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < 4; ++j) {
+        uint32_t va = s_scan[j], vb = s_scan[4 + j];
+        oa += j < w ? va : 0u;
+        ob += j < w ? vb : 0u;
+        sa += va;
+        sb += vb;
+    }
+    ea = oa + ia - a;
+    eb = ob + ib - b;
+    ta = sa;
+    tb = sb;
+}
+
+template <int B, typename T>
+__global__ __launch_bounds__(256) void k_extract_count(ExtractArgs a)
+{
+    constexpr int E = B + 1;
+    constexpr int NTILE = E * E * E;
+    constexpr int NC = EB<B>::NC;
+    constexpr int CPT = EB<B>::CPT;
+    using Entry = typename EB<B>::Entry;
+    __shared__ uint8_t s_below[NTILE];
+    __shared__ uint32_t s_scan[8];
+    const T* __restrict__ vol = static_cast<const T*>(a.vol.data);
+    const uint32_t X = a.vol.X, Y = a.vol.Y, Z = a.vol.Z;
+    const uint32_t na = *a.count;
+    // a volume thinner than 2 voxels on any axis has no cells, hence no triangles
+    // and no vertices (every vertex is a corner of some triangle)
+    const bool degen = X < 2 || Y < 2 || Z < 2;
+    Entry* __restrict__ table = static_cast<Entry*>(a.corner_table);
+    for (uint32_t k = 0; !item_done(k, na); ++k) {
+        const uint32_t i = item_index(k, na);
+        if (i >= na) break;
+        const uint32_t id = a.ids[i];
+        const uint32_t bx = id % a.grid.gx, by = (id / a.grid.gx) % a.grid.gy;
+        const uint32_t bz = id / (a.grid.gx * a.grid.gy);
+        const uint32_t ox = bx * B, oy = by * B, oz = bz * B;
+        for (uint32_t j = threadIdx.x; j < (uint32_t)NTILE; j += 256u) {
+            uint32_t lx = j % E, ly = (j / E) % E, lz = j / (E * E);
+            uint32_t x = ox + lx, y = oy + ly, z = oz + lz;
+            bool in = x < X && y < Y && z < Z;
+            T v = vol[in ? x + (uint64_t)X * (y + (uint64_t)Y * z) : 0];
+            s_below[j] = (uint8_t)(in && vfloat(v) <= a.iso);
+        }
+        __syncthreads();
+        uint32_t fl[CPT];
+        uint32_t nv = 0, nt = 0;
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+            const uint32_t c = threadIdx.x * CPT + j;
+            fl[j] = 0;
+            if (c >= (uint32_t)NC) continue;
+            const uint32_t lx = c % B, ly = (c / B) % B, lz = c / (B * B);
+            const uint32_t x = ox + lx, y = oy + ly, z = oz + lz;
+            const uint32_t t0 = lx + E * (ly + E * lz);
+            uint32_t cs = 0;
+            if (!degen && x < X && y < Y && z < Z) {
+                const uint32_t b0 = s_below[t0];
+                uint32_t f = 0;
+                if (x + 1 < X) f |= (b0 ^ s_below[t0 + 1]);
+                if (y + 1 < Y) f |= (b0 ^ s_below[t0 + E]) << 1;
+                if (z + 1 < Z) f |= (b0 ^ s_below[t0 + E * E]) << 2;
+                fl[j] = f;
+                nv += __popc(f);
+                if (x + 1 < X && y + 1 < Y && z + 1 < Z) {
+#pragma unroll
+                    for (uint32_t q = 0; q < 8; ++q)
+                        cs |= (uint32_t)s_below[t0 + mtx_corner_x(q) + E * (mtx_corner_y(q) + E * mtx_corner_z(q))] << q;
+                    nt += mtx_word_ntri(mtx_tri_words[cs]);
+                }
+            }
+            if (a.case_out) a.case_out[(uint64_t)i * NC + c] = (uint8_t)cs;
+        }
+        uint32_t ev, et, tv, tt;
+        wg_scan2(nv, nt, ev, et, tv, tt, s_scan);
+        uint32_t run = ev;
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+            const uint32_t c = threadIdx.x * CPT + j;
+            if (c < (uint32_t)NC) table[(uint64_t)i * NC + c] = (Entry)((run << 3) | fl[j]);
+            run += __popc(fl[j]);
+        }
+        if (threadIdx.x == 0) {
+            a.blk_counts[2 * (uint64_t)i] = tv;
+            a.blk_counts[2 * (uint64_t)i + 1] = tt;
+        }
+        __syncthreads();
+    }
+}
+
+// Exclusive scan of per-block {nv, nt} into vertex/triangle bases.  Scan state:
+// 2 * tiles status words (vertices, then triangles) plus a ticket counter.
+constexpr uint32_t kScanItems = 4;
+uint32_t scan_tile_items() { return 256 * kScanItems; }
+
+__global__ __launch_bounds__(256) void k_extract_scan(const uint32_t* __restrict__ cnt,
+                                                      const uint32_t* __restrict__ count,
+                                                      uint64_t* __restrict__ vbase,
+                                                      uint64_t* __restrict__ tbase,
+                                                      uint64_t* __restrict__ totals,
+                                                      uint32_t* __restrict__ out_counts,
+                                                      uint64_t* status, uint32_t tcap,
+                                                      uint32_t* ticket)
+{
+    __shared__ uint32_t s_tile;
+    __shared__ uint64_t s_w[8];
+    __shared__ uint64_t s_ex[2];
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    const uint32_t na = *count;
+    uint32_t ntiles = (na + 256 * kScanItems - 1) / (256 * kScanItems);
+    if (ntiles == 0) ntiles = 1;
+    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    if (tile >= ntiles) return;
+    uint64_t v[kScanItems], t[kScanItems];
+    uint64_t sv = 0, st = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kScanItems; ++k) {
+        uint64_t i = (uint64_t)tile * 256 * kScanItems + threadIdx.x * kScanItems + k;
+        v[k] = i < na ? cnt[2 * i] : 0;
+        t[k] = i < na ? cnt[2 * i + 1] : 0;
+        sv += v[k];
+        st += t[k];
+    }
+    uint64_t iv = wave_incl_scan<uint64_t>(sv);
+    uint64_t it = wave_incl_scan<uint64_t>(st);
+    if (lane == 63) {
+        s_w[w] = iv;
+        s_w[4 + w] = it;
+    }
+    __syncthreads();
+    uint64_t ov = 0, ot = 0, av = 0, at = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < 4; ++j) {
+        ov += j < w ? s_w[j] : 0;
+        ot += j < w ? s_w[4 + j] : 0;
+        av += s_w[j];
+        at += s_w[4 + j];
+    }
+    if (w == 0) {
+        // two independent look-backs: vertex counts in status[0, tcap), triangles in [tcap, 2 tcap)
+        uint64_t exv = lookback(status, tile, av);
+        uint64_t ext = lookback(status + tcap, tile, at);
+        if (lane == 0) {
+            s_ex[0] = exv;
+            s_ex[1] = ext;
+            if (tile == ntiles - 1) {
+                totals[0] = exv + av;
+                totals[1] = ext + at;
+                if (out_counts) {
+                    uint64_t V = exv + av, I = 3 * (ext + at);
+                    out_counts[0] = V > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)V;
+                    out_counts[1] = I > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)I;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    uint64_t rv = s_ex[0] + ov + iv - sv;
+    uint64_t rt = s_ex[1] + ot + it - st;
+#pragma unroll
+    for (uint32_t k = 0; k < kScanItems; ++k) {
+        uint64_t i = (uint64_t)tile * 256 * kScanItems + threadIdx.x * kScanItems + k;
+        if (i < na) {
+            vbase[i] = rv;
+            tbase[i] = rt;
+        }
+        rv += v[k];
+        rt += t[k];
+    }
+}
+
+// Lower-bound search of an ascending ID list.
+__device__ __forceinline__ int32_t find_active(const uint32_t* __restrict__ ids, uint32_t na,
+                                               uint32_t key)
+{
+    uint32_t lo = 0, hi = na;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (ids[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo < na && ids[lo] == key) ? (int32_t)lo : -1;
+}
+
+__device__ __forceinline__ void grad_normal(const float* s, uint32_t t, uint32_t E, float n[3])
+{
+    float gx = s[t - 1] - s[t + 1];
+    float gy = s[t - E] - s[t + E];
+    float gz = s[t - E * E] - s[t + E * E];
+    float l = sqrtf(gx * gx + gy * gy + gz * gz);
+    if (l < 1e-5f) { n[0] = 0.f; n[1] = 1.f; n[2] = 0.f; return; }
+    n[0] = gx / l; n[1] = gy / l; n[2] = gz / l;
+}
+
+template <int B, typename T, bool NORMALS>
+__global__ __launch_bounds__(256) void k_extract_emit(ExtractArgs a)
+{
+    constexpr int A = NORMALS ? 1 : 0;  // apron below/above for central differences
+    constexpr int E = B + 1 + 2 * A;
+    constexpr int NTILE = E * E * E;
+    constexpr int NC = EB<B>::NC;
+    constexpr int CPT = EB<B>::CPT;
+    using Entry = typename EB<B>::Entry;
+    __shared__ float s_val[NTILE];
+    __shared__ Entry s_tab[NC];
+    __shared__ int32_t s_nbr[8];
+    __shared__ uint32_t s_scan[8];
+    const T* __restrict__ vol = static_cast<const T*>(a.vol.data);
+    const uint32_t X = a.vol.X, Y = a.vol.Y, Z = a.vol.Z;
+    const float iso = a.iso;
+    const uint32_t na = *a.count;
+    const Entry* __restrict__ table = static_cast<const Entry*>(a.corner_table);
+    const float inv2x = 2.0f / (float)X, inv2y = 2.0f / (float)Y, inv2z = 2.0f / (float)Z;
+    const bool degen = X < 2 || Y < 2 || Z < 2;
+    for (uint32_t k = 0; !item_done(k, na); ++k) {
+        const uint32_t i = item_index(k, na);
+        if (i >= na) break;
+        const uint32_t id = a.ids[i];
+        const uint32_t bx = id % a.grid.gx, by = (id / a.grid.gx) % a.grid.gy;
+        const uint32_t bz = id / (a.grid.gx * a.grid.gy);
+        const uint32_t ox = bx * B, oy = by * B, oz = bz * B;
+        const uint64_t vb = a.blk_vbase[i], tb = a.blk_tbase[i];
+        const uint32_t bnv = a.blk_counts[2 * (uint64_t)i], bnt = a.blk_counts[2 * (uint64_t)i + 1];
+        const bool fits = vb + bnv <= a.vcap && 3 * (tb + bnt) <= a.icap;
+        if (threadIdx.x < 8) {
+            const uint32_t n = threadIdx.x;
+            int32_t r = -1;
+            const uint32_t nx = bx + (n & 1u), ny = by + ((n >> 1) & 1u), nz = bz + (n >> 2);
+            if (n != 0 && n != 7 && nx < a.grid.gx && ny < a.grid.gy && nz < a.grid.gz)
+                r = find_active(a.ids, na, nx + a.grid.gx * (ny + a.grid.gy * nz));
+            s_nbr[n] = r;
+        }
+        for (uint32_t j = threadIdx.x; j < (uint32_t)NTILE; j += 256u) {
+            int32_t x = (int32_t)(ox + j % E) - A, y = (int32_t)(oy + (j / E) % E) - A;
+            int32_t z = (int32_t)(oz + j / (E * E)) - A;
+            x = min(max(x, 0), (int32_t)X - 1);
+            y = min(max(y, 0), (int32_t)Y - 1);
+            z = min(max(z, 0), (int32_t)Z - 1);
+            s_val[j] = vfloat(vol[(uint32_t)x + (uint64_t)X * ((uint32_t)y + (uint64_t)Y * (uint32_t)z)]);
+        }
+        __syncthreads();
+        uint32_t fl[CPT], cs[CPT];
+        uint32_t nv = 0, nt = 0;
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+            const uint32_t c = threadIdx.x * CPT + j;
+            fl[j] = 0;
+            cs[j] = 0xFFFFFFFFu;
+            if (c >= (uint32_t)NC) continue;
+            const uint32_t lx = c % B, ly = (c / B) % B, lz = c / (B * B);
+            const uint32_t x = ox + lx, y = oy + ly, z = oz + lz;
+            if (degen || !(x < X && y < Y && z < Z)) continue;
+            const uint32_t t0 = (lx + A) + E * ((ly + A) + E * (lz + A));
+            const uint32_t b0 = s_val[t0] <= iso;
+            uint32_t f = 0;
+            if (x + 1 < X) f |= b0 ^ (uint32_t)(s_val[t0 + 1] <= iso);
+            if (y + 1 < Y) f |= (b0 ^ (uint32_t)(s_val[t0 + E] <= iso)) << 1;
+            if (z + 1 < Z) f |= (b0 ^ (uint32_t)(s_val[t0 + E * E] <= iso)) << 2;
+            fl[j] = f;
+            nv += __popc(f);
+            if (x + 1 < X && y + 1 < Y && z + 1 < Z) {
+                uint32_t q8 = 0;
+#pragma unroll
+                for (uint32_t q = 0; q < 8; ++q)
+                    q8 |= (uint32_t)(s_val[t0 + mtx_corner_x(q) + E * (mtx_corner_y(q) + E * mtx_corner_z(q))] <= iso) << q;
+                cs[j] = q8;
+                nt += mtx_word_ntri(mtx_tri_words[q8]);
+            }
+        }
+        uint32_t ev, et, tv, tt;
+        wg_scan2(nv, nt, ev, et, tv, tt, s_scan);
+        {
+            uint32_t run = ev;
+#pragma unroll
+            for (int j = 0; j < CPT; ++j) {
+                const uint32_t c = threadIdx.x * CPT + j;
+                if (c < (uint32_t)NC) s_tab[c] = (Entry)((run << 3) | fl[j]);
+                run += __popc(fl[j]);
+            }
+        }
+        __syncthreads();
+        if (fits) {
+            uint64_t rv = vb + ev, rt = tb + et;
+#pragma unroll
+            for (int j = 0; j < CPT; ++j) {
+                const uint32_t c = threadIdx.x * CPT + j;
+                if (c >= (uint32_t)NC) continue;
+                const uint32_t lx = c % B, ly = (c / B) % B, lz = c / (B * B);
+                const uint32_t x = ox + lx, y = oy + ly, z = oz + lz;
+                const uint32_t t0 = (lx + A) + E * ((ly + A) + E * (lz + A));
+                const uint32_t f = fl[j];
+                if (f) {
+                    float n0[3] = { 0.f, 0.f, 0.f };
+                    if (NORMALS) grad_normal(s_val, t0, E, n0);
+                    const float v0 = s_val[t0];
+#pragma unroll
+                    for (uint32_t ax = 0; ax < 3; ++ax) {
+                        if (!((f >> ax) & 1u)) continue;
+                        const uint32_t st = ax == 0 ? 1u : (ax == 1 ? (uint32_t)E : (uint32_t)(E * E));
+                        const float v1 = s_val[t0 + st];
+                        float t = (iso - v0) / (v1 - v0);
+                        t = fminf(fmaxf(t, 0.0f), 1.0f);
+                        const float px = (float)x + (ax == 0 ? t : 0.0f);
+                        const float py = (float)y + (ax == 1 ? t : 0.0f);
+                        const float pz = (float)z + (ax == 2 ? t : 0.0f);
+                        float4 pos = make_float4(px * inv2x - 1.0f, py * inv2y - 1.0f, pz * inv2z - 1.0f, 1.0f);
+                        float4 nrm = make_float4(0.f, 0.f, 0.f, 0.f);
+                        if (NORMALS) {
+                            float n1[3];
+                            grad_normal(s_val, t0 + st, E, n1);
+                            float m0 = n0[0] * (1.0f - t) + n1[0] * t;
+                            float m1 = n0[1] * (1.0f - t) + n1[1] * t;
+                            float m2 = n0[2] * (1.0f - t) + n1[2] * t;
+                            float l = sqrtf(m0 * m0 + m1 * m1 + m2 * m2);
+                            if (l < 1e-5f) { m0 = n0[0]; m1 = n0[1]; m2 = n0[2]; l = sqrtf(m0 * m0 + m1 * m1 + m2 * m2); }
+                            nrm = make_float4(m0 / l, m1 / l, m2 / l, 0.0f);
+                        }
+                        float4* dst = reinterpret_cast<float4*>(a.vertices) + 2 * rv;
+                        dst[0] = pos;
+                        dst[1] = nrm;
+                        if (a.edge_ids) a.edge_ids[rv] = 3ull * (x + (uint64_t)X * (y + (uint64_t)Y * z)) + ax;
+                        if (a.vox_pos) {
+                            a.vox_pos[3 * rv + 0] = px;
+                            a.vox_pos[3 * rv + 1] = py;
+                            a.vox_pos[3 * rv + 2] = pz;
+                        }
+                        ++rv;
+                    }
+                }
+                if (cs[j] != 0xFFFFFFFFu) {
+                    const uint64_t wd = mtx_tri_words[cs[j]];
+                    const uint32_t ntri = mtx_word_ntri(wd);
+                    for (uint32_t q = 0; q < ntri; ++q) {
+                        uint32_t idx3[3];
+#pragma unroll
+                        for (uint32_t e3 = 0; e3 < 3; ++e3) {
+                            const uint32_t o = mtx_edge_owner(mtx_word_edge(wd, q, e3));
+                            const uint32_t qx = lx + (o & 1u), qy = ly + ((o >> 1) & 1u), qz = lz + ((o >> 2) & 1u);
+                            const uint32_t ax = o >> 3;
+                            uint32_t ent;
+                            uint64_t base;
+                            if (qx < (uint32_t)B && qy < (uint32_t)B && qz < (uint32_t)B) {
+                                ent = s_tab[qx + B * (qy + B * qz)];
+                                base = vb;
+                            } else {
+                                const uint32_t nx = qx >= (uint32_t)B, ny = qy >= (uint32_t)B, nz = qz >= (uint32_t)B;
+                                const int32_t jn = s_nbr[nx | (ny << 1) | (nz << 2)];
+                                if (jn < 0) {
+                                    // an active edge whose owner block is not in the active
+                                    // list: the list is inconsistent with this volume/isovalue
+                                    atomicOr(reinterpret_cast<unsigned long long*>(&a.totals[2]), 2ull);
+                                    ent = 0;
+                                    base = 0;
+                                } else {
+                                    const uint32_t cx = qx - B * nx, cy = qy - B * ny, cz = qz - B * nz;
+                                    ent = table[(uint64_t)jn * NC + cx + B * (cy + B * cz)];
+                                    base = a.blk_vbase[jn];
+                                }
+                            }
+                            idx3[e3] = (uint32_t)(base + (ent >> 3) + __popc(ent & 7u & ((1u << ax) - 1u)));
+                        }
+                        uint32_t* dst = a.indices + 3 * rt;
+                        dst[0] = idx3[0];
+                        dst[1] = idx3[1];
+                        dst[2] = idx3[2];
+                        ++rt;
+                    }
+                }
+            }
+        } else if (threadIdx.x == 0) {
+            atomicOr(reinterpret_cast<unsigned long long*>(&a.totals[2]), 1ull);
+        }
+        __syncthreads();
+    }
+}
+
+template <int B>
+static hipError_t launch_count_b(const ExtractArgs& a, hipStream_t s, uint32_t nwg)
+{
+    if (a.vol.dtype == kF32)
+        hipLaunchKernelGGL((k_extract_count<B, float>), dim3(nwg), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_extract_count<B, uint8_t>), dim3(nwg), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int B>
+static hipError_t launch_emit_b(const ExtractArgs& a, hipStream_t s, uint32_t nwg)
+{
+    if (a.vol.dtype == kF32) {
+        if (a.normals) hipLaunchKernelGGL((k_extract_emit<B, float, true>), dim3(nwg), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((k_extract_emit<B, float, false>), dim3(nwg), dim3(256), 0, s, a);
+    } else {
+        if (a.normals) hipLaunchKernelGGL((k_extract_emit<B, uint8_t, true>), dim3(nwg), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((k_extract_emit<B, uint8_t, false>), dim3(nwg), dim3(256), 0, s, a);
+    }
+    return hipGetLastError();
+}
+
+// Persistent grid: enough workgroups to fill the chip, never more than blocks.
+static uint32_t persistent_wgs(uint32_t max_blocks)
+{
+    uint32_t n = 2048;
+    if (max_blocks < n) n = max_blocks < 8 ? 8 : ((max_blocks + 7) / 8) * 8;
+    return n;
+}
+
+#define MTX_DISPATCH_B(FN, ...)                                   \
+    switch (a.grid.B) {                                           \
+    case 1: return FN<1>(__VA_ARGS__);                            \
+    case 2: return FN<2>(__VA_ARGS__);                            \
+    case 3: return FN<3>(__VA_ARGS__);                            \
+    case 4: return FN<4>(__VA_ARGS__);                            \
+    case 5: return FN<5>(__VA_ARGS__);                            \
+    case 6: return FN<6>(__VA_ARGS__);                            \
+    case 7: return FN<7>(__VA_ARGS__);                            \
+    case 8: return FN<8>(__VA_ARGS__);                            \
+    case 9: return FN<9>(__VA_ARGS__);                            \
+    case 10: return FN<10>(__VA_ARGS__);                          \
+    case 11: return FN<11>(__VA_ARGS__);                          \
+    case 12: return FN<12>(__VA_ARGS__);                          \
+    case 13: return FN<13>(__VA_ARGS__);                          \
+    case 14: return FN<14>(__VA_ARGS__);                          \
+    case 15: return FN<15>(__VA_ARGS__);                          \
+    case 16: return FN<16>(__VA_ARGS__);                          \
+    default: return hipErrorInvalidValue;                         \
+    }
+
+hipError_t launch_extract_count(const ExtractArgs& a, hipStream_t s)
+{
+    const uint32_t nwg = persistent_wgs(a.max_blocks);
+    MTX_DISPATCH_B(launch_count_b, a, s, nwg)
+}
+
+hipError_t launch_extract_emit(const ExtractArgs& a, hipStream_t s)
+{
+    const uint32_t nwg = persistent_wgs(a.max_blocks);
+    MTX_DISPATCH_B(launch_emit_b, a, s, nwg)
+}
+
+hipError_t launch_extract_scan(const ExtractArgs& a, ScanState st, hipStream_t s)
+{
+    uint64_t ntiles = ((uint64_t)a.max_blocks + 256 * kScanItems - 1) / (256 * kScanItems);
+    if (ntiles == 0) ntiles = 1;
+    if (ntiles > 0x7FFFFFFFull) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_extract_scan, dim3((uint32_t)ntiles), dim3(256), 0, s, a.blk_counts,
+                       a.count, a.blk_vbase, a.blk_tbase, a.totals, a.out_counts, st.status,
+                       (uint32_t)ntiles, st.ticket);
+    return hipGetLastError();
+}
+
+}  // namespace mtx

@@ -1,0 +1,347 @@
+"""Python host API with the reference's names, over the C ABI.
+
+    computeMinMaxMip(context, volume, pushConstants, ...)      src/minMaxManager.h:20-21
+    filterActiveBlocks(context, minMaxOutput, pushConstants)   src/filteringManager.h:9-10
+    readActiveBlockCount(context, filteringOutput)             src/filteringManager.h:13
+    extractMeshletDescriptors(context, mm, filt, pushConstants) src/extractionManager.h:10-11
+
+Outputs own device memory (freed by cleanup() or on garbage collection) and
+expose the reference's field names (activeBlockCount, vertexCount,
+indexCount, meshletCount).  Every non-OK status raises RuntimeError, as the
+reference's managers throw std::runtime_error (src/extractionManager.cpp:276).
+Readback helpers return numpy arrays.
+"""
+import ctypes
+import re
+
+import numpy as np
+
+from . import _lib as L
+
+_DT = {"uint8": L.MTX_U8, "float32": L.MTX_F32}
+
+
+class MtxError(RuntimeError):
+    def __init__(self, status, message):
+        self.status = status
+        super().__init__("%s: %s" % (L.lib().mtx_status_string(status).decode(), message))
+
+
+def _check(ctx, st):
+    if st != L.MTX_OK:
+        msg = L.lib().mtx_context_last_error(ctx.handle if ctx else None)
+        raise MtxError(st, msg.decode() if msg else "")
+
+
+class Context:
+    """One GPU (the reference's VulkanContext, src/vulkan_context.h:7-45)."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        st = L.lib().mtx_context_create(device, ctypes.byref(h))
+        if st != L.MTX_OK:
+            raise MtxError(st, "cannot create a context on device %d" % device)
+        self.handle = h
+        self.device = device
+
+    def close(self):
+        if self.handle:
+            L.lib().mtx_context_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class DeviceBuffer:
+    """Raw device allocation with host copies (no torch needed)."""
+
+    def __init__(self, nbytes):
+        p = ctypes.c_void_p()
+        st = L.lib().mtx_device_malloc(ctypes.byref(p), nbytes)
+        if st != L.MTX_OK:
+            raise MtxError(st, "device allocation of %d bytes" % nbytes)
+        self.ptr = p.value
+        self.nbytes = nbytes
+
+    @classmethod
+    def from_numpy(cls, a):
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        if a.nbytes:
+            _ok(L.lib().mtx_memcpy_h2d(b.ptr, a.ctypes.data, a.nbytes))
+        return b
+
+    def to_numpy(self, dtype, count=None, offset_bytes=0):
+        dtype = np.dtype(dtype)
+        if count is None:
+            count = (self.nbytes - offset_bytes) // dtype.itemsize
+        return read_device(self.ptr + offset_bytes, dtype, count)
+
+    def free(self):
+        if self.ptr:
+            L.lib().mtx_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _ok(st):
+    if st != L.MTX_OK:
+        raise MtxError(st, "device copy")
+
+
+def read_device(ptr, dtype, count):
+    out = np.empty(int(count), dtype=dtype)
+    if count and ptr:
+        _ok(L.lib().mtx_memcpy_d2h(out.ctypes.data, ptr, out.nbytes))
+    return out
+
+
+class Volume:
+    """Volume of src/volume.h:8-12: dims, type name and data (host numpy or DeviceBuffer)."""
+
+    def __init__(self, volume_dims, volume_type, volume_data):
+        self.volume_dims = tuple(int(d) for d in volume_dims)
+        if volume_type not in _DT:
+            raise ValueError("unsupported volume type %r (uint8 or float32)" % volume_type)
+        self.volume_type = volume_type
+        self.volume_data = volume_data
+
+    def _c(self):
+        v = L.CVolume()
+        v.dims[:] = self.volume_dims
+        v.dtype = _DT[self.volume_type]
+        if isinstance(self.volume_data, DeviceBuffer):
+            v.data = self.volume_data.ptr
+            v.on_device = 1
+        elif isinstance(self.volume_data, int):
+            v.data = self.volume_data
+            v.on_device = 1
+        else:
+            a = np.ascontiguousarray(self.volume_data)
+            want = np.dtype(self.volume_type)
+            if a.dtype != want:
+                raise ValueError("volume data is %s, type says %s" % (a.dtype, want))
+            if a.size != int(np.prod(self.volume_dims, dtype=np.int64)):
+                raise ValueError("volume data has %d elements for dims %s" % (a.size, self.volume_dims))
+            self._host = a
+            v.data = a.ctypes.data
+            v.on_device = 0
+        return v
+
+
+_NAME_RE = re.compile(r"(\w+)_(\d+)x(\d+)x(\d+)_(.+)\.raw")
+
+
+def loadVolume(path):
+    """Raw-file loader with the reference's naming contract (src/volume.h:14-42):
+    '<name>_<X>x<Y>x<Z>_<type>.raw'.  Types uint8 and float32 are read; the
+    reference reads 2 bytes per voxel for any non-uint8 type (src/volume.h:30),
+    which has no consumer on its path (its upload is R8 only), so other types
+    are rejected here."""
+    m = _NAME_RE.search(str(path))
+    if not m:
+        raise RuntimeError("Invalaid raw file naming scheme: expected "
+                           "'<name>_<X>x<Y>x<Z>_<data type>.raw' but %r did not match" % str(path))
+    dims = (int(m.group(2)), int(m.group(3)), int(m.group(4)))
+    vtype = m.group(5)
+    if vtype not in _DT:
+        raise RuntimeError("unsupported raw volume type %r" % vtype)
+    n = dims[0] * dims[1] * dims[2]
+    data = np.fromfile(path, dtype=np.dtype(vtype), count=n)
+    if data.size != n:
+        raise RuntimeError("Failed to read volume data: %d of %d voxels" % (data.size, n))
+    return Volume(dims, vtype, data)
+
+
+def makePushConstants(dims, block_dim, isovalue):
+    """PushConstants as src/meshtrex.cpp:76-84 builds them (grid = ceil(dims/B))."""
+    pc = L.PushConstants()
+    d = (L.u32 * 3)(*dims)
+    L.lib().mtx_push_constants_init(ctypes.byref(pc), d, block_dim, isovalue)
+    return pc
+
+
+def makeOptions(**kw):
+    o = L.Options()
+    L.lib().mtx_options_default(ctypes.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k) or k.startswith("_"):
+            raise TypeError("unknown option %r" % k)
+        setattr(o, k, int(v))
+    return o
+
+
+def _stream(s):
+    if s is None:
+        return None
+    return int(s)
+
+
+class MinMaxOutput:
+    """MinMaxOutput (src/minMaxOutput.h:10-64): device volume + min/max pyramid."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self.c = L.CMinMaxOutput()
+
+    @property
+    def num_levels(self):
+        return self.c.num_levels
+
+    @property
+    def grid(self):
+        return tuple(self.c.grid)
+
+    def level_extent(self, level):
+        return tuple(self.c.level_extent[level])
+
+    def read_pairs(self, level=0):
+        """(n, 2) uint32 key pairs of one level, x-fastest."""
+        o0, o1 = self.c.level_offset[level], self.c.level_offset[level + 1]
+        return read_device(self.c.minmax + 8 * o0, np.uint32, 2 * (o1 - o0)).reshape(-1, 2)
+
+    def cleanup(self, device=None):
+        if self.ctx is not None and self.c.minmax:
+            L.lib().mtx_minmax_release(self.ctx.handle, ctypes.byref(self.c))
+
+    def __del__(self):
+        try:
+            self.cleanup()
+        except Exception:
+            pass
+
+
+class FilteringOutput:
+    """FilteringOutput (src/filteringOutput.h:11-74)."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self.c = L.CFilteringOutput()
+
+    @property
+    def activeBlockCount(self):
+        return self.c.activeBlockCount
+
+    def read_ids(self):
+        n = read_device(self.c.active_block_count, np.uint32, 1)[0]
+        return read_device(self.c.compacted_block_ids, np.uint32, n)
+
+    def cleanup(self, device=None):
+        if self.ctx is not None and self.c.compacted_block_ids:
+            L.lib().mtx_filtering_release(self.ctx.handle, ctypes.byref(self.c))
+
+    def __del__(self):
+        try:
+            self.cleanup()
+        except Exception:
+            pass
+
+
+class ExtractionOutput:
+    """ExtractionOutput (src/extractionOutput.h:10-122) with RAII cleanup."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self.c = L.CExtractionOutput()
+
+    @property
+    def vertexCount(self):
+        return self.c.vertexCount
+
+    @property
+    def indexCount(self):
+        return self.c.indexCount
+
+    @property
+    def meshletCount(self):
+        return self.c.meshletCount
+
+    def read_counts(self):
+        _check(self.ctx, L.lib().mtx_read_extraction_counts(self.ctx.handle, ctypes.byref(self.c)))
+        return self.c.vertexCount, self.c.indexCount, self.c.meshletCount
+
+    def read_vertices(self):
+        """(n, 8) float32: position xyzw (NDC, w=1), normal xyzw (w=0)."""
+        return read_device(self.c.vertices, np.float32, 8 * self.c.vertexCount).reshape(-1, 8)
+
+    def read_indices(self):
+        return read_device(self.c.indices, np.uint32, self.c.indexCount)
+
+    def read_edge_ids(self):
+        return read_device(self.c.vertex_edge_ids, np.uint64, self.c.vertexCount)
+
+    def read_voxel_positions(self):
+        return read_device(self.c.vertex_voxel_pos, np.float32, 3 * self.c.vertexCount).reshape(-1, 3)
+
+    def read_case_indices(self, n_active, block_dim):
+        return read_device(self.c.case_indices, np.uint8, n_active * block_dim ** 3)
+
+    def cleanup(self):
+        if self.ctx is not None and self.c.workspace:
+            L.lib().mtx_extraction_release(self.ctx.handle, ctypes.byref(self.c))
+
+    def __del__(self):
+        try:
+            self.cleanup()
+        except Exception:
+            pass
+
+
+def computeMinMaxMip(context, volume, pushConstants, stream=None, profiler=None, options=None):
+    out = MinMaxOutput(context)
+    v = volume._c()
+    st = L.lib().mtx_compute_min_max(context.handle, ctypes.byref(v), ctypes.byref(pushConstants),
+                                     ctypes.byref(options) if options is not None else None,
+                                     _stream(stream), ctypes.byref(out.c))
+    _check(context, st)
+    out._volume_ref = volume
+    return out
+
+
+def filterActiveBlocks(context, minMaxOutput, pushConstants, stream=None, profiler=None,
+                       options=None, out=None):
+    out = out if out is not None else FilteringOutput(context)
+    st = L.lib().mtx_filter_active_blocks(context.handle, ctypes.byref(minMaxOutput.c),
+                                          ctypes.byref(pushConstants),
+                                          ctypes.byref(options) if options is not None else None,
+                                          _stream(stream), ctypes.byref(out.c))
+    _check(context, st)
+    return out
+
+
+def readActiveBlockCount(context, filteringOutput):
+    _check(context, L.lib().mtx_read_active_count(context.handle, ctypes.byref(filteringOutput.c)))
+
+
+def extractMeshletDescriptors(context, minMaxOutput, filterOutput, pushConstants, stream=None,
+                              profiler=None, options=None, out=None):
+    out = out if out is not None else ExtractionOutput(context)
+    st = L.lib().mtx_extract(context.handle, ctypes.byref(minMaxOutput.c), ctypes.byref(filterOutput.c),
+                             ctypes.byref(pushConstants),
+                             ctypes.byref(options) if options is not None else None,
+                             _stream(stream), ctypes.byref(out.c))
+    _check(context, st)
+    return out
+
+
+def key_f32(x):
+    return L.lib().mtx_key_f32(x)
+
+
+def unkey_f32(k):
+    return L.lib().mtx_unkey_f32(k)

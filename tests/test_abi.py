@@ -1,0 +1,80 @@
+"""The C-ABI library loads on a host without a GPU, exports every function
+include/meshtrex_amd/mtx.h declares, and the ctypes mirrors match the C
+layouts.  No compute calls (no GPU here)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from meshtrex_amd import _lib as L
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include", "meshtrex_amd")):
+        if not fn.endswith(".h"):
+            continue
+        text = open(os.path.join(ROOT, "include", "meshtrex_amd", fn)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b(mtx_\w+)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_header_declares_the_path():
+    names = declared_functions()
+    for n in ("mtx_compute_min_max", "mtx_filter_active_blocks", "mtx_read_active_count",
+              "mtx_extract", "mtx_context_create"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(L.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_bindings_cover_every_declared_symbol():
+    assert set(declared_functions()) <= set(L.exported_symbols())
+
+
+@pytest.mark.parametrize("name,cls", [
+    ("mtx_push_constants", L.PushConstants), ("mtx_volume", L.CVolume),
+    ("mtx_options", L.Options), ("mtx_minmax_output", L.CMinMaxOutput),
+    ("mtx_filtering_output", L.CFilteringOutput), ("mtx_extraction_output", L.CExtractionOutput)])
+def test_struct_layouts(name, cls):
+    assert L.lib().mtx_struct_size(name.encode()) == ctypes.sizeof(cls)
+
+
+def test_push_constants_are_64_bytes_like_the_reference():
+    # src/common.h:29-34: alignas(16) { uvec4 volumeDim, blockDim, blockGridDim; float iso }
+    assert ctypes.sizeof(L.PushConstants) == 64
+    assert L.lib().mtx_struct_size(b"mtx_vertex") == 32      # VertexData
+    assert L.lib().mtx_struct_size(b"mtx_meshlet") == 16     # MeshletDescriptor
+
+
+def test_push_constants_init_matches_reference_main():
+    from meshtrex_amd import makePushConstants
+    pc = makePushConstants((1024, 1024, 314), 3, 80.0)   # src/meshtrex.cpp:76-84
+    assert list(pc.blockGridDim)[:3] == [342, 342, 105]
+    assert list(pc.volumeDim) == [1024, 1024, 314, 1] and pc.isovalue == 80.0
+
+
+def test_key_roundtrip_and_order():
+    import numpy as np
+    xs = np.array([-np.inf, -1e30, -1.5, -0.0, 0.0, 1e-40, 2.5, np.inf], np.float32)
+    keys = [L.lib().mtx_key_f32(float(x)) for x in xs]
+    assert keys == sorted(keys) and len(set(keys)) == len(keys)
+    for x, k in zip(xs, keys):
+        assert np.float32(L.lib().mtx_unkey_f32(k)).tobytes() == x.tobytes()
+
+
+def test_context_create_fails_cleanly_without_gpu():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    h = ctypes.c_void_p()
+    assert L.lib().mtx_context_create(0, ctypes.byref(h)) == L.MTX_EDEVICE
