@@ -81,7 +81,8 @@ typedef struct mtx_volume {
     uint32_t _pad;
 } mtx_volume;
 
-/* Options; pass NULL for the defaults shown. */
+/* Options; pass NULL for the defaults shown.  Output structs passed to the
+ * entry points must be zero-initialised, or come from an earlier call. */
 typedef struct mtx_options {
     uint32_t overlap;        /* 1: block window [B*b, B*b+B] (live GPU shader, default);
                                 0: [B*b, B*b+B-1] (reference CPU oracle; min-max only) */
@@ -94,9 +95,10 @@ typedef struct mtx_options {
     uint32_t case_indices;   /* 1: also write the case index of every cell of every active
                                 block (default 0) */
     uint32_t voxel_positions;/* 1: also write float3 positions in voxel units (default 0) */
-    uint32_t reuse;          /* 1: mtx_extract reuses the output's existing buffers: no
-                                allocation, no host synchronisation; overflow is reported
-                                by mtx_read_extraction_counts (default 0) */
+    uint32_t reuse;          /* 1: reuse the output's existing buffers when they fit the
+                                problem: no allocation, no host synchronisation.  For
+                                mtx_extract an overflow is then reported by
+                                mtx_read_extraction_counts (default 0) */
     uint32_t _reserved[7];
 } mtx_options;
 
@@ -107,7 +109,8 @@ typedef struct mtx_minmax_output {
     uint32_t* minmax;                          /* device, 2 * level_offset[num_levels] */
     uint64_t level_offset[MTX_MAX_LEVELS + 1]; /* in pairs */
     uint32_t level_extent[MTX_MAX_LEVELS][3];
-    uint32_t num_levels;
+    uint32_t num_levels;                       /* 1 + floor(log2(max grid extent)) */
+    uint32_t levels_built;                     /* levels holding data (1 = leaf only) */
     uint32_t dtype;
     uint32_t dims[3];
     uint32_t block_dim;
@@ -151,7 +154,8 @@ typedef struct mtx_extraction_output {
     uint32_t* counts;            /* device uint32[3]: vertexCount, indexCount, meshletCount */
     uint32_t* indices;           /* device, 3 per triangle, global vertex indices */
     mtx_meshlet* meshlets;       /* device, one per active block (may be NULL) */
-    uint64_t* vertex_edge_ids;   /* optional: 3*linear(owner corner) + axis */
+    uint64_t* vertex_edge_ids;   /* optional (edge_ids): 3*linear(owner corner) + axis */
+    float* vertex_t;             /* optional (edge_ids): t along the owning edge, in [0,1] */
     float* vertex_voxel_pos;     /* optional: float3 per vertex, voxel units */
     uint8_t* case_indices;       /* optional: B^3 bytes per active block */
     void* workspace;             /* per-block topology kept for the adjoint */
@@ -189,6 +193,9 @@ mtx_status mtx_compute_min_max(mtx_context* ctx, const mtx_volume* volume,
                                const mtx_push_constants* pc, const mtx_options* opt, void* stream,
                                mtx_minmax_output* out);
 void mtx_minmax_release(mtx_context* ctx, mtx_minmax_output* out);
+/* Levels 1.. of the min/max pyramid (src/minMaxManager.cpp:145-179), for an
+ * output computed with build_pyramid = 0. */
+mtx_status mtx_build_pyramid(mtx_context* ctx, mtx_minmax_output* mm, void* stream);
 
 mtx_status mtx_filter_active_blocks(mtx_context* ctx, const mtx_minmax_output* mm,
                                     const mtx_push_constants* pc, const mtx_options* opt,

@@ -39,7 +39,8 @@ class Options(ctypes.Structure):
 
 class CMinMaxOutput(ctypes.Structure):
     _fields_ = [("minmax", vp), ("level_offset", u64 * (MTX_MAX_LEVELS + 1)),
-                ("level_extent", (u32 * 3) * MTX_MAX_LEVELS), ("num_levels", u32), ("dtype", u32),
+                ("level_extent", (u32 * 3) * MTX_MAX_LEVELS), ("num_levels", u32),
+                ("levels_built", u32), ("dtype", u32),
                 ("dims", u32 * 3), ("block_dim", u32), ("grid", u32 * 3), ("overlap", u32),
                 ("volume", vp), ("owned_volume", vp), ("active_mask", vp),
                 ("mask_words_per_row", u32), ("mask_isovalue", f32), ("mask_rule", u32),
@@ -53,7 +54,7 @@ class CFilteringOutput(ctypes.Structure):
 
 class CExtractionOutput(ctypes.Structure):
     _fields_ = [("vertices", vp), ("counts", vp), ("indices", vp), ("meshlets", vp),
-                ("vertex_edge_ids", vp), ("vertex_voxel_pos", vp), ("case_indices", vp),
+                ("vertex_edge_ids", vp), ("vertex_t", vp), ("vertex_voxel_pos", vp), ("case_indices", vp),
                 ("workspace", vp), ("workspace_bytes", u64), ("vertex_capacity", u64),
                 ("index_capacity", u64), ("case_capacity", u64), ("meshletCount", u32),
                 ("vertexCount", u32), ("indexCount", u32), ("block_dim", u32),
@@ -72,6 +73,7 @@ _SIGS = {
     "mtx_compute_min_max": (ctypes.c_int, [vp, ctypes.POINTER(CVolume), ctypes.POINTER(PushConstants),
                                            ctypes.POINTER(Options), vp, ctypes.POINTER(CMinMaxOutput)]),
     "mtx_minmax_release": (None, [vp, ctypes.POINTER(CMinMaxOutput)]),
+    "mtx_build_pyramid": (ctypes.c_int, [vp, ctypes.POINTER(CMinMaxOutput), vp]),
     "mtx_filter_active_blocks": (ctypes.c_int, [vp, ctypes.POINTER(CMinMaxOutput),
                                                 ctypes.POINTER(PushConstants), ctypes.POINTER(Options),
                                                 vp, ctypes.POINTER(CFilteringOutput)]),
@@ -100,6 +102,7 @@ SYNTH_SIGS = {
     "mtx_synth_vessels_u8": (ctypes.c_int, [vp, ctypes.POINTER(u32), u64, ctypes.c_int, ctypes.c_int, vp]),
     "mtx_synth_sdf_f32": (ctypes.c_int, [vp, ctypes.POINTER(u32), u64, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_double, ctypes.c_double, vp]),
+    "mtx_bench_copy": (ctypes.c_int, [vp, vp, u64, vp]),
 }
 
 _lib = None

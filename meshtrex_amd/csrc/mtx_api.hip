@@ -235,7 +235,6 @@ mtx_status mtx_compute_min_max(mtx_context* ctx, const mtx_volume* volume,
     if (!volume || !pc || !out) return fail(ctx, MTX_EINVAL, "null argument");
     if (!volume->data) return fail(ctx, MTX_EINVAL, "volume data is null");
     const mtx_options o = opt ? *opt : defaults();
-    std::memset(out, 0, sizeof *out);
     const uint32_t X = volume->dims[0], Y = volume->dims[1], Z = volume->dims[2];
     const uint32_t B = pc->blockDim[0];
     if (volume->dtype != MTX_U8 && volume->dtype != MTX_F32)
@@ -259,62 +258,76 @@ mtx_status mtx_compute_min_max(mtx_context* ctx, const mtx_volume* volume,
     Bound bd;
     if (mtx_status st = bind(ctx, stream, bd)) return st;
 
-    out->dtype = volume->dtype;
-    out->dims[0] = X; out->dims[1] = Y; out->dims[2] = Z;
-    out->block_dim = B;
-    out->grid[0] = g.gx; out->grid[1] = g.gy; out->grid[2] = g.gz;
-    out->overlap = o.overlap;
-    out->num_levels = pyramid_layout(out->grid, out->level_extent, out->level_offset);
-    if (out->num_levels > MTX_MAX_LEVELS) return fail(ctx, MTX_EINVAL, "pyramid too deep");
     const uint64_t esz = volume->dtype == MTX_F32 ? 4 : 1;
     const uint64_t vbytes = (uint64_t)X * Y * Z * esz;
-
+    const bool fast = B == 8 && o.overlap == 1;
+    const bool fuse = fast && o.fuse_filter;
+    // reuse: same problem geometry as the buffers in `out` -> no allocation
+    const bool reuse = o.reuse && out->minmax && out->dtype == volume->dtype && out->dims[0] == X &&
+                       out->dims[1] == Y && out->dims[2] == Z && out->block_dim == B &&
+                       (volume->on_device || out->owned_volume) && (!fuse || out->active_mask);
+    if (!reuse) {
+        if (o.reuse && out->minmax) mtx_minmax_release(ctx, out);
+        std::memset(out, 0, sizeof *out);
+        out->dtype = volume->dtype;
+        out->dims[0] = X; out->dims[1] = Y; out->dims[2] = Z;
+        out->block_dim = B;
+        out->grid[0] = g.gx; out->grid[1] = g.gy; out->grid[2] = g.gz;
+        out->num_levels = pyramid_layout(out->grid, out->level_extent, out->level_offset);
+        if (out->num_levels > MTX_MAX_LEVELS) return fail(ctx, MTX_EINVAL, "pyramid too deep");
+        if (!volume->on_device) MTX_HIP(ctx, hipMalloc(&out->owned_volume, vbytes));
+        hipError_t e = hipMalloc(&out->minmax, 8 * out->level_offset[out->num_levels]);
+        if (e == hipSuccess && fuse) {
+            out->mask_words_per_row = (g.gx + 31) / 32;
+            e = hipMalloc(&out->active_mask, 4 * (uint64_t)out->mask_words_per_row * g.gy * g.gz);
+        }
+        if (e != hipSuccess) {
+            mtx_minmax_release(ctx, out);
+            return fail(ctx, MTX_ENOMEM, "min/max buffers");
+        }
+    }
+    out->overlap = o.overlap;
+    out->mask_valid = 0;
+    out->levels_built = 1;
     if (volume->on_device) {
         out->volume = volume->data;
     } else {
-        MTX_HIP(ctx, hipMalloc(&out->owned_volume, vbytes));
         out->volume = out->owned_volume;
-        hipError_t e = hipMemcpyAsync(out->owned_volume, volume->data, vbytes,
-                                      hipMemcpyHostToDevice, bd.s);
-        if (e != hipSuccess) {
-            mtx_minmax_release(ctx, out);
-            return fail(ctx, MTX_EDEVICE, "volume upload: %s", hipGetErrorString(e));
-        }
-    }
-    const uint64_t pairs = o.build_pyramid ? out->level_offset[out->num_levels] : g.nb();
-    if (!o.build_pyramid) out->num_levels = 1;
-    hipError_t e = hipMalloc(&out->minmax, 8 * pairs);
-    if (e != hipSuccess) {
-        mtx_minmax_release(ctx, out);
-        return fail(ctx, MTX_ENOMEM, "min/max buffer (%llu pairs)", (unsigned long long)pairs);
+        hipError_t e = hipMemcpyAsync(out->owned_volume, volume->data, vbytes, hipMemcpyHostToDevice, bd.s);
+        if (e != hipSuccess) return fail(ctx, MTX_EDEVICE, "volume upload: %s", hipGetErrorString(e));
     }
     VolDesc v{ out->volume, X, Y, Z, (int)volume->dtype };
-    const bool fast = B == 8 && o.overlap == 1;
-    const bool fuse = fast && o.fuse_filter;
     ActParams act = make_act((int)volume->dtype, pc->isovalue, (int)o.rule);
+    hipError_t e;
+    if (fast)
+        e = launch_minmax_b8(v, g, out->minmax, fuse ? out->active_mask : nullptr, &act, nullptr, 0, bd.s);
+    else
+        e = launch_minmax_generic(v, g, (int)o.overlap, out->minmax, bd.s);
+    if (e == hipSuccess && o.build_pyramid && out->num_levels > 1) {
+        e = launch_pyramid(out->minmax, out->num_levels, out->level_extent, out->level_offset, bd.s);
+        out->levels_built = out->num_levels;
+    }
+    if (e != hipSuccess) return fail(ctx, MTX_EDEVICE, "min/max launch: %s", hipGetErrorString(e));
     if (fuse) {
-        out->mask_words_per_row = (g.gx + 31) / 32;
-        uint64_t words = (uint64_t)out->mask_words_per_row * g.gy * g.gz;
-        e = hipMalloc(&out->active_mask, 4 * words);
-        if (e != hipSuccess) {
-            mtx_minmax_release(ctx, out);
-            return fail(ctx, MTX_ENOMEM, "activity mask");
-        }
         out->mask_isovalue = pc->isovalue;
         out->mask_rule = o.rule;
         out->mask_valid = 1;
     }
-    if (fast)
-        e = launch_minmax_b8(v, g, out->minmax, fuse ? out->active_mask : nullptr, &act, nullptr, 0,
-                             bd.s);
-    else
-        e = launch_minmax_generic(v, g, (int)o.overlap, out->minmax, bd.s);
-    if (e == hipSuccess && o.build_pyramid && out->num_levels > 1)
-        e = launch_pyramid(out->minmax, out->num_levels, out->level_extent, out->level_offset, bd.s);
-    if (e != hipSuccess) {
-        mtx_minmax_release(ctx, out);
-        return fail(ctx, MTX_EDEVICE, "min/max launch: %s", hipGetErrorString(e));
+    return finish(bd);
+}
+
+mtx_status mtx_build_pyramid(mtx_context* ctx, mtx_minmax_output* mm, void* stream)
+{
+    if (!ctx) return MTX_EINVAL;
+    if (!mm || !mm->minmax) return fail(ctx, MTX_ESTATE, "min/max output is empty");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    Bound bd;
+    if (mtx_status st = bind(ctx, stream, bd)) return st;
+    if (mm->num_levels > 1) {
+        hipError_t e = launch_pyramid(mm->minmax, mm->num_levels, mm->level_extent, mm->level_offset, bd.s);
+        if (e != hipSuccess) return fail(ctx, MTX_EDEVICE, "pyramid launch: %s", hipGetErrorString(e));
     }
+    mm->levels_built = mm->num_levels;
     return finish(bd);
 }
 
@@ -433,6 +446,7 @@ static ExtractArgs make_extract_args(const mtx_minmax_output* mm, const mtx_filt
     a.indices = out->indices;
     a.edge_ids = out->vertex_edge_ids;
     a.vox_pos = out->vertex_voxel_pos;
+    a.edge_t = out->vertex_t;
     a.vcap = out->vertex_capacity;
     a.icap = out->index_capacity;
     a.out_counts = out->counts;
@@ -475,7 +489,10 @@ mtx_status mtx_extract(mtx_context* ctx, const mtx_minmax_output* mm,
             return fail(ctx, MTX_ESTATE, "reuse: output buffers were made for another problem");
         ExtractArgs a = make_extract_args(mm, filt, pc, out, w);
         a.normals = o.normals ? 1 : 0;
-        if (!o.edge_ids) a.edge_ids = nullptr;
+        if (!o.edge_ids) {
+            a.edge_ids = nullptr;
+            a.edge_t = nullptr;
+        }
         if (!o.voxel_positions) a.vox_pos = nullptr;
         if (!o.case_indices) a.case_out = nullptr;
         if (a.case_out && out->case_capacity < nb * B * B * B) a.case_out = nullptr;
@@ -532,6 +549,7 @@ mtx_status mtx_extract(mtx_context* ctx, const mtx_minmax_output* mm,
     e = hipMalloc(&out->vertices, sizeof(mtx_vertex) * (tot[0] ? tot[0] : 1));
     if (e == hipSuccess) e = hipMalloc(&out->indices, 4 * (tot[1] ? 3 * tot[1] : 1));
     if (e == hipSuccess && o.edge_ids) e = hipMalloc(&out->vertex_edge_ids, 8 * (tot[0] ? tot[0] : 1));
+    if (e == hipSuccess && o.edge_ids) e = hipMalloc(&out->vertex_t, 4 * (tot[0] ? tot[0] : 1));
     if (e == hipSuccess && o.voxel_positions) e = hipMalloc(&out->vertex_voxel_pos, 12 * (tot[0] ? tot[0] : 1));
     if (e != hipSuccess) {
         mtx_extraction_release(ctx, out);
@@ -573,7 +591,7 @@ void mtx_extraction_release(mtx_context* ctx, mtx_extraction_output* out)
     if (!out) return;
     if (ctx) hipSetDevice(ctx->device);
     void* ptrs[] = { out->vertices, out->counts, out->indices, out->meshlets, out->vertex_edge_ids,
-                     out->vertex_voxel_pos, out->case_indices, out->workspace };
+                     out->vertex_t, out->vertex_voxel_pos, out->case_indices, out->workspace };
     for (void* p : ptrs)
         if (p) hipFree(p);
     std::memset(out, 0, sizeof *out);

@@ -88,6 +88,7 @@ struct ExtractArgs {
     uint32_t* indices;          // 3 per triangle
     uint64_t* edge_ids;         // optional
     float* vox_pos;             // optional: 3 floats per vertex, voxel units
+    float* edge_t;              // optional: t along the owning edge
     uint64_t vcap, icap;        // capacities (vertices, indices)
     uint32_t* out_counts;       // optional device uint32[3]: vertexCount, indexCount, meshletCount
     int normals;

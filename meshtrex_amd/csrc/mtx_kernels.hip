@@ -989,6 +989,7 @@ __global__ __launch_bounds__(256) void k_extract_emit(ExtractArgs a)
                         dst[0] = pos;
                         dst[1] = nrm;
                         if (a.edge_ids) a.edge_ids[rv] = 3ull * (x + (uint64_t)X * (y + (uint64_t)Y * z)) + ax;
+                        if (a.edge_t) a.edge_t[rv] = t;
                         if (a.vox_pos) {
                             a.vox_pos[3 * rv + 0] = px;
                             a.vox_pos[3 * rv + 1] = py;

@@ -161,3 +161,22 @@ int mtx_synth_sdf_f32(void* out, const uint32_t dims[3], uint64_t seed, int nsph
 }
 
 }  // extern "C"
+
+// Plain device copy of known size: the calibration kernel for HBM byte
+// counters (FETCH_SIZE / WRITE_SIZE) of the benchmark.
+namespace {
+__global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst,
+                                              uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256u)
+        dst[i] = src[i];
+}
+}  // namespace
+
+extern "C" int mtx_bench_copy(void* dst, const void* src, uint64_t bytes, void* stream)
+{
+    if (!dst || !src || bytes % 16) return 1;
+    hipLaunchKernelGGL(k_copy, dim3(8192), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       static_cast<const float4*>(src), static_cast<float4*>(dst), bytes / 16);
+    return hipGetLastError() == hipSuccess ? 0 : 4;
+}

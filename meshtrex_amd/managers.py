@@ -285,6 +285,10 @@ class ExtractionOutput:
     def read_edge_ids(self):
         return read_device(self.c.vertex_edge_ids, np.uint64, self.c.vertexCount)
 
+    def read_t(self):
+        """t of each vertex along its owning edge (written with the edge IDs)."""
+        return read_device(self.c.vertex_t, np.float32, self.c.vertexCount)
+
     def read_voxel_positions(self):
         return read_device(self.c.vertex_voxel_pos, np.float32, 3 * self.c.vertexCount).reshape(-1, 3)
 

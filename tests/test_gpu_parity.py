@@ -160,13 +160,17 @@ def check_extraction(ctx, data, dims, B, iso, rule=0, normals=True):
     np.testing.assert_array_equal(oracle.canonical_triangles(tris, eid),
                                   oracle.canonical_triangles(ref["tris"], ref["edge_ids"]))
     if nv:
+        # the contract's record: (owning edge ID, t), t within 1e-6 of FP64
+        assert np.max(np.abs(ex.read_t().astype(np.float64) - ref["t"])) <= 1e-6
+        # voxel-space positions: one fp32 rounding of owner + t
         vp = ex.read_voxel_positions().astype(np.float64)
         axis = (eid % 3).astype(np.int64)
         vox = eid // 3
         X, Y = dims[0], dims[1]
         owner = np.stack([vox % X, (vox // X) % Y, vox // (X * Y)], axis=1).astype(np.float64)
-        t = vp[np.arange(nv), axis] - owner[np.arange(nv), axis]
-        assert np.max(np.abs(t - ref["t"])) <= 1e-6
+        p64 = owner.copy()
+        p64[np.arange(nv), axis] += ref["t"]
+        assert np.all(np.abs(vp - p64) <= 2.0 ** -23 * np.maximum(np.abs(p64), 1.0) + 1e-6)
         verts = ex.read_vertices()
         assert np.max(np.abs(verts[:, :3] - ref["ndc"])) <= 2e-6
         assert np.all(verts[:, 3] == 1.0)
