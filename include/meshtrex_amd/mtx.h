@@ -128,6 +128,7 @@ typedef struct mtx_minmax_output {
 typedef struct mtx_filtering_output {
     uint32_t* compacted_block_ids;  /* device, ascending, capacity = number of blocks */
     uint32_t* active_block_count;   /* device, one uint32 */
+    uint32_t* block_to_active;      /* device, one per block: index in the list, or 0xFFFFFFFF */
     uint32_t activeBlockCount;      /* host mirror, filled by mtx_read_active_count */
     uint32_t capacity;
     float isovalue;
@@ -153,7 +154,9 @@ typedef struct mtx_extraction_output {
     mtx_vertex* vertices;        /* device */
     uint32_t* counts;            /* device uint32[3]: vertexCount, indexCount, meshletCount */
     uint32_t* indices;           /* device, 3 per triangle, global vertex indices */
-    mtx_meshlet* meshlets;       /* device, one per active block (may be NULL) */
+    mtx_meshlet* meshlets;       /* device: each active block's triangles in runs of <= 126
+                                    (MAX_PRIMS_PER_MESHLET, src/shaders/marching_cubes_pmb.mesh.glsl:30);
+                                    vertexOffset/vertexCount = the block's own vertices */
     uint64_t* vertex_edge_ids;   /* optional (edge_ids): 3*linear(owner corner) + axis */
     float* vertex_t;             /* optional (edge_ids): t along the owning edge, in [0,1] */
     float* vertex_voxel_pos;     /* optional: float3 per vertex, voxel units */
@@ -162,6 +165,7 @@ typedef struct mtx_extraction_output {
     uint64_t workspace_bytes;
     uint64_t vertex_capacity;    /* in vertices */
     uint64_t index_capacity;     /* in uint32 indices */
+    uint64_t meshlet_capacity;   /* in descriptors */
     uint64_t case_capacity;      /* in bytes */
     uint32_t meshletCount;       /* host mirrors (mtx_read_extraction_counts) */
     uint32_t vertexCount;

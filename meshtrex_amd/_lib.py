@@ -48,7 +48,7 @@ class CMinMaxOutput(ctypes.Structure):
 
 
 class CFilteringOutput(ctypes.Structure):
-    _fields_ = [("compacted_block_ids", vp), ("active_block_count", vp),
+    _fields_ = [("compacted_block_ids", vp), ("active_block_count", vp), ("block_to_active", vp),
                 ("activeBlockCount", u32), ("capacity", u32), ("isovalue", f32), ("rule", u32)]
 
 
@@ -56,7 +56,8 @@ class CExtractionOutput(ctypes.Structure):
     _fields_ = [("vertices", vp), ("counts", vp), ("indices", vp), ("meshlets", vp),
                 ("vertex_edge_ids", vp), ("vertex_t", vp), ("vertex_voxel_pos", vp), ("case_indices", vp),
                 ("workspace", vp), ("workspace_bytes", u64), ("vertex_capacity", u64),
-                ("index_capacity", u64), ("case_capacity", u64), ("meshletCount", u32),
+                ("index_capacity", u64), ("meshlet_capacity", u64), ("case_capacity", u64),
+                ("meshletCount", u32),
                 ("vertexCount", u32), ("indexCount", u32), ("block_dim", u32),
                 ("max_blocks", u32), ("flags", u32)]
 

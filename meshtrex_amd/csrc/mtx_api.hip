@@ -106,7 +106,7 @@ uint32_t pyramid_layout(const uint32_t G[3], uint32_t (*ext)[3], uint64_t* off)
 
 // Extraction workspace layout (offsets in bytes).
 struct WsLayout {
-    uint64_t counts, vbase, tbase, table, totals, scan, bytes;
+    uint64_t counts, vbase, tbase, mbase, table, totals, scan, bytes;
     uint32_t scan_tiles;
 };
 
@@ -117,12 +117,13 @@ WsLayout ws_layout(uint64_t max_blocks, uint32_t B)
     w.counts = o; o = align256(o + 8 * max_blocks);
     w.vbase = o;  o = align256(o + 8 * max_blocks);
     w.tbase = o;  o = align256(o + 8 * max_blocks);
+    w.mbase = o;  o = align256(o + 8 * max_blocks);
     w.table = o;  o = align256(o + corner_entry_bytes(B) * (uint64_t)B * B * B * max_blocks);
     w.totals = o; o = align256(o + 8 * 4);
     uint64_t tiles = (max_blocks + scan_tile_items() - 1) / scan_tile_items();
     if (tiles == 0) tiles = 1;
     w.scan_tiles = (uint32_t)tiles;
-    w.scan = o;   o = align256(o + 8 * (2 * tiles + 1));
+    w.scan = o;   o = align256(o + 8 * (3 * tiles + 1));
     w.bytes = o;
     return w;
 }
@@ -360,11 +361,13 @@ mtx_status mtx_filter_active_blocks(mtx_context* ctx, const mtx_minmax_output* m
     Bound bd;
     if (mtx_status st = bind(ctx, stream, bd)) return st;
 
-    const bool reuse = out->compacted_block_ids && out->capacity >= g.nb();
+    const bool reuse = out->compacted_block_ids && out->block_to_active && out->capacity >= g.nb();
     if (!reuse) {
+        if (out->compacted_block_ids) mtx_filtering_release(ctx, out);
         std::memset(out, 0, sizeof *out);
         MTX_HIP(ctx, hipMalloc(&out->compacted_block_ids, 4 * (g.nb() ? g.nb() : 1)));
         hipError_t e = hipMalloc(&out->active_block_count, 256 + 8ull * 65536);
+        if (e == hipSuccess) e = hipMalloc(&out->block_to_active, 4 * (g.nb() ? g.nb() : 1));
         if (e != hipSuccess) {
             mtx_filtering_release(ctx, out);
             return fail(ctx, MTX_ENOMEM, "filter buffers");
@@ -392,10 +395,12 @@ mtx_status mtx_filter_active_blocks(mtx_context* ctx, const mtx_minmax_output* m
     if (e == hipSuccess) {
         if (use_mask)
             e = launch_compact_mask(mm->active_mask, g, mm->mask_words_per_row,
-                                    out->compacted_block_ids, out->active_block_count, st, bd.s);
+                                    out->compacted_block_ids, out->active_block_count,
+                                    out->block_to_active, st, bd.s);
         else
             e = launch_filter_pairs(mm->minmax, g, make_act((int)mm->dtype, pc->isovalue, (int)o.rule),
-                                    out->compacted_block_ids, out->active_block_count, st, bd.s);
+                                    out->compacted_block_ids, out->active_block_count,
+                                    out->block_to_active, st, bd.s);
     }
     if (e != hipSuccess) return fail(ctx, MTX_EDEVICE, "filter launch: %s", hipGetErrorString(e));
     return finish(bd);
@@ -420,6 +425,7 @@ void mtx_filtering_release(mtx_context* ctx, mtx_filtering_output* out)
     if (ctx) hipSetDevice(ctx->device);
     if (out->compacted_block_ids) hipFree(out->compacted_block_ids);
     if (out->active_block_count) hipFree(out->active_block_count);
+    if (out->block_to_active) hipFree(out->block_to_active);
     std::memset(out, 0, sizeof *out);
 }
 
@@ -435,10 +441,12 @@ static ExtractArgs make_extract_args(const mtx_minmax_output* mm, const mtx_filt
     a.iso = pc->isovalue;
     a.ids = filt->compacted_block_ids;
     a.count = filt->active_block_count;
+    a.blk_map = filt->block_to_active;
     char* ws = static_cast<char*>(out->workspace);
     a.blk_counts = reinterpret_cast<uint32_t*>(ws + w.counts);
     a.blk_vbase = reinterpret_cast<uint64_t*>(ws + w.vbase);
     a.blk_tbase = reinterpret_cast<uint64_t*>(ws + w.tbase);
+    a.blk_mbase = reinterpret_cast<uint64_t*>(ws + w.mbase);
     a.corner_table = ws + w.table;
     a.totals = reinterpret_cast<uint64_t*>(ws + w.totals);
     a.case_out = out->case_indices;
@@ -447,8 +455,10 @@ static ExtractArgs make_extract_args(const mtx_minmax_output* mm, const mtx_filt
     a.edge_ids = out->vertex_edge_ids;
     a.vox_pos = out->vertex_voxel_pos;
     a.edge_t = out->vertex_t;
+    a.meshlets = reinterpret_cast<uint4*>(out->meshlets);
     a.vcap = out->vertex_capacity;
     a.icap = out->index_capacity;
+    a.mcap = out->meshlet_capacity;
     a.out_counts = out->counts;
     a.max_blocks = out->max_blocks;
     return a;
@@ -485,7 +495,8 @@ mtx_status mtx_extract(mtx_context* ctx, const mtx_minmax_output* mm,
 
     if (o.reuse) {
         if (!out->workspace || out->workspace_bytes < w.bytes || out->block_dim != B ||
-            out->max_blocks < nb || (out->flags & want) != want || !out->vertices || !out->indices)
+            out->max_blocks < nb || (out->flags & want) != want || !out->vertices || !out->indices ||
+            !out->meshlets)
             return fail(ctx, MTX_ESTATE, "reuse: output buffers were made for another problem");
         ExtractArgs a = make_extract_args(mm, filt, pc, out, w);
         a.normals = o.normals ? 1 : 0;
@@ -497,9 +508,9 @@ mtx_status mtx_extract(mtx_context* ctx, const mtx_minmax_output* mm,
         if (!o.case_indices) a.case_out = nullptr;
         if (a.case_out && out->case_capacity < nb * B * B * B) a.case_out = nullptr;
         uint64_t* scan = reinterpret_cast<uint64_t*>(static_cast<char*>(out->workspace) + w.scan);
-        ScanState st{ scan, reinterpret_cast<uint32_t*>(scan + 2 * w.scan_tiles) };
+        ScanState st{ scan, reinterpret_cast<uint32_t*>(scan + 3 * w.scan_tiles) };
         hipError_t e = launch_zero64(reinterpret_cast<uint64_t*>(static_cast<char*>(out->workspace) + w.totals),
-                                     (w.scan - w.totals) / 8 + 2 * w.scan_tiles + 1, bd.s);
+                                     (w.scan - w.totals) / 8 + 3 * w.scan_tiles + 1, bd.s);
         if (e == hipSuccess) e = launch_extract_count(a, bd.s);
         if (e == hipSuccess) e = launch_extract_scan(a, st, bd.s);
         if (e == hipSuccess) e = launch_extract_emit(a, bd.s);
@@ -507,7 +518,8 @@ mtx_status mtx_extract(mtx_context* ctx, const mtx_minmax_output* mm,
         return finish(bd);
     }
 
-    // Allocating form: count, scan, read the two totals, allocate exactly, emit.
+    // Allocating form: count, scan, read the totals, allocate exactly, emit.
+    if (out->workspace) mtx_extraction_release(ctx, out);
     std::memset(out, 0, sizeof *out);
     out->block_dim = B;
     out->max_blocks = (uint32_t)nb;
@@ -526,14 +538,14 @@ mtx_status mtx_extract(mtx_context* ctx, const mtx_minmax_output* mm,
     ExtractArgs a = make_extract_args(mm, filt, pc, out, w);
     a.normals = o.normals ? 1 : 0;
     uint64_t* scan = reinterpret_cast<uint64_t*>(static_cast<char*>(out->workspace) + w.scan);
-    ScanState st{ scan, reinterpret_cast<uint32_t*>(scan + 2 * w.scan_tiles) };
+    ScanState st{ scan, reinterpret_cast<uint32_t*>(scan + 3 * w.scan_tiles) };
     e = launch_zero64(reinterpret_cast<uint64_t*>(static_cast<char*>(out->workspace) + w.totals),
-                      (w.scan - w.totals) / 8 + 2 * w.scan_tiles + 1, bd.s);
+                      (w.scan - w.totals) / 8 + 3 * w.scan_tiles + 1, bd.s);
     if (e == hipSuccess) e = hipMemsetAsync(out->counts, 0, 256, bd.s);
     if (e == hipSuccess) e = launch_extract_count(a, bd.s);
     if (e == hipSuccess) e = launch_extract_scan(a, st, bd.s);
-    uint64_t tot[2] = { 0, 0 };
-    if (e == hipSuccess) e = hipMemcpyAsync(tot, a.totals, 16, hipMemcpyDeviceToHost, bd.s);
+    uint64_t tot[4] = { 0, 0, 0, 0 };
+    if (e == hipSuccess) e = hipMemcpyAsync(tot, a.totals, 32, hipMemcpyDeviceToHost, bd.s);
     if (e == hipSuccess) e = hipStreamSynchronize(bd.s);
     if (e != hipSuccess) {
         mtx_extraction_release(ctx, out);
@@ -546,7 +558,9 @@ mtx_status mtx_extract(mtx_context* ctx, const mtx_minmax_output* mm,
     }
     out->vertex_capacity = tot[0];
     out->index_capacity = 3 * tot[1];
+    out->meshlet_capacity = tot[3];
     e = hipMalloc(&out->vertices, sizeof(mtx_vertex) * (tot[0] ? tot[0] : 1));
+    if (e == hipSuccess) e = hipMalloc(&out->meshlets, sizeof(mtx_meshlet) * (tot[3] ? tot[3] : 1));
     if (e == hipSuccess) e = hipMalloc(&out->indices, 4 * (tot[1] ? 3 * tot[1] : 1));
     if (e == hipSuccess && o.edge_ids) e = hipMalloc(&out->vertex_edge_ids, 8 * (tot[0] ? tot[0] : 1));
     if (e == hipSuccess && o.edge_ids) e = hipMalloc(&out->vertex_t, 4 * (tot[0] ? tot[0] : 1));
@@ -562,6 +576,7 @@ mtx_status mtx_extract(mtx_context* ctx, const mtx_minmax_output* mm,
     if (e != hipSuccess) return fail(ctx, MTX_EDEVICE, "extract emit: %s", hipGetErrorString(e));
     out->vertexCount = (uint32_t)tot[0];
     out->indexCount = (uint32_t)(3 * tot[1]);
+    out->meshletCount = (uint32_t)tot[3];
     return finish(bd);
 }
 

@@ -58,13 +58,16 @@ hipError_t launch_pyramid(uint32_t* pairs, uint32_t levels, const uint32_t (*ext
 // Number of 64-bit words a compaction's scan state needs for nb blocks.
 uint32_t filter_scan_words(uint64_t nb, bool from_mask, uint32_t words_per_row, const Grid& g);
 
-// Ordered compaction of active block IDs from the fused bit mask.
+// Ordered compaction of active block IDs from the fused bit mask.  Also writes
+// the dense map block ID -> active index (0xFFFFFFFF for inactive blocks).
 hipError_t launch_compact_mask(const uint32_t* mask, const Grid& g, uint32_t words_per_row,
-                               uint32_t* ids, uint32_t* count, ScanState st, hipStream_t s);
+                               uint32_t* ids, uint32_t* count, uint32_t* map, ScanState st,
+                               hipStream_t s);
 
-// Ordered compaction of active block IDs from leaf min/max pairs.
+// Ordered compaction of active block IDs from leaf min/max pairs (+ dense map).
 hipError_t launch_filter_pairs(const uint32_t* pairs, const Grid& g, const ActParams& act,
-                               uint32_t* ids, uint32_t* count, ScanState st, hipStream_t s);
+                               uint32_t* ids, uint32_t* count, uint32_t* map, ScanState st,
+                               hipStream_t s);
 
 // Clear n 64-bit words (a small fill kernel, safe under graph capture).
 hipError_t launch_zero64(uint64_t* p, uint64_t n, hipStream_t s);
@@ -77,11 +80,13 @@ struct ExtractArgs {
     float iso;
     const uint32_t* ids;        // active block IDs (ascending)
     const uint32_t* count;      // device active count
+    const uint32_t* blk_map;    // dense block ID -> active index (0xFFFFFFFF: inactive)
     uint32_t* blk_counts;       // per active block: {nv, nt}
     void* corner_table;         // per active block: B^3 entries (prefix<<3 | flags)
     uint64_t* blk_vbase;        // per active block vertex base
     uint64_t* blk_tbase;        // per active block triangle base
-    uint64_t* totals;           // [0]=vertices [1]=triangles [2]=overflow flag
+    uint64_t* blk_mbase;        // per active block meshlet base
+    uint64_t* totals;           // [0]=vertices [1]=triangles [2]=error flags [3]=meshlets
     uint8_t* case_out;          // optional: B^3 case bytes per active block
     // outputs
     float* vertices;            // 8 floats per vertex (position xyzw, normal xyzw)
@@ -89,7 +94,8 @@ struct ExtractArgs {
     uint64_t* edge_ids;         // optional
     float* vox_pos;             // optional: 3 floats per vertex, voxel units
     float* edge_t;              // optional: t along the owning edge
-    uint64_t vcap, icap;        // capacities (vertices, indices)
+    uint4* meshlets;            // optional: {vertexOffset, indexOffset, vertexCount, primitiveCount}
+    uint64_t vcap, icap, mcap;  // capacities (vertices, indices, meshlets)
     uint32_t* out_counts;       // optional device uint32[3]: vertexCount, indexCount, meshletCount
     int normals;
     uint32_t max_blocks;        // capacity of the per-block arrays

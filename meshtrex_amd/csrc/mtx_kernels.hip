@@ -5,6 +5,7 @@
 #include "mtx_internal.h"
 #include "mc_cases.h"
 
+#include <cstdlib>
 #include <type_traits>
 
 namespace mtx {
@@ -266,7 +267,7 @@ __global__ __launch_bounds__(256) void k_minmax_b8(
     for (uint32_t bz = bz0; bz < bz1; ++bz) {
         MMAcc acc = first;
         const uint32_t zl = min(bz * 8u + 8u, Z - 1u);
-#pragma unroll 2
+#pragma unroll 1
         for (uint32_t dz = 1; dz <= 8u; ++dz) {
             uint32_t z = min(bz * 8u + dz, zl);
             MMAcc p;
@@ -301,6 +302,16 @@ __global__ __launch_bounds__(256) void k_minmax_b8(
     }
 }
 
+// Compute units of the current device.
+static int device_cus()
+{
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+        n = 256;
+    return n;
+}
+
 template <typename T, bool MASK>
 static hipError_t launch_mm_b8_t(const VolDesc& v, const Grid& g, uint32_t* pairs, uint32_t* mask,
                                  const ActParams* act, uint64_t* zero, uint32_t zero_words,
@@ -309,8 +320,16 @@ static hipError_t launch_mm_b8_t(const VolDesc& v, const Grid& g, uint32_t* pair
     const uint32_t ntx = (g.gx + 31u) / 32u;
     const uint32_t ngy = (g.gy + 3u) / 4u;
     const uint64_t cols = (uint64_t)ntx * ngy;
-    // z-chunk: as long as possible while keeping >= 1024 workgroups (4 per CU)
-    uint64_t zc = (cols * g.gz) / 1024u;
+    // z-chunk: the shortest that keeps every workgroup resident in one round
+    // (no tail of late workgroups); longer chunks re-read fewer shared planes
+    static int resident = 0;
+    if (!resident) {
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_minmax_b8<T, true, MASK>, 256, 0);
+        if (resident < 1) resident = 1;
+    }
+    const uint64_t target = (uint64_t)device_cus() * resident;
+    uint64_t zc = (cols * g.gz + target - 1) / target;
+    if (const char* env = getenv("MTX_MINMAX_ZCHUNK")) zc = strtoull(env, nullptr, 10);
     if (zc < 1) zc = 1;
     if (zc > g.gz) zc = g.gz;
     const uint32_t ncz = (uint32_t)((g.gz + zc - 1) / zc);
@@ -494,6 +513,7 @@ uint32_t filter_scan_words(uint64_t nb, bool from_mask, uint32_t words_per_row, 
 __global__ __launch_bounds__(256) void k_filter_pairs(const uint2* __restrict__ pairs, uint64_t nb,
                                                       ActParams act, uint32_t* __restrict__ ids,
                                                       uint32_t* __restrict__ count,
+                                                      uint32_t* __restrict__ map,
                                                       uint64_t* status, uint32_t* ticket,
                                                       uint32_t ntiles)
 {
@@ -533,21 +553,26 @@ __global__ __launch_bounds__(256) void k_filter_pairs(const uint2* __restrict__ 
     const uint64_t lt = (1ull << lane) - 1ull;
 #pragma unroll
     for (uint32_t k = 0; k < kFilterItems; ++k) {
+        const uint64_t b = base + k * 256u + threadIdx.x;
+        uint32_t slot = 0xFFFFFFFFu;
         if ((bal[k] >> lane) & 1ull) {
             uint64_t pos = s_excl + s_cnt[k * 4 + w] + (uint32_t)__popcll(bal[k] & lt);
-            ids[pos] = (uint32_t)(base + k * 256u + threadIdx.x);
+            ids[pos] = (uint32_t)b;
+            slot = (uint32_t)pos;
         }
+        if (b < nb) map[b] = slot;
     }
 }
 
 hipError_t launch_filter_pairs(const uint32_t* pairs, const Grid& g, const ActParams& act,
-                               uint32_t* ids, uint32_t* count, ScanState st, hipStream_t s)
+                               uint32_t* ids, uint32_t* count, uint32_t* map, ScanState st,
+                               hipStream_t s)
 {
     const uint64_t nb = g.nb();
     uint64_t ntiles = (nb + 256 * kFilterItems - 1) / (256 * kFilterItems);
     if (ntiles == 0) ntiles = 1;
     hipLaunchKernelGGL(k_filter_pairs, dim3((uint32_t)ntiles), dim3(256), 0, s,
-                       reinterpret_cast<const uint2*>(pairs), nb, act, ids, count, st.status,
+                       reinterpret_cast<const uint2*>(pairs), nb, act, ids, count, map, st.status,
                        st.ticket, (uint32_t)ntiles);
     return hipGetLastError();
 }
@@ -557,6 +582,7 @@ __global__ __launch_bounds__(256) void k_compact_mask(const uint32_t* __restrict
                                                       uint64_t nwords, uint32_t gx, uint32_t wpr,
                                                       uint32_t* __restrict__ ids,
                                                       uint32_t* __restrict__ count,
+                                                      uint32_t* __restrict__ map,
                                                       uint64_t* status, uint32_t* ticket,
                                                       uint32_t ntiles)
 {
@@ -588,24 +614,31 @@ __global__ __launch_bounds__(256) void k_compact_mask(const uint32_t* __restrict
     }
     __syncthreads();
     uint64_t pos = s_excl + woff + inc - c;
-    const uint64_t row = wi / wpr;
-    const uint32_t base = (uint32_t)(row * gx + (wi % wpr) * 32u);
-    uint32_t m = word;
-    while (m) {
-        uint32_t bit = __builtin_ctz(m);
-        m &= m - 1u;
-        ids[pos++] = base + bit;
+    if (wi < nwords) {
+        const uint64_t row = wi / wpr;
+        const uint32_t x0 = (uint32_t)(wi % wpr) * 32u;
+        const uint32_t base = (uint32_t)(row * gx + x0);
+        const uint32_t nbits = min(32u, gx - x0);
+        for (uint32_t bit = 0; bit < nbits; ++bit) {
+            uint32_t slot = 0xFFFFFFFFu;
+            if ((word >> bit) & 1u) {
+                ids[pos] = base + bit;
+                slot = (uint32_t)pos++;
+            }
+            map[base + bit] = slot;
+        }
     }
 }
 
 hipError_t launch_compact_mask(const uint32_t* mask, const Grid& g, uint32_t words_per_row,
-                               uint32_t* ids, uint32_t* count, ScanState st, hipStream_t s)
+                               uint32_t* ids, uint32_t* count, uint32_t* map, ScanState st,
+                               hipStream_t s)
 {
     const uint64_t nwords = (uint64_t)g.gy * g.gz * words_per_row;
     uint64_t ntiles = (nwords + 255) / 256;
     if (ntiles == 0) ntiles = 1;
     hipLaunchKernelGGL(k_compact_mask, dim3((uint32_t)ntiles), dim3(256), 0, s, mask, nwords, g.gx,
-                       words_per_row, ids, count, st.status, st.ticket, (uint32_t)ntiles);
+                       words_per_row, ids, count, map, st.status, st.ticket, (uint32_t)ntiles);
     return hipGetLastError();
 }
 
@@ -618,65 +651,70 @@ hipError_t launch_compact_mask(const uint32_t* mask, const Grid& g, uint32_t wor
 // active block the count pass stores one entry per corner:
 // (vertex prefix within the block << 3) | (3 edge flags), which lets the emit
 // pass resolve edges owned by the +x/+y/+z neighbour blocks without
-// recounting them.
+// recounting them; the neighbours' active indices come from the dense
+// block -> active-index map the compaction writes.
+//
+// One wavefront owns one active block at a time (no workgroup barriers): it
+// stages the block's voxels in its own LDS tile, classifies its B^3 corners
+// (B^3/64 consecutive corners per lane, a full x-row for B = 8), scans the
+// per-lane counts with wave shuffles and writes its outputs.  Waves of a
+// workgroup work on different blocks, so a wave waiting on memory never stalls
+// the others.
 // ---------------------------------------------------------------------------
 
 template <int B> struct EB {
     static constexpr int NC = B * B * B;
-    static constexpr int CPT = (NC + 255) / 256;
+    static constexpr int CPL = (NC + 63) / 64;  // corners per lane
     using Entry = typename std::conditional<(B <= 13), uint16_t, uint32_t>::type;
 };
 
 bool extract_supported_block(uint32_t B) { return B >= 1 && B <= 16; }
 size_t corner_entry_bytes(uint32_t B) { return B <= 13 ? 2 : 4; }
 
-__device__ __forceinline__ uint32_t item_index(uint32_t k, uint32_t na)
+// Triangles per meshlet descriptor: MAX_PRIMS_PER_MESHLET of
+// src/shaders/marching_cubes_pmb.mesh.glsl:30.
+constexpr uint32_t kMeshletPrims = 126;
+
+// Orders LDS accesses between the lanes of one wave: the LDS unit executes a
+// wave's instructions in program order, so only compiler reordering must be
+// prevented.
+__device__ __forceinline__ void wave_lds_sync()
 {
-    // XCD-aware: XCD x walks a contiguous eighth of the active list, so blocks
-    // that are x-neighbours (consecutive IDs) share an L2
-    const uint32_t nwg = gridDim.x;
-    if (nwg % 8u) return blockIdx.x + k * nwg;
-    const uint32_t per = nwg / 8u;
-    const uint32_t chunk = (na + 7u) / 8u;
-    const uint32_t xcd = blockIdx.x % 8u, slot = blockIdx.x / 8u;
-    const uint32_t j = slot + k * per;
-    return j < chunk ? xcd * chunk + j : 0xFFFFFFFFu;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ bool item_done(uint32_t k, uint32_t na)
+// Items of this wave: each XCD owns a contiguous eighth of the active list, so
+// x-neighbour blocks (consecutive IDs) are processed close together on one L2.
+struct WaveItems {
+    uint32_t first, stride, end;
+};
+
+__device__ __forceinline__ WaveItems wave_items(uint32_t na, uint32_t wpg)
 {
-    const uint32_t nwg = gridDim.x;
-    if (nwg % 8u) return blockIdx.x + (uint64_t)k * nwg >= na;
-    const uint32_t per = nwg / 8u;
+    const uint32_t w = threadIdx.x >> 6;
+    WaveItems r;
+    if (gridDim.x % 8u) {
+        r.first = blockIdx.x * wpg + w;
+        r.stride = gridDim.x * wpg;
+        r.end = na;
+        return r;
+    }
     const uint32_t chunk = (na + 7u) / 8u;
-    return blockIdx.x / 8u + (uint64_t)k * per >= chunk;
+    const uint32_t xcd = blockIdx.x % 8u;
+    r.first = xcd * chunk + (blockIdx.x / 8u) * wpg + w;
+    r.stride = (gridDim.x / 8u) * wpg;
+    r.end = min(xcd * chunk + chunk, na);
+    return r;
 }
 
-// Workgroup exclusive scan of two counters; returns exclusive values and totals.
-__device__ __forceinline__ void wg_scan2(uint32_t a, uint32_t b, uint32_t& ea, uint32_t& eb,
-                                         uint32_t& ta, uint32_t& tb, uint32_t* s_scan /*[8]*/)
+template <typename V>
+__device__ __forceinline__ V wave_excl_scan(V v, V& total)
 {
-    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
-    uint32_t ia = wave_incl_scan<uint32_t>(a);
-    uint32_t ib = wave_incl_scan<uint32_t>(b);
-    if (lane == 63) {
-        s_scan[w] = ia;
-        s_scan[4 + w] = ib;
-    }
-    __syncthreads();
-    uint32_t oa = 0, ob = 0, sa = 0, sb = 0;
-#pragma unroll
-    for (uint32_t j = 0; j < 4; ++j) {
-        uint32_t va = s_scan[j], vb = s_scan[4 + j];
-        oa += j < w ? va : 0u;
-        ob += j < w ? vb : 0u;
-        sa += va;
-        sb += vb;
-    }
-    ea = oa + ia - a;
-    eb = ob + ib - b;
-    ta = sa;
-    tb = sb;
+    V inc = wave_incl_scan<V>(v);
+    total = __shfl(inc, 63, 64);
+    return inc - v;
 }
 
 template <int B, typename T>
@@ -685,79 +723,104 @@ __global__ __launch_bounds__(256) void k_extract_count(ExtractArgs a)
     constexpr int E = B + 1;
     constexpr int NTILE = E * E * E;
     constexpr int NC = EB<B>::NC;
-    constexpr int CPT = EB<B>::CPT;
+    constexpr int CPL = EB<B>::CPL;
     using Entry = typename EB<B>::Entry;
-    __shared__ uint8_t s_below[NTILE];
-    __shared__ uint32_t s_scan[8];
+    __shared__ uint8_t s_all[4][NTILE];
+    __shared__ uint8_t s_ntri[256];
+    for (uint32_t c = threadIdx.x; c < 256u; c += blockDim.x) s_ntri[c] = (uint8_t)mtx_word_ntri(mtx_tri_words[c]);
+    __syncthreads();
+    const uint32_t lane = lane_id();
+    uint8_t* s_below = s_all[threadIdx.x >> 6];
     const T* __restrict__ vol = static_cast<const T*>(a.vol.data);
     const uint32_t X = a.vol.X, Y = a.vol.Y, Z = a.vol.Z;
-    const uint32_t na = *a.count;
     // a volume thinner than 2 voxels on any axis has no cells, hence no triangles
     // and no vertices (every vertex is a corner of some triangle)
     const bool degen = X < 2 || Y < 2 || Z < 2;
+    const uint32_t na = *a.count;
     Entry* __restrict__ table = static_cast<Entry*>(a.corner_table);
-    for (uint32_t k = 0; !item_done(k, na); ++k) {
-        const uint32_t i = item_index(k, na);
-        if (i >= na) break;
+    const WaveItems it = wave_items(na, 4);
+    for (uint32_t i = it.first; i < it.end; i += it.stride) {
         const uint32_t id = a.ids[i];
         const uint32_t bx = id % a.grid.gx, by = (id / a.grid.gx) % a.grid.gy;
         const uint32_t bz = id / (a.grid.gx * a.grid.gy);
         const uint32_t ox = bx * B, oy = by * B, oz = bz * B;
-        for (uint32_t j = threadIdx.x; j < (uint32_t)NTILE; j += 256u) {
-            uint32_t lx = j % E, ly = (j / E) % E, lz = j / (E * E);
-            uint32_t x = ox + lx, y = oy + ly, z = oz + lz;
-            bool in = x < X && y < Y && z < Z;
-            T v = vol[in ? x + (uint64_t)X * (y + (uint64_t)Y * z) : 0];
-            s_below[j] = (uint8_t)(in && vfloat(v) <= a.iso);
-        }
-        __syncthreads();
-        uint32_t fl[CPT];
-        uint32_t nv = 0, nt = 0;
+        // all of a batch's loads are issued before the first is consumed
+        constexpr int LPL = (NTILE + 63) / 64, BATCH = LPL < 16 ? LPL : 16;
 #pragma unroll
-        for (int j = 0; j < CPT; ++j) {
-            const uint32_t c = threadIdx.x * CPT + j;
-            fl[j] = 0;
-            if (c >= (uint32_t)NC) continue;
-            const uint32_t lx = c % B, ly = (c / B) % B, lz = c / (B * B);
-            const uint32_t x = ox + lx, y = oy + ly, z = oz + lz;
-            const uint32_t t0 = lx + E * (ly + E * lz);
-            uint32_t cs = 0;
-            if (!degen && x < X && y < Y && z < Z) {
-                const uint32_t b0 = s_below[t0];
-                uint32_t f = 0;
-                if (x + 1 < X) f |= (b0 ^ s_below[t0 + 1]);
-                if (y + 1 < Y) f |= (b0 ^ s_below[t0 + E]) << 1;
-                if (z + 1 < Z) f |= (b0 ^ s_below[t0 + E * E]) << 2;
-                fl[j] = f;
-                nv += __popc(f);
-                if (x + 1 < X && y + 1 < Y && z + 1 < Z) {
+        for (int k0 = 0; k0 < LPL; k0 += BATCH) {
+            T v[BATCH];
+            bool in[BATCH];
 #pragma unroll
-                    for (uint32_t q = 0; q < 8; ++q)
-                        cs |= (uint32_t)s_below[t0 + mtx_corner_x(q) + E * (mtx_corner_y(q) + E * mtx_corner_z(q))] << q;
-                    nt += mtx_word_ntri(mtx_tri_words[cs]);
-                }
+            for (int k = 0; k < BATCH; ++k) {
+                const uint32_t j = lane + 64u * (k0 + k);
+                const uint32_t x = ox + j % E, y = oy + (j / E) % E, z = oz + j / (E * E);
+                in[k] = j < (uint32_t)NTILE && x < X && y < Y && z < Z;
+                v[k] = vol[in[k] ? x + (uint64_t)X * (y + (uint64_t)Y * z) : 0];
             }
-            if (a.case_out) a.case_out[(uint64_t)i * NC + c] = (uint8_t)cs;
-        }
-        uint32_t ev, et, tv, tt;
-        wg_scan2(nv, nt, ev, et, tv, tt, s_scan);
-        uint32_t run = ev;
 #pragma unroll
-        for (int j = 0; j < CPT; ++j) {
-            const uint32_t c = threadIdx.x * CPT + j;
-            if (c < (uint32_t)NC) table[(uint64_t)i * NC + c] = (Entry)((run << 3) | fl[j]);
+            for (int k = 0; k < BATCH; ++k) {
+                const uint32_t j = lane + 64u * (k0 + k);
+                if (j < (uint32_t)NTILE) s_below[j] = (uint8_t)(in[k] && vfloat(v[k]) <= a.iso);
+            }
+        }
+        wave_lds_sync();
+        uint32_t nv = 0, nt = 0;
+        uint32_t fl[CPL];
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
+            const uint32_t c = lane * CPL + j;
+            fl[j] = 0;
+            uint32_t cs = 0;
+            if (c < (uint32_t)NC) {
+                const uint32_t lx = c % B, ly = (c / B) % B, lz = c / (B * B);
+                const uint32_t x = ox + lx, y = oy + ly, z = oz + lz;
+                const uint32_t t0 = lx + E * (ly + E * lz);
+                if (!degen && x < X && y < Y && z < Z) {
+                    const uint32_t b0 = s_below[t0];
+                    uint32_t f = 0;
+                    if (x + 1 < X) f |= b0 ^ s_below[t0 + 1];
+                    if (y + 1 < Y) f |= (b0 ^ s_below[t0 + E]) << 1;
+                    if (z + 1 < Z) f |= (b0 ^ s_below[t0 + E * E]) << 2;
+                    fl[j] = f;
+                    nv += __popc(f);
+                    if (x + 1 < X && y + 1 < Y && z + 1 < Z) {
+#pragma unroll
+                        for (uint32_t q = 0; q < 8; ++q)
+                            cs |= (uint32_t)s_below[t0 + mtx_corner_x(q) + E * (mtx_corner_y(q) + E * mtx_corner_z(q))] << q;
+                        nt += s_ntri[cs];
+                    }
+                }
+                if (a.case_out) a.case_out[(uint64_t)i * NC + c] = (uint8_t)cs;
+            }
+        }
+        uint32_t tv, tt;
+        uint32_t run = wave_excl_scan<uint32_t>(nv, tv);
+        nt = wave_sum<uint32_t>(nt);
+        tt = nt;
+        Entry ent[CPL];
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
+            ent[j] = (Entry)((run << 3) | fl[j]);
             run += __popc(fl[j]);
         }
-        if (threadIdx.x == 0) {
+        Entry* dst = table + (uint64_t)i * NC + lane * CPL;
+        if constexpr (CPL * sizeof(Entry) == 16) {
+            if (lane * CPL < (uint32_t)NC) *reinterpret_cast<uint4*>(dst) = *reinterpret_cast<const uint4*>(ent);
+        } else {
+#pragma unroll
+            for (int j = 0; j < CPL; ++j)
+                if (lane * CPL + j < (uint32_t)NC) dst[j] = ent[j];
+        }
+        if (lane == 0) {
             a.blk_counts[2 * (uint64_t)i] = tv;
             a.blk_counts[2 * (uint64_t)i + 1] = tt;
         }
-        __syncthreads();
+        wave_lds_sync();
     }
 }
 
-// Exclusive scan of per-block {nv, nt} into vertex/triangle bases.  Scan state:
-// 2 * tiles status words (vertices, then triangles) plus a ticket counter.
+// Exclusive scan of per-block {vertices, triangles, meshlets} into bases.
+// Scan state: 3 * tiles status words plus a ticket counter.
 constexpr uint32_t kScanItems = 4;
 uint32_t scan_tile_items() { return 256 * kScanItems; }
 
@@ -765,61 +828,73 @@ __global__ __launch_bounds__(256) void k_extract_scan(const uint32_t* __restrict
                                                       const uint32_t* __restrict__ count,
                                                       uint64_t* __restrict__ vbase,
                                                       uint64_t* __restrict__ tbase,
+                                                      uint64_t* __restrict__ mbase,
                                                       uint64_t* __restrict__ totals,
                                                       uint32_t* __restrict__ out_counts,
                                                       uint64_t* status, uint32_t tcap,
                                                       uint32_t* ticket)
 {
     __shared__ uint32_t s_tile;
-    __shared__ uint64_t s_w[8];
-    __shared__ uint64_t s_ex[2];
+    __shared__ uint64_t s_w[12];
+    __shared__ uint64_t s_ex[3];
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
     const uint32_t na = *count;
     uint32_t ntiles = (na + 256 * kScanItems - 1) / (256 * kScanItems);
     if (ntiles == 0) ntiles = 1;
+    if (blockIdx.x >= ntiles) return;  // surplus workgroups leave before taking a ticket
     if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
     __syncthreads();
     const uint32_t tile = s_tile;
-    if (tile >= ntiles) return;
-    uint64_t v[kScanItems], t[kScanItems];
-    uint64_t sv = 0, st = 0;
+    uint64_t v[kScanItems], t[kScanItems], m[kScanItems];
+    uint64_t sv = 0, st = 0, sm = 0;
 #pragma unroll
     for (uint32_t k = 0; k < kScanItems; ++k) {
         uint64_t i = (uint64_t)tile * 256 * kScanItems + threadIdx.x * kScanItems + k;
         v[k] = i < na ? cnt[2 * i] : 0;
         t[k] = i < na ? cnt[2 * i + 1] : 0;
+        m[k] = (t[k] + kMeshletPrims - 1) / kMeshletPrims;
         sv += v[k];
         st += t[k];
+        sm += m[k];
     }
     uint64_t iv = wave_incl_scan<uint64_t>(sv);
     uint64_t it = wave_incl_scan<uint64_t>(st);
+    uint64_t im = wave_incl_scan<uint64_t>(sm);
     if (lane == 63) {
         s_w[w] = iv;
         s_w[4 + w] = it;
+        s_w[8 + w] = im;
     }
     __syncthreads();
-    uint64_t ov = 0, ot = 0, av = 0, at = 0;
+    uint64_t ov = 0, ot = 0, om = 0, av = 0, at = 0, am = 0;
 #pragma unroll
     for (uint32_t j = 0; j < 4; ++j) {
         ov += j < w ? s_w[j] : 0;
         ot += j < w ? s_w[4 + j] : 0;
+        om += j < w ? s_w[8 + j] : 0;
         av += s_w[j];
         at += s_w[4 + j];
+        am += s_w[8 + j];
     }
     if (w == 0) {
-        // two independent look-backs: vertex counts in status[0, tcap), triangles in [tcap, 2 tcap)
+        // independent look-backs: vertices in status[0, tcap), triangles in
+        // [tcap, 2 tcap), meshlets in [2 tcap, 3 tcap)
         uint64_t exv = lookback(status, tile, av);
         uint64_t ext = lookback(status + tcap, tile, at);
+        uint64_t exm = lookback(status + 2 * (uint64_t)tcap, tile, am);
         if (lane == 0) {
             s_ex[0] = exv;
             s_ex[1] = ext;
+            s_ex[2] = exm;
             if (tile == ntiles - 1) {
                 totals[0] = exv + av;
                 totals[1] = ext + at;
+                totals[3] = exm + am;
                 if (out_counts) {
-                    uint64_t V = exv + av, I = 3 * (ext + at);
+                    uint64_t V = exv + av, I = 3 * (ext + at), M = exm + am;
                     out_counts[0] = V > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)V;
                     out_counts[1] = I > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)I;
+                    out_counts[2] = M > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)M;
                 }
             }
         }
@@ -827,54 +902,88 @@ __global__ __launch_bounds__(256) void k_extract_scan(const uint32_t* __restrict
     __syncthreads();
     uint64_t rv = s_ex[0] + ov + iv - sv;
     uint64_t rt = s_ex[1] + ot + it - st;
+    uint64_t rm = s_ex[2] + om + im - sm;
 #pragma unroll
     for (uint32_t k = 0; k < kScanItems; ++k) {
         uint64_t i = (uint64_t)tile * 256 * kScanItems + threadIdx.x * kScanItems + k;
         if (i < na) {
             vbase[i] = rv;
             tbase[i] = rt;
+            mbase[i] = rm;
         }
         rv += v[k];
         rt += t[k];
+        rm += m[k];
     }
-}
-
-// Lower-bound search of an ascending ID list.
-__device__ __forceinline__ int32_t find_active(const uint32_t* __restrict__ ids, uint32_t na,
-                                               uint32_t key)
-{
-    uint32_t lo = 0, hi = na;
-    while (lo < hi) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (ids[mid] < key) lo = mid + 1;
-        else hi = mid;
-    }
-    return (lo < na && ids[lo] == key) ? (int32_t)lo : -1;
 }
 
 __device__ __forceinline__ void grad_normal(const float* s, uint32_t t, uint32_t E, float n[3])
 {
-    float gx = s[t - 1] - s[t + 1];
-    float gy = s[t - E] - s[t + E];
-    float gz = s[t - E * E] - s[t + E * E];
-    float l = sqrtf(gx * gx + gy * gy + gz * gz);
-    if (l < 1e-5f) { n[0] = 0.f; n[1] = 1.f; n[2] = 0.f; return; }
-    n[0] = gx / l; n[1] = gy / l; n[2] = gz / l;
+    // normalize(s(p-x)-s(p+x), s(p-y)-s(p+y), s(p-z)-s(p+z)); (0,1,0) where the
+    // gradient vanishes (|g| < 1e-5)
+    const float gx = s[t - 1] - s[t + 1];
+    const float gy = s[t - E] - s[t + E];
+    const float gz = s[t - E * E] - s[t + E * E];
+    const float l2 = gx * gx + gy * gy + gz * gz;
+    if (l2 < 1e-10f) { n[0] = 0.f; n[1] = 1.f; n[2] = 0.f; return; }
+    const float r = rsqrtf(l2);
+    n[0] = gx * r; n[1] = gy * r; n[2] = gz * r;
 }
 
+template <int B, bool NORMALS> struct EmitCfg {
+    static constexpr int A = NORMALS ? 1 : 0;  // apron for central differences
+    static constexpr int E = B + 1 + 2 * A;
+    static constexpr int NTILE = E * E * E;
+    static constexpr int F = B + 1;             // corner table edge: owner corners [0, B]^3
+    static constexpr int NF = F * F * F;
+    static constexpr int LIST = 256;            // work-list chunk (vertices, triangles)
+    static constexpr int WAVE_LDS = NTILE * 4 + NF * 5 + LIST * 6 + 64;
+    // waves per workgroup: the most resident waves per CU (160 KiB of LDS on
+    // gfx950, 2 KiB of case table per workgroup, at most 64 KiB per workgroup)
+    static constexpr int waves_per_cu(int wpg)
+    {
+        return wpg * WAVE_LDS + 2048 > 65536 ? 0 : wpg * (163840 / (wpg * WAVE_LDS + 2048));
+    }
+    static constexpr int WPG = waves_per_cu(4) >= waves_per_cu(2) && waves_per_cu(4) >= waves_per_cu(1)
+                                   ? 4 : (waves_per_cu(2) >= waves_per_cu(1) ? 2 : 1);
+};
+
+// Per wave and block, the emit pass builds in LDS the global index of the
+// first vertex at every owner corner of [0, B]^3 and its 3 edge flags: its own
+// corners from its scan, the corners on the +x/+y/+z faces from the neighbour
+// blocks' corner tables (one batched gather per block).  It then lists the
+// block's vertices (corner, axis) and triangles (cell, case, k) in LDS and
+// lets the 64 lanes take consecutive list entries: the work is balanced across
+// lanes whatever the shape of the surface, and consecutive lanes write
+// consecutive vertices and triangles.
 template <int B, typename T, bool NORMALS>
-__global__ __launch_bounds__(256) void k_extract_emit(ExtractArgs a)
+__global__ __launch_bounds__((64 * EmitCfg<B, NORMALS>::WPG)) __attribute__((amdgpu_waves_per_eu(4)))
+void k_extract_emit(ExtractArgs a)
 {
-    constexpr int A = NORMALS ? 1 : 0;  // apron below/above for central differences
-    constexpr int E = B + 1 + 2 * A;
-    constexpr int NTILE = E * E * E;
+    using C = EmitCfg<B, NORMALS>;
+    constexpr int A = C::A, E = C::E, NTILE = C::NTILE, WPG = C::WPG, F = C::F, NF = C::NF;
+    constexpr uint32_t LIST = C::LIST;
     constexpr int NC = EB<B>::NC;
-    constexpr int CPT = EB<B>::CPT;
+    constexpr int CPL = EB<B>::CPL;
     using Entry = typename EB<B>::Entry;
-    __shared__ float s_val[NTILE];
-    __shared__ Entry s_tab[NC];
-    __shared__ int32_t s_nbr[8];
-    __shared__ uint32_t s_scan[8];
+    __shared__ float s_val_all[WPG][NTILE];
+    __shared__ uint32_t s_gidx_all[WPG][NF];
+    __shared__ uint8_t s_gfl_all[WPG][NF];
+    __shared__ uint16_t s_vl_all[WPG][LIST];
+    __shared__ uint32_t s_tl_all[WPG][LIST];
+    __shared__ int32_t s_nbr_all[WPG][8];
+    __shared__ uint32_t s_nbase_all[WPG][8];
+    __shared__ uint64_t s_words[256];
+    for (uint32_t c = threadIdx.x; c < 256u; c += blockDim.x) s_words[c] = mtx_tri_words[c];
+    __syncthreads();
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    float* s_val = s_val_all[w];
+    uint32_t* s_gidx = s_gidx_all[w];
+    uint8_t* s_gfl = s_gfl_all[w];
+    uint16_t* s_vl = s_vl_all[w];
+    uint32_t* s_tl = s_tl_all[w];
+    int32_t* s_nbr = s_nbr_all[w];
+    uint32_t* s_nbase = s_nbase_all[w];
     const T* __restrict__ vol = static_cast<const T*>(a.vol.data);
     const uint32_t X = a.vol.X, Y = a.vol.Y, Z = a.vol.Z;
     const float iso = a.iso;
@@ -882,38 +991,99 @@ __global__ __launch_bounds__(256) void k_extract_emit(ExtractArgs a)
     const Entry* __restrict__ table = static_cast<const Entry*>(a.corner_table);
     const float inv2x = 2.0f / (float)X, inv2y = 2.0f / (float)Y, inv2z = 2.0f / (float)Z;
     const bool degen = X < 2 || Y < 2 || Z < 2;
-    for (uint32_t k = 0; !item_done(k, na); ++k) {
-        const uint32_t i = item_index(k, na);
-        if (i >= na) break;
+    const WaveItems it = wave_items(na, WPG);
+    for (uint32_t i = it.first; i < it.end; i += it.stride) {
         const uint32_t id = a.ids[i];
         const uint32_t bx = id % a.grid.gx, by = (id / a.grid.gx) % a.grid.gy;
         const uint32_t bz = id / (a.grid.gx * a.grid.gy);
         const uint32_t ox = bx * B, oy = by * B, oz = bz * B;
-        const uint64_t vb = a.blk_vbase[i], tb = a.blk_tbase[i];
+        const uint64_t vb = a.blk_vbase[i], tb = a.blk_tbase[i], mb = a.blk_mbase[i];
         const uint32_t bnv = a.blk_counts[2 * (uint64_t)i], bnt = a.blk_counts[2 * (uint64_t)i + 1];
-        const bool fits = vb + bnv <= a.vcap && 3 * (tb + bnt) <= a.icap;
-        if (threadIdx.x < 8) {
-            const uint32_t n = threadIdx.x;
+        const uint32_t bnm = (bnt + kMeshletPrims - 1) / kMeshletPrims;
+        const bool fits = vb + bnv <= a.vcap && 3 * (tb + bnt) <= a.icap &&
+                          (!a.meshlets || mb + bnm <= a.mcap);
+        if (!fits) {
+            if (lane == 0) atomicOr(reinterpret_cast<unsigned long long*>(&a.totals[2]), 1ull);
+            continue;
+        }
+        if (lane < 8) {
             int32_t r = -1;
-            const uint32_t nx = bx + (n & 1u), ny = by + ((n >> 1) & 1u), nz = bz + (n >> 2);
-            if (n != 0 && n != 7 && nx < a.grid.gx && ny < a.grid.gy && nz < a.grid.gz)
-                r = find_active(a.ids, na, nx + a.grid.gx * (ny + a.grid.gy * nz));
-            s_nbr[n] = r;
+            uint32_t base = 0;
+            const uint32_t nx = bx + (lane & 1u), ny = by + ((lane >> 1) & 1u), nz = bz + (lane >> 2);
+            if (lane != 0 && lane != 7 && nx < a.grid.gx && ny < a.grid.gy && nz < a.grid.gz) {
+                r = (int32_t)a.blk_map[nx + a.grid.gx * (ny + (uint64_t)a.grid.gy * nz)];
+                if (r >= 0) base = (uint32_t)a.blk_vbase[r];
+            }
+            s_nbr[lane] = r;
+            s_nbase[lane] = base;
         }
-        for (uint32_t j = threadIdx.x; j < (uint32_t)NTILE; j += 256u) {
-            int32_t x = (int32_t)(ox + j % E) - A, y = (int32_t)(oy + (j / E) % E) - A;
-            int32_t z = (int32_t)(oz + j / (E * E)) - A;
-            x = min(max(x, 0), (int32_t)X - 1);
-            y = min(max(y, 0), (int32_t)Y - 1);
-            z = min(max(z, 0), (int32_t)Z - 1);
-            s_val[j] = vfloat(vol[(uint32_t)x + (uint64_t)X * ((uint32_t)y + (uint64_t)Y * (uint32_t)z)]);
+        // voxel tile, all of a batch's loads in flight together
+        constexpr int LPL = (NTILE + 63) / 64, BATCH = LPL < 8 ? LPL : 8;
+#pragma unroll 1
+        for (int k0 = 0; k0 < LPL; k0 += BATCH) {
+            T v[BATCH];
+#pragma unroll
+            for (int k = 0; k < BATCH; ++k) {
+                const uint32_t j = min(lane + 64u * (k0 + k), (uint32_t)NTILE - 1u);
+                int32_t x = (int32_t)(ox + j % E) - A, y = (int32_t)(oy + (j / E) % E) - A;
+                int32_t z = (int32_t)(oz + j / (E * E)) - A;
+                x = min(max(x, 0), (int32_t)X - 1);
+                y = min(max(y, 0), (int32_t)Y - 1);
+                z = min(max(z, 0), (int32_t)Z - 1);
+                v[k] = vol[(uint32_t)x + (uint64_t)X * ((uint32_t)y + (uint64_t)Y * (uint32_t)z)];
+            }
+#pragma unroll
+            for (int k = 0; k < BATCH; ++k) {
+                const uint32_t j = lane + 64u * (k0 + k);
+                if (j < (uint32_t)NTILE) s_val[j] = vfloat(v[k]);
+            }
         }
-        __syncthreads();
-        uint32_t fl[CPT], cs[CPT];
+        wave_lds_sync();
+        // neighbour-owned corners (a coordinate equal to B): one batched gather.
+        // Face corner f enumerates x = B (F*F corners), then y = B with x < B
+        // (B*F), then z = B with x, y < B (B*B): F^3 - B^3 in all.
+        {
+            constexpr int NFACE = NF - NC;
+            constexpr int NPL = (NFACE + 63) / 64;
+            Entry ent[NPL];
+            int32_t jn[NPL];
+            uint32_t qs[NPL], nsel[NPL];
+#pragma unroll
+            for (int k = 0; k < NPL; ++k) {
+                const uint32_t f = lane + 64u * k;
+                uint32_t qx, qy, qz;
+                if (f < (uint32_t)(F * F)) {
+                    qx = B; qy = f % F; qz = f / F;
+                } else if (f < (uint32_t)(F * F + B * F)) {
+                    const uint32_t g = f - F * F;
+                    qx = g % B; qy = B; qz = g / B;
+                } else {
+                    const uint32_t g = f - F * F - B * F;
+                    qx = g % B; qy = g / B; qz = B;
+                }
+                const uint32_t nx = qx == (uint32_t)B, ny = qy == (uint32_t)B, nz = qz == (uint32_t)B;
+                nsel[k] = nx | (ny << 1) | (nz << 2);
+                qs[k] = qx + F * (qy + F * qz);
+                jn[k] = f < (uint32_t)NFACE ? s_nbr[nsel[k]] : -1;
+                ent[k] = 0;
+                if (jn[k] >= 0)
+                    ent[k] = table[(uint64_t)jn[k] * NC + (qx - B * nx) + B * ((qy - B * ny) + B * (qz - B * nz))];
+            }
+#pragma unroll
+            for (int k = 0; k < NPL; ++k) {
+                const uint32_t f = lane + 64u * k;
+                if (f < (uint32_t)NFACE) {
+                    s_gidx[qs[k]] = jn[k] >= 0 ? s_nbase[nsel[k]] + (uint32_t)(ent[k] >> 3) : 0u;
+                    s_gfl[qs[k]] = jn[k] >= 0 ? (uint8_t)(ent[k] & 7u) : (uint8_t)0xF0u;
+                }
+            }
+        }
+        // classify my corners/cells
+        uint32_t fl[CPL], cs[CPL];
         uint32_t nv = 0, nt = 0;
 #pragma unroll
-        for (int j = 0; j < CPT; ++j) {
-            const uint32_t c = threadIdx.x * CPT + j;
+        for (int j = 0; j < CPL; ++j) {
+            const uint32_t c = lane * CPL + j;
             fl[j] = 0;
             cs[j] = 0xFFFFFFFFu;
             if (c >= (uint32_t)NC) continue;
@@ -934,114 +1104,125 @@ __global__ __launch_bounds__(256) void k_extract_emit(ExtractArgs a)
                 for (uint32_t q = 0; q < 8; ++q)
                     q8 |= (uint32_t)(s_val[t0 + mtx_corner_x(q) + E * (mtx_corner_y(q) + E * mtx_corner_z(q))] <= iso) << q;
                 cs[j] = q8;
-                nt += mtx_word_ntri(mtx_tri_words[q8]);
+                nt += mtx_word_ntri(s_words[q8]);
             }
         }
-        uint32_t ev, et, tv, tt;
-        wg_scan2(nv, nt, ev, et, tv, tt, s_scan);
+        uint32_t tv, tt;
+        const uint32_t ev = wave_excl_scan<uint32_t>(nv, tv);
+        const uint32_t et = wave_excl_scan<uint32_t>(nt, tt);
         {
-            uint32_t run = ev;
+            uint32_t run = (uint32_t)vb + ev;
 #pragma unroll
-            for (int j = 0; j < CPT; ++j) {
-                const uint32_t c = threadIdx.x * CPT + j;
-                if (c < (uint32_t)NC) s_tab[c] = (Entry)((run << 3) | fl[j]);
+            for (int j = 0; j < CPL; ++j) {
+                const uint32_t c = lane * CPL + j;
+                if (c < (uint32_t)NC) {
+                    const uint32_t q = c % B + F * ((c / B) % B + F * (c / (B * B)));
+                    s_gidx[q] = run;
+                    s_gfl[q] = (uint8_t)fl[j];
+                }
                 run += __popc(fl[j]);
             }
         }
-        __syncthreads();
-        if (fits) {
-            uint64_t rv = vb + ev, rt = tb + et;
+        // vertices, LIST at a time
+        for (uint32_t c0 = 0; c0 < tv; c0 += LIST) {
+            uint32_t run = ev;
 #pragma unroll
-            for (int j = 0; j < CPT; ++j) {
-                const uint32_t c = threadIdx.x * CPT + j;
-                if (c >= (uint32_t)NC) continue;
+            for (int j = 0; j < CPL; ++j) {
+                const uint32_t c = lane * CPL + j;
+#pragma unroll
+                for (uint32_t ax = 0; ax < 3; ++ax)
+                    if ((fl[j] >> ax) & 1u) {
+                        if (run - c0 < LIST) s_vl[run - c0] = (uint16_t)(c | (ax << 12));
+                        ++run;
+                    }
+            }
+            wave_lds_sync();
+            const uint32_t n = min(LIST, tv - c0);
+            for (uint32_t v = lane; v < n; v += 64u) {
+                const uint32_t e = s_vl[v];
+                const uint32_t c = e & 0xFFFu, ax = e >> 12;
                 const uint32_t lx = c % B, ly = (c / B) % B, lz = c / (B * B);
                 const uint32_t x = ox + lx, y = oy + ly, z = oz + lz;
                 const uint32_t t0 = (lx + A) + E * ((ly + A) + E * (lz + A));
-                const uint32_t f = fl[j];
-                if (f) {
-                    float n0[3] = { 0.f, 0.f, 0.f };
-                    if (NORMALS) grad_normal(s_val, t0, E, n0);
-                    const float v0 = s_val[t0];
-#pragma unroll
-                    for (uint32_t ax = 0; ax < 3; ++ax) {
-                        if (!((f >> ax) & 1u)) continue;
-                        const uint32_t st = ax == 0 ? 1u : (ax == 1 ? (uint32_t)E : (uint32_t)(E * E));
-                        const float v1 = s_val[t0 + st];
-                        float t = (iso - v0) / (v1 - v0);
-                        t = fminf(fmaxf(t, 0.0f), 1.0f);
-                        const float px = (float)x + (ax == 0 ? t : 0.0f);
-                        const float py = (float)y + (ax == 1 ? t : 0.0f);
-                        const float pz = (float)z + (ax == 2 ? t : 0.0f);
-                        float4 pos = make_float4(px * inv2x - 1.0f, py * inv2y - 1.0f, pz * inv2z - 1.0f, 1.0f);
-                        float4 nrm = make_float4(0.f, 0.f, 0.f, 0.f);
-                        if (NORMALS) {
-                            float n1[3];
-                            grad_normal(s_val, t0 + st, E, n1);
-                            float m0 = n0[0] * (1.0f - t) + n1[0] * t;
-                            float m1 = n0[1] * (1.0f - t) + n1[1] * t;
-                            float m2 = n0[2] * (1.0f - t) + n1[2] * t;
-                            float l = sqrtf(m0 * m0 + m1 * m1 + m2 * m2);
-                            if (l < 1e-5f) { m0 = n0[0]; m1 = n0[1]; m2 = n0[2]; l = sqrtf(m0 * m0 + m1 * m1 + m2 * m2); }
-                            nrm = make_float4(m0 / l, m1 / l, m2 / l, 0.0f);
-                        }
-                        float4* dst = reinterpret_cast<float4*>(a.vertices) + 2 * rv;
-                        dst[0] = pos;
-                        dst[1] = nrm;
-                        if (a.edge_ids) a.edge_ids[rv] = 3ull * (x + (uint64_t)X * (y + (uint64_t)Y * z)) + ax;
-                        if (a.edge_t) a.edge_t[rv] = t;
-                        if (a.vox_pos) {
-                            a.vox_pos[3 * rv + 0] = px;
-                            a.vox_pos[3 * rv + 1] = py;
-                            a.vox_pos[3 * rv + 2] = pz;
-                        }
-                        ++rv;
-                    }
+                const uint32_t st = ax == 0 ? 1u : (ax == 1 ? (uint32_t)E : (uint32_t)(E * E));
+                const float v0 = s_val[t0], v1 = s_val[t0 + st];
+                float t = (iso - v0) / (v1 - v0);
+                t = fminf(fmaxf(t, 0.0f), 1.0f);
+                const float px = (float)x + (ax == 0 ? t : 0.0f);
+                const float py = (float)y + (ax == 1 ? t : 0.0f);
+                const float pz = (float)z + (ax == 2 ? t : 0.0f);
+                const float4 pos = make_float4(px * inv2x - 1.0f, py * inv2y - 1.0f, pz * inv2z - 1.0f, 1.0f);
+                float4 nrm = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (NORMALS) {
+                    float n0[3], n1[3];
+                    grad_normal(s_val, t0, E, n0);
+                    grad_normal(s_val, t0 + st, E, n1);
+                    float m0 = n0[0] * (1.0f - t) + n1[0] * t;
+                    float m1 = n0[1] * (1.0f - t) + n1[1] * t;
+                    float m2 = n0[2] * (1.0f - t) + n1[2] * t;
+                    float l2 = m0 * m0 + m1 * m1 + m2 * m2;
+                    if (l2 < 1e-10f) { m0 = n0[0]; m1 = n0[1]; m2 = n0[2]; l2 = m0 * m0 + m1 * m1 + m2 * m2; }
+                    const float r = rsqrtf(l2);
+                    nrm = make_float4(m0 * r, m1 * r, m2 * r, 0.0f);
                 }
-                if (cs[j] != 0xFFFFFFFFu) {
-                    const uint64_t wd = mtx_tri_words[cs[j]];
-                    const uint32_t ntri = mtx_word_ntri(wd);
-                    for (uint32_t q = 0; q < ntri; ++q) {
-                        uint32_t idx3[3];
-#pragma unroll
-                        for (uint32_t e3 = 0; e3 < 3; ++e3) {
-                            const uint32_t o = mtx_edge_owner(mtx_word_edge(wd, q, e3));
-                            const uint32_t qx = lx + (o & 1u), qy = ly + ((o >> 1) & 1u), qz = lz + ((o >> 2) & 1u);
-                            const uint32_t ax = o >> 3;
-                            uint32_t ent;
-                            uint64_t base;
-                            if (qx < (uint32_t)B && qy < (uint32_t)B && qz < (uint32_t)B) {
-                                ent = s_tab[qx + B * (qy + B * qz)];
-                                base = vb;
-                            } else {
-                                const uint32_t nx = qx >= (uint32_t)B, ny = qy >= (uint32_t)B, nz = qz >= (uint32_t)B;
-                                const int32_t jn = s_nbr[nx | (ny << 1) | (nz << 2)];
-                                if (jn < 0) {
-                                    // an active edge whose owner block is not in the active
-                                    // list: the list is inconsistent with this volume/isovalue
-                                    atomicOr(reinterpret_cast<unsigned long long*>(&a.totals[2]), 2ull);
-                                    ent = 0;
-                                    base = 0;
-                                } else {
-                                    const uint32_t cx = qx - B * nx, cy = qy - B * ny, cz = qz - B * nz;
-                                    ent = table[(uint64_t)jn * NC + cx + B * (cy + B * cz)];
-                                    base = a.blk_vbase[jn];
-                                }
-                            }
-                            idx3[e3] = (uint32_t)(base + (ent >> 3) + __popc(ent & 7u & ((1u << ax) - 1u)));
-                        }
-                        uint32_t* dst = a.indices + 3 * rt;
-                        dst[0] = idx3[0];
-                        dst[1] = idx3[1];
-                        dst[2] = idx3[2];
-                        ++rt;
-                    }
+                const uint64_t gv = vb + c0 + v;
+                float4* dst = reinterpret_cast<float4*>(a.vertices) + 2 * gv;
+                dst[0] = pos;
+                dst[1] = nrm;
+                if (a.edge_ids) a.edge_ids[gv] = 3ull * (x + (uint64_t)X * (y + (uint64_t)Y * z)) + ax;
+                if (a.edge_t) a.edge_t[gv] = t;
+                if (a.vox_pos) {
+                    a.vox_pos[3 * gv + 0] = px;
+                    a.vox_pos[3 * gv + 1] = py;
+                    a.vox_pos[3 * gv + 2] = pz;
                 }
             }
-        } else if (threadIdx.x == 0) {
-            atomicOr(reinterpret_cast<unsigned long long*>(&a.totals[2]), 1ull);
+            wave_lds_sync();
         }
-        __syncthreads();
+        // triangles, LIST at a time
+        for (uint32_t c0 = 0; c0 < tt; c0 += LIST) {
+            uint32_t run = et;
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) {
+                if (cs[j] == 0xFFFFFFFFu) continue;
+                const uint32_t c = lane * CPL + j;
+                const uint32_t ntri = mtx_word_ntri(s_words[cs[j]]);
+                for (uint32_t k = 0; k < ntri; ++k, ++run)
+                    if (run - c0 < LIST) s_tl[run - c0] = c | (k << 12) | (cs[j] << 16);
+            }
+            wave_lds_sync();
+            const uint32_t n = min(LIST, tt - c0);
+            for (uint32_t q = lane; q < n; q += 64u) {
+                const uint32_t e = s_tl[q];
+                const uint32_t c = e & 0xFFFu, k = (e >> 12) & 7u;
+                const uint64_t wd = s_words[e >> 16];
+                const uint32_t q0 = c % B + F * ((c / B) % B + F * (c / (B * B)));
+                uint32_t idx3[3];
+#pragma unroll
+                for (uint32_t e3 = 0; e3 < 3; ++e3) {
+                    const uint32_t o = mtx_edge_owner(mtx_word_edge(wd, k, e3));
+                    const uint32_t qq = q0 + (o & 1u) + F * (((o >> 1) & 1u) + F * ((o >> 2) & 1u));
+                    const uint32_t ax = o >> 3;
+                    const uint32_t gf = s_gfl[qq];
+                    if (gf & 0xF0u) {
+                        // an active edge whose owner block is not in the active list:
+                        // the list does not belong to this volume/isovalue
+                        atomicOr(reinterpret_cast<unsigned long long*>(&a.totals[2]), 2ull);
+                    }
+                    idx3[e3] = s_gidx[qq] + __popc(gf & 7u & ((1u << ax) - 1u));
+                }
+                *reinterpret_cast<uint3*>(a.indices + 3 * (tb + c0 + q)) = make_uint3(idx3[0], idx3[1], idx3[2]);
+            }
+            wave_lds_sync();
+        }
+        if (a.meshlets) {
+            for (uint32_t k = lane; k < bnm; k += 64u) {
+                const uint32_t prims = min(kMeshletPrims, bnt - k * kMeshletPrims);
+                a.meshlets[mb + k] = make_uint4((uint32_t)vb, (uint32_t)(3 * (tb + (uint64_t)k * kMeshletPrims)),
+                                                bnv, prims);
+            }
+        }
+        wave_lds_sync();
     }
 }
 
@@ -1058,21 +1239,25 @@ static hipError_t launch_count_b(const ExtractArgs& a, hipStream_t s, uint32_t n
 template <int B>
 static hipError_t launch_emit_b(const ExtractArgs& a, hipStream_t s, uint32_t nwg)
 {
+    const uint32_t tn = 64 * EmitCfg<B, true>::WPG, tf = 64 * EmitCfg<B, false>::WPG;
+    const uint32_t gn = nwg * (4 / EmitCfg<B, true>::WPG), gf = nwg * (4 / EmitCfg<B, false>::WPG);
     if (a.vol.dtype == kF32) {
-        if (a.normals) hipLaunchKernelGGL((k_extract_emit<B, float, true>), dim3(nwg), dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((k_extract_emit<B, float, false>), dim3(nwg), dim3(256), 0, s, a);
+        if (a.normals) hipLaunchKernelGGL((k_extract_emit<B, float, true>), dim3(gn), dim3(tn), 0, s, a);
+        else hipLaunchKernelGGL((k_extract_emit<B, float, false>), dim3(gf), dim3(tf), 0, s, a);
     } else {
-        if (a.normals) hipLaunchKernelGGL((k_extract_emit<B, uint8_t, true>), dim3(nwg), dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((k_extract_emit<B, uint8_t, false>), dim3(nwg), dim3(256), 0, s, a);
+        if (a.normals) hipLaunchKernelGGL((k_extract_emit<B, uint8_t, true>), dim3(gn), dim3(tn), 0, s, a);
+        else hipLaunchKernelGGL((k_extract_emit<B, uint8_t, false>), dim3(gf), dim3(tf), 0, s, a);
     }
     return hipGetLastError();
 }
 
-// Persistent grid: enough workgroups to fill the chip, never more than blocks.
+// Persistent grid: enough waves to fill the chip (and a multiple of 8
+// workgroups for the XCD split), never far more than there are blocks.
 static uint32_t persistent_wgs(uint32_t max_blocks)
 {
     uint32_t n = 2048;
-    if (max_blocks < n) n = max_blocks < 8 ? 8 : ((max_blocks + 7) / 8) * 8;
+    const uint32_t need = (max_blocks + 3) / 4;
+    if (need < n) n = need < 8 ? 8 : ((need + 7) / 8) * 8;
     return n;
 }
 
@@ -1115,8 +1300,8 @@ hipError_t launch_extract_scan(const ExtractArgs& a, ScanState st, hipStream_t s
     if (ntiles == 0) ntiles = 1;
     if (ntiles > 0x7FFFFFFFull) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_extract_scan, dim3((uint32_t)ntiles), dim3(256), 0, s, a.blk_counts,
-                       a.count, a.blk_vbase, a.blk_tbase, a.totals, a.out_counts, st.status,
-                       (uint32_t)ntiles, st.ticket);
+                       a.count, a.blk_vbase, a.blk_tbase, a.blk_mbase, a.totals, a.out_counts,
+                       st.status, (uint32_t)ntiles, st.ticket);
     return hipGetLastError();
 }
 

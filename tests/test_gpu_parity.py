@@ -270,3 +270,44 @@ def test_extract_reuse_mode_and_capacity(ctx):
         from meshtrex_amd import MtxError
         with pytest.raises(MtxError, match="MTX_ECAPACITY"):
             ex.read_counts()
+
+
+def test_block_map_and_meshlets(ctx):
+    """Dense block -> active-index map, and meshlet descriptors: each active
+    block's triangles in runs of <= 126, over the block's own vertex range."""
+    dims = (90, 77, 61)
+    data = oracle.gen_blobs_u8(dims, 91, 10)
+    B, iso = 8, 128.0
+    for fuse in (1, 0):
+        mm, pc = run_minmax(ctx, data, dims, B, iso=iso, fuse_filter=fuse)
+        f = filterActiveBlocks(ctx, mm, pc)
+        ids = f.read_ids()
+        G = oracle.grid(dims, B)
+        nb = G[0] * G[1] * G[2]
+        from meshtrex_amd.managers import read_device
+        m = read_device(f.c.block_to_active, np.uint32, nb)
+        want = np.full(nb, 0xFFFFFFFF, np.uint32)
+        want[ids] = np.arange(ids.size, dtype=np.uint32)
+        np.testing.assert_array_equal(m, want)
+    ex = extractMeshletDescriptors(ctx, mm, f, pc, options=makeOptions(edge_ids=1))
+    nv, ni, nm = ex.read_counts()
+    ref = oracle.extract(data, dims, B, iso, normals=False)
+    # per-block vertex counts from owning edges, triangle counts from case indices
+    vox = ref["edge_ids"].astype(np.int64) // 3
+    X, Y = dims[0], dims[1]
+    owner_block = ((vox % X) // B) + G[0] * (((vox // X) % Y) // B + G[1] * ((vox // (X * Y)) // B))
+    pos = np.searchsorted(ids, owner_block)
+    assert np.all(ids[pos] == owner_block)
+    bnv = np.bincount(pos, minlength=ids.size)
+    ntri = np.array([int(np.sum(r >= 0)) // 3 for r in oracle.tri_table()])
+    bnt = ntri[oracle.case_indices(data, dims, B, iso, ids).reshape(ids.size, -1)].sum(axis=1)
+    vb = np.concatenate([[0], np.cumsum(bnv)[:-1]])
+    tb = np.concatenate([[0], np.cumsum(bnt)[:-1]])
+    exp = []
+    for i in range(ids.size):
+        for k in range(-(-int(bnt[i]) // 126)):
+            exp.append((vb[i], 3 * (tb[i] + 126 * k), bnv[i], min(126, bnt[i] - 126 * k)))
+    from meshtrex_amd.managers import read_device
+    got = read_device(ex.c.meshlets, np.uint32, 4 * nm).reshape(-1, 4)
+    assert nm == len(exp)
+    np.testing.assert_array_equal(got, np.array(exp, np.uint32).reshape(-1, 4))
