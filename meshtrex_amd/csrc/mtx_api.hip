@@ -301,7 +301,7 @@ mtx_status mtx_compute_min_max(mtx_context* ctx, const mtx_volume* volume,
     ActParams act = make_act((int)volume->dtype, pc->isovalue, (int)o.rule);
     hipError_t e;
     if (fast)
-        e = launch_minmax_b8(v, g, out->minmax, fuse ? out->active_mask : nullptr, &act, nullptr, 0, bd.s);
+        e = launch_minmax_b8(v, g, out->minmax, fuse ? out->active_mask : nullptr, &act, bd.s);
     else
         e = launch_minmax_generic(v, g, (int)o.overlap, out->minmax, bd.s);
     if (e == hipSuccess && o.build_pyramid && out->num_levels > 1) {

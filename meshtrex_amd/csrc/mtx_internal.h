@@ -41,11 +41,8 @@ struct ScanState {
 
 // Block min/max, B=8, overlap 1, coalesced wave64 kernel.  Optionally writes
 // the activity bit mask (one 32-bit word per 32 x-blocks) for `act`.
-// zero/zero_words: a buffer the kernel clears on the way (scan state of the
-// following compaction), may be null.
 hipError_t launch_minmax_b8(const VolDesc& v, const Grid& g, uint32_t* pairs, uint32_t* mask,
-                            const ActParams* act, uint64_t* zero, uint32_t zero_words,
-                            hipStream_t s);
+                            const ActParams* act, hipStream_t s);
 
 // Block min/max for any B, overlap 0 or 1 (one wave per block).
 hipError_t launch_minmax_generic(const VolDesc& v, const Grid& g, int overlap, uint32_t* pairs,

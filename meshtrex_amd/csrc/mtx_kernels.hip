@@ -145,16 +145,19 @@ hipError_t launch_zero64(uint64_t* p, uint64_t n, hipStream_t s)
 // ---------------------------------------------------------------------------
 // block min/max, B = 8, overlap 1: the roofline kernel
 //
-// A workgroup (4 waves) owns 32 x-blocks x 4 y-blocks x zc z-blocks.  Lane l of
-// wave w reads 4 consecutive voxels (16 B for f32, 4 B for u8) of every row of
-// y-block 4*gy+w, so a wave issues one fully coalesced row segment per load.
-// Lanes 2k, 2k+1 cover x-block k; its overlap column x = 8k+8 is lane 2k+2's
-// first voxel (the last block of the tile takes it from one wave-uniform
-// load).  The z sweep carries the shared plane 8bz+8 from block bz into bz+1,
-// so every plane is read once; the y overlap row is the next wave's first row
-// and hits L2.  Out-of-window rows/planes are clamped onto the last in-window
-// row/plane: re-reading it cannot change a min/max, and it keeps every load
-// unconditional so all 9 rows of a plane are in flight together.
+// Lane l of a wave reads 4 consecutive voxels (16 B for f32, 4 B for u8) of a
+// row, so one wave load is a fully coalesced 1 KiB (f32) row segment covering
+// 32 x-blocks.  Lanes 2k, 2k+1 cover x-block k; its overlap column x = 8k+8 is
+// lane 2k+2's first voxel (the last block of the segment takes it from one
+// wave-uniform load).  A wave walks M consecutive y-blocks within each plane,
+// carrying the shared row 8(by+1) from block by into by+1, and sweeps a chunk
+// of zc z-blocks, carrying the shared plane 8(bz+1) likewise: inside a wave
+// every voxel is read once; only the rows/planes at wave boundaries are read
+// twice (1/(8M) + 1/(8zc) of the volume).  The four waves of a workgroup take
+// adjacent x segments (4 KiB contiguous per row) or, for narrow volumes,
+// adjacent y groups.  Out-of-window rows/planes are clamped onto the last
+// in-window one: re-reading it cannot change a min/max, and it keeps every
+// load unconditional so a batch of 8 rows is in flight together.
 // ---------------------------------------------------------------------------
 
 struct MMAcc {
@@ -162,12 +165,6 @@ struct MMAcc {
     uint32_t emin, emax;  // first voxel of this lane (overlap column for lane-2)
     uint32_t xmin, xmax;  // wave-uniform extra column
 };
-
-__device__ __forceinline__ void mm_neutral(MMAcc& a)
-{
-    a.amin = a.emin = a.xmin = 0xFFFFFFFFu;
-    a.amax = a.emax = a.xmax = 0u;
-}
 
 __device__ __forceinline__ void mm_merge(MMAcc& a, const MMAcc& b)
 {
@@ -180,72 +177,99 @@ template <typename T> struct Vec4;
 template <> struct Vec4<float> { using type = float4; };
 template <> struct Vec4<uint8_t> { using type = uchar4; };
 
+// Keys of this lane's 4 voxels of a row (neutral-min for voxels past X) and of
+// the extra column.
 template <typename T, bool ALIGNED>
-__device__ __forceinline__ void mm_plane(MMAcc& acc, const T* __restrict__ vol, uint64_t zoff,
-                                         uint32_t X, uint32_t y0, uint32_t ylast, uint32_t x0,
-                                         bool lane_ok, uint32_t xe, bool xe_ok)
+__device__ __forceinline__ void mm_load(const T* __restrict__ row, uint32_t X, uint32_t x0, bool lane_ok,
+                                        uint32_t xe, bool xe_ok, uint32_t (&k)[4], uint32_t& kx)
 {
     using V4 = typename Vec4<T>::type;
-    uint32_t k[9][4];
-    uint32_t kx[9];
+    if (ALIGNED) {
+        V4 v = *reinterpret_cast<const V4*>(row + (lane_ok ? x0 : 0u));
+        k[0] = vkey(v.x); k[1] = vkey(v.y); k[2] = vkey(v.z); k[3] = vkey(v.w);
+        if (!lane_ok) k[0] = k[1] = k[2] = k[3] = 0xFFFFFFFFu;
+    } else {
 #pragma unroll
-    for (int r = 0; r < 9; ++r) {
-        uint32_t y = min(y0 + (uint32_t)r, ylast);
-        const T* row = vol + zoff + (uint64_t)y * X;
-        if (ALIGNED) {
-            V4 v = *reinterpret_cast<const V4*>(row + (lane_ok ? x0 : 0u));
-            k[r][0] = vkey(v.x); k[r][1] = vkey(v.y); k[r][2] = vkey(v.z); k[r][3] = vkey(v.w);
-            if (!lane_ok) { k[r][0] = k[r][1] = k[r][2] = k[r][3] = 0xFFFFFFFFu; }
-        } else {
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                bool ok = x0 + j < X;
-                k[r][j] = vkey(row[ok ? x0 + j : 0u]);
-                if (!ok) k[r][j] = 0xFFFFFFFFu;
-            }
+        for (int j = 0; j < 4; ++j) {
+            const bool ok = x0 + j < X;
+            k[j] = vkey(row[ok ? x0 + j : 0u]);
+            if (!ok) k[j] = 0xFFFFFFFFu;
         }
-        kx[r] = vkey(row[xe_ok ? xe : 0u]);
     }
+    kx = vkey(row[xe_ok ? xe : 0u]);
+}
+
+template <bool ALIGNED>
+__device__ __forceinline__ MMAcc mm_row(const uint32_t (&k)[4], uint32_t kx, uint32_t X, uint32_t x0,
+                                        bool lane_ok, bool xe_ok)
+{
+    // neutral (all-ones) entries must not raise the max: mask them out
+    const bool ok0 = ALIGNED ? lane_ok : (x0 < X);
+    const bool ok1 = ALIGNED ? lane_ok : (x0 + 1 < X);
+    const bool ok2 = ALIGNED ? lane_ok : (x0 + 2 < X);
+    const bool ok3 = ALIGNED ? lane_ok : (x0 + 3 < X);
+    MMAcc a;
+    a.amin = min(min(k[0], k[1]), min(k[2], k[3]));
+    a.amax = max(max(ok0 ? k[0] : 0u, ok1 ? k[1] : 0u), max(ok2 ? k[2] : 0u, ok3 ? k[3] : 0u));
+    a.emin = k[0];
+    a.emax = ok0 ? k[0] : 0u;
+    a.xmin = xe_ok ? kx : 0xFFFFFFFFu;
+    a.xmax = xe_ok ? kx : 0u;
+    return a;
+}
+
+// One plane for M y-blocks starting at row y0: out[k] = (merge ? out[k] (+) : ) rows of block k.
+template <typename T, bool ALIGNED, int M>
+__device__ __forceinline__ void mm_plane(MMAcc (&out)[M], bool merge, const T* __restrict__ vol,
+                                         uint64_t zoff, uint32_t X, uint32_t Y, uint32_t y0,
+                                         uint32_t x0, bool lane_ok, uint32_t xe, bool xe_ok)
+{
+    uint32_t k0[4], kx0;
+    mm_load<T, ALIGNED>(vol + zoff + (uint64_t)min(y0, Y - 1u) * X, X, x0, lane_ok, xe, xe_ok, k0, kx0);
+    MMAcc carry = mm_row<ALIGNED>(k0, kx0, X, x0, lane_ok, xe_ok);
 #pragma unroll
-    for (int r = 0; r < 9; ++r) {
-        // neutral (all-ones) entries must not raise the max: mask them out
-        bool ok0 = ALIGNED ? lane_ok : (x0 < X);
-        bool ok1 = ALIGNED ? lane_ok : (x0 + 1 < X);
-        bool ok2 = ALIGNED ? lane_ok : (x0 + 2 < X);
-        bool ok3 = ALIGNED ? lane_ok : (x0 + 3 < X);
-        acc.amin = min(acc.amin, min(min(k[r][0], k[r][1]), min(k[r][2], k[r][3])));
-        uint32_t m0 = ok0 ? k[r][0] : 0u, m1 = ok1 ? k[r][1] : 0u;
-        uint32_t m2 = ok2 ? k[r][2] : 0u, m3 = ok3 ? k[r][3] : 0u;
-        acc.amax = max(acc.amax, max(max(m0, m1), max(m2, m3)));
-        acc.emin = min(acc.emin, k[r][0]);
-        acc.emax = max(acc.emax, m0);
-        if (xe_ok) {
-            acc.xmin = min(acc.xmin, kx[r]);
-            acc.xmax = max(acc.xmax, kx[r]);
+    for (int b = 0; b < M; ++b) {
+        const uint32_t yb = y0 + 8u * b;
+        const uint32_t ylast = min(yb + 8u, Y - 1u);
+        uint32_t k[8][4], kx[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const uint32_t y = min(yb + 1u + r, ylast);
+            mm_load<T, ALIGNED>(vol + zoff + (uint64_t)y * X, X, x0, lane_ok, xe, xe_ok, k[r], kx[r]);
         }
+        MMAcc a = carry;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const MMAcc row = mm_row<ALIGNED>(k[r], kx[r], X, x0, lane_ok, xe_ok);
+            mm_merge(a, row);
+            if (r == 7) carry = row;
+        }
+        if (merge) mm_merge(out[b], a);
+        else out[b] = a;
+        // one batch of 8 rows in flight at a time: keeps the register budget at
+        // 4+ waves/SIMD instead of hoisting every block's loads
+        __builtin_amdgcn_sched_barrier(0);
     }
 }
 
-template <typename T, bool ALIGNED, bool MASK>
+template <typename T, bool ALIGNED, bool MASK, int M>
 __global__ __launch_bounds__(256) void k_minmax_b8(
     const T* __restrict__ vol, uint32_t X, uint32_t Y, uint32_t Z, uint32_t gx, uint32_t gy,
-    uint32_t gz, uint32_t ntx, uint32_t ngy, uint32_t zc, uint2* __restrict__ pairs,
-    uint32_t* __restrict__ mask, uint32_t wpr, ActParams act, uint64_t* __restrict__ zero,
-    uint32_t zero_words)
+    uint32_t gz, uint32_t ntx, uint32_t wx, uint32_t ntxg, uint32_t nygrp, uint32_t zc,
+    uint2* __restrict__ pairs, uint32_t* __restrict__ mask, uint32_t wpr, ActParams act)
 {
-    if (zero) {
-        for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < zero_words; i += gridDim.x * 256u)
-            zero[i] = 0;
-    }
+    // tile order: x groups fastest, then y groups, then z chunks; XCD-contiguous
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
-    const uint32_t tgy = tile % ngy;
-    const uint32_t rest = tile / ngy;
-    const uint32_t tx = rest % ntx;
-    const uint32_t cz = rest / ntx;
+    const uint32_t txg = tile % ntxg;
+    const uint32_t rest = tile / ntxg;
+    const uint32_t ygrp = rest % nygrp;
+    const uint32_t cz = rest / nygrp;
     const uint32_t lane = lane_id();
     const uint32_t w = threadIdx.x >> 6;
-    const uint32_t by = tgy * 4u + w;
-    if (by >= gy) return;
+    const uint32_t wy = 4u / wx;
+    const uint32_t tx = txg * wx + w % wx;
+    const uint32_t by0 = (ygrp * wy + w / wx) * (uint32_t)M;
+    if (tx >= ntx || by0 >= gy) return;
     const uint32_t bz0 = cz * zc;
     if (bz0 >= gz) return;
     const uint32_t bz1 = min(bz0 + zc, gz);
@@ -254,50 +278,50 @@ __global__ __launch_bounds__(256) void k_minmax_b8(
     const bool lane_ok = x0 < X;
     const uint32_t xe = tx * 256u + 256u;
     const bool xe_ok = xe < X;
-    const uint32_t y0 = by * 8u;
-    const uint32_t ylast = min(y0 + 8u, Y - 1u);
+    const uint32_t y0 = by0 * 8u;
     const uint64_t XY = (uint64_t)X * Y;
 
-    MMAcc first;
-    mm_neutral(first);
-    mm_plane<T, ALIGNED>(first, vol, (uint64_t)(bz0 * 8u) * XY, X, y0, ylast, x0, lane_ok, xe,
-                         xe_ok);
+    MMAcc first[M];
+    mm_plane<T, ALIGNED, M>(first, false, vol, (uint64_t)(bz0 * 8u) * XY, X, Y, y0, x0, lane_ok, xe, xe_ok);
 
     const uint32_t bx = tx * 32u + (lane >> 1);
     for (uint32_t bz = bz0; bz < bz1; ++bz) {
-        MMAcc acc = first;
+        MMAcc acc[M];
+#pragma unroll
+        for (int b = 0; b < M; ++b) acc[b] = first[b];
         const uint32_t zl = min(bz * 8u + 8u, Z - 1u);
 #pragma unroll 1
-        for (uint32_t dz = 1; dz <= 8u; ++dz) {
-            uint32_t z = min(bz * 8u + dz, zl);
-            MMAcc p;
-            mm_neutral(p);
-            mm_plane<T, ALIGNED>(p, vol, (uint64_t)z * XY, X, y0, ylast, x0, lane_ok, xe, xe_ok);
-            mm_merge(acc, p);
-            if (dz == 8u) first = p;
-        }
-        // x combine: pair (2k, 2k+1) plus lane 2k+2's first voxel
-        uint32_t bmin = min(acc.amin, __shfl_xor(acc.amin, 1, 64));
-        uint32_t bmax = max(acc.amax, __shfl_xor(acc.amax, 1, 64));
-        uint32_t src = min(lane + 2u, 63u);
-        uint32_t nmin = __shfl(acc.emin, (int)src, 64);
-        uint32_t nmax = __shfl(acc.emax, (int)src, 64);
-        if (lane >= 62u) { nmin = acc.xmin; nmax = acc.xmax; }
-        bmin = min(bmin, nmin);
-        bmax = max(bmax, nmax);
-        const bool writer = !(lane & 1u) && bx < gx;
-        const uint64_t id = bx + (uint64_t)gx * (by + (uint64_t)gy * bz);
-        if (writer) pairs[id] = make_uint2(bmin, bmax);
-        if (MASK) {
-            uint64_t bal = __ballot(writer && block_active(bmin, bmax, act));
-            // compress the even bits (one per x-block) into 32 bits
-            uint64_t x = bal & 0x5555555555555555ull;
-            x = (x | (x >> 1)) & 0x3333333333333333ull;
-            x = (x | (x >> 2)) & 0x0F0F0F0F0F0F0F0Full;
-            x = (x | (x >> 4)) & 0x00FF00FF00FF00FFull;
-            x = (x | (x >> 8)) & 0x0000FFFF0000FFFFull;
-            x = (x | (x >> 16)) & 0x00000000FFFFFFFFull;
-            if (lane == 0) mask[((uint64_t)bz * gy + by) * wpr + tx] = (uint32_t)x;
+        for (uint32_t dz = 1; dz < 8u; ++dz)
+            mm_plane<T, ALIGNED, M>(acc, true, vol, (uint64_t)min(bz * 8u + dz, zl) * XY, X, Y, y0, x0,
+                                    lane_ok, xe, xe_ok);
+        mm_plane<T, ALIGNED, M>(first, false, vol, (uint64_t)zl * XY, X, Y, y0, x0, lane_ok, xe, xe_ok);
+#pragma unroll
+        for (int b = 0; b < M; ++b) {
+            mm_merge(acc[b], first[b]);
+            const uint32_t by = by0 + b;
+            // x combine: pair (2k, 2k+1) plus lane 2k+2's first voxel
+            uint32_t bmin = min(acc[b].amin, __shfl_xor(acc[b].amin, 1, 64));
+            uint32_t bmax = max(acc[b].amax, __shfl_xor(acc[b].amax, 1, 64));
+            const uint32_t src = min(lane + 2u, 63u);
+            uint32_t nmin = __shfl(acc[b].emin, (int)src, 64);
+            uint32_t nmax = __shfl(acc[b].emax, (int)src, 64);
+            if (lane >= 62u) { nmin = acc[b].xmin; nmax = acc[b].xmax; }
+            bmin = min(bmin, nmin);
+            bmax = max(bmax, nmax);
+            const bool writer = !(lane & 1u) && bx < gx && by < gy;
+            const uint64_t id = bx + (uint64_t)gx * (by + (uint64_t)gy * bz);
+            if (writer) pairs[id] = make_uint2(bmin, bmax);
+            if (MASK && by < gy) {
+                uint64_t bal = __ballot(writer && block_active(bmin, bmax, act));
+                // compress the even bits (one per x-block) into 32 bits
+                uint64_t x = bal & 0x5555555555555555ull;
+                x = (x | (x >> 1)) & 0x3333333333333333ull;
+                x = (x | (x >> 2)) & 0x0F0F0F0F0F0F0F0Full;
+                x = (x | (x >> 4)) & 0x00FF00FF00FF00FFull;
+                x = (x | (x >> 8)) & 0x0000FFFF0000FFFFull;
+                x = (x | (x >> 16)) & 0x00000000FFFFFFFFull;
+                if (lane == 0) mask[((uint64_t)bz * gy + by) * wpr + tx] = (uint32_t)x;
+            }
         }
     }
 }
@@ -312,56 +336,95 @@ static int device_cus()
     return n;
 }
 
-template <typename T, bool MASK>
-static hipError_t launch_mm_b8_t(const VolDesc& v, const Grid& g, uint32_t* pairs, uint32_t* mask,
-                                 const ActParams* act, uint64_t* zero, uint32_t zero_words,
-                                 hipStream_t s)
+struct MMPlan {
+    uint32_t M, wx, ntxg, nygrp, zc, ncz;
+    uint64_t nwg;
+};
+
+// Pick y-blocks per wave (M) and the z chunk: every workgroup resident in one
+// round (no tail), the fewest voxels read twice (1/(8M) + 1/(8zc)).
+static MMPlan plan_minmax(const Grid& g, uint32_t ntx, const int (&resident)[3])
 {
-    const uint32_t ntx = (g.gx + 31u) / 32u;
-    const uint32_t ngy = (g.gy + 3u) / 4u;
-    const uint64_t cols = (uint64_t)ntx * ngy;
-    // z-chunk: the shortest that keeps every workgroup resident in one round
-    // (no tail of late workgroups); longer chunks re-read fewer shared planes
-    static int resident = 0;
-    if (!resident) {
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_minmax_b8<T, true, MASK>, 256, 0);
-        if (resident < 1) resident = 1;
+    const uint32_t wx = ntx >= 4 ? 4u : (ntx >= 2 ? 2u : 1u);
+    MMPlan best{};
+    double best_cost = 1e30;
+    const uint32_t ms[3] = { 1, 2, 4 };
+    for (int mi = 0; mi < 3; ++mi) {
+        const uint32_t M = ms[mi];
+        if (const char* env = getenv("MTX_MINMAX_M"))
+            if ((uint32_t)atoi(env) != M) continue;
+        const uint64_t target = (uint64_t)device_cus() * (uint64_t)(resident[mi] > 0 ? resident[mi] : 1);
+        MMPlan p{};
+        p.M = M;
+        p.wx = wx;
+        p.ntxg = (ntx + wx - 1) / wx;
+        const uint32_t ygroups = (g.gy + M - 1) / M;
+        p.nygrp = (ygroups + (4 / wx) - 1) / (4 / wx);
+        const uint64_t cols = (uint64_t)p.ntxg * p.nygrp;
+        uint64_t zc = (cols * g.gz + target - 1) / target;
+        if (const char* env = getenv("MTX_MINMAX_ZCHUNK")) zc = strtoull(env, nullptr, 10);
+        if (zc < 1) zc = 1;
+        if (zc > g.gz) zc = g.gz;
+        p.zc = (uint32_t)zc;
+        p.ncz = (uint32_t)((g.gz + zc - 1) / zc);
+        p.nwg = cols * p.ncz;
+        const double cost = 1.0 / (8.0 * M) + 1.0 / (8.0 * (double)zc);
+        if (cost < best_cost) {
+            best_cost = cost;
+            best = p;
+        }
     }
-    const uint64_t target = (uint64_t)device_cus() * resident;
-    uint64_t zc = (cols * g.gz + target - 1) / target;
-    if (const char* env = getenv("MTX_MINMAX_ZCHUNK")) zc = strtoull(env, nullptr, 10);
-    if (zc < 1) zc = 1;
-    if (zc > g.gz) zc = g.gz;
-    const uint32_t ncz = (uint32_t)((g.gz + zc - 1) / zc);
-    const uint64_t nwg = cols * ncz;
-    if (nwg > 0x7FFFFFFFull) return hipErrorInvalidValue;
-    const uint32_t wpr = (g.gx + 31u) / 32u;
-    const bool aligned = (v.X % 4u) == 0 && (((uintptr_t)v.data) % (4 * sizeof(T))) == 0;
-    ActParams a{};
-    if (act) a = *act;
-    const T* p = static_cast<const T*>(v.data);
-    uint2* pr = reinterpret_cast<uint2*>(pairs);
-    if (aligned)
-        hipLaunchKernelGGL((k_minmax_b8<T, true, MASK>), dim3((uint32_t)nwg), dim3(256), 0, s, p,
-                           v.X, v.Y, v.Z, g.gx, g.gy, g.gz, ntx, ngy, (uint32_t)zc, pr, mask, wpr,
-                           a, zero, zero_words);
-    else
-        hipLaunchKernelGGL((k_minmax_b8<T, false, MASK>), dim3((uint32_t)nwg), dim3(256), 0, s, p,
-                           v.X, v.Y, v.Z, g.gx, g.gy, g.gz, ntx, ngy, (uint32_t)zc, pr, mask, wpr,
-                           a, zero, zero_words);
+    return best;
+}
+
+template <typename T, bool ALIGNED, bool MASK, int M>
+static hipError_t launch_mm_m(const VolDesc& v, const Grid& g, const MMPlan& p, uint32_t ntx,
+                              uint32_t* pairs, uint32_t* mask, const ActParams& a, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_minmax_b8<T, ALIGNED, MASK, M>), dim3((uint32_t)p.nwg), dim3(256), 0, s,
+                       static_cast<const T*>(v.data), v.X, v.Y, v.Z, g.gx, g.gy, g.gz, ntx, p.wx,
+                       p.ntxg, p.nygrp, p.zc, reinterpret_cast<uint2*>(pairs), mask, (g.gx + 31u) / 32u, a);
     return hipGetLastError();
 }
 
+template <typename T, bool MASK>
+static hipError_t launch_mm_b8_t(const VolDesc& v, const Grid& g, uint32_t* pairs, uint32_t* mask,
+                                 const ActParams* act, hipStream_t s)
+{
+    const uint32_t ntx = (g.gx + 31u) / 32u;
+    static int resident[3] = { 0, 0, 0 };
+    if (!resident[0]) {
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident[0], k_minmax_b8<T, true, MASK, 1>, 256, 0);
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident[1], k_minmax_b8<T, true, MASK, 2>, 256, 0);
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident[2], k_minmax_b8<T, true, MASK, 4>, 256, 0);
+        for (int& r : resident)
+            if (r < 1) r = 1;
+    }
+    const MMPlan p = plan_minmax(g, ntx, resident);
+    if (p.nwg == 0 || p.nwg > 0x7FFFFFFFull) return hipErrorInvalidValue;
+    const bool aligned = (v.X % 4u) == 0 && (((uintptr_t)v.data) % (4 * sizeof(T))) == 0;
+    ActParams a{};
+    if (act) a = *act;
+#define MTX_MM_CASE(MM)                                                                            \
+    if (p.M == MM)                                                                                 \
+        return aligned ? launch_mm_m<T, true, MASK, MM>(v, g, p, ntx, pairs, mask, a, s)           \
+                       : launch_mm_m<T, false, MASK, MM>(v, g, p, ntx, pairs, mask, a, s);
+    MTX_MM_CASE(1)
+    MTX_MM_CASE(2)
+    MTX_MM_CASE(4)
+#undef MTX_MM_CASE
+    return hipErrorInvalidValue;
+}
+
 hipError_t launch_minmax_b8(const VolDesc& v, const Grid& g, uint32_t* pairs, uint32_t* mask,
-                            const ActParams* act, uint64_t* zero, uint32_t zero_words,
-                            hipStream_t s)
+                            const ActParams* act, hipStream_t s)
 {
     const bool m = mask != nullptr;
     if (v.dtype == kF32)
-        return m ? launch_mm_b8_t<float, true>(v, g, pairs, mask, act, zero, zero_words, s)
-                 : launch_mm_b8_t<float, false>(v, g, pairs, mask, act, zero, zero_words, s);
-    return m ? launch_mm_b8_t<uint8_t, true>(v, g, pairs, mask, act, zero, zero_words, s)
-             : launch_mm_b8_t<uint8_t, false>(v, g, pairs, mask, act, zero, zero_words, s);
+        return m ? launch_mm_b8_t<float, true>(v, g, pairs, mask, act, s)
+                 : launch_mm_b8_t<float, false>(v, g, pairs, mask, act, s);
+    return m ? launch_mm_b8_t<uint8_t, true>(v, g, pairs, mask, act, s)
+             : launch_mm_b8_t<uint8_t, false>(v, g, pairs, mask, act, s);
 }
 
 // ---------------------------------------------------------------------------
