@@ -92,8 +92,8 @@ typedef struct mtx_options {
                                 pc->isovalue, so the filter does not re-read the pairs */
     uint32_t normals;        /* 1 (default): vertex normals; 0: normal = (0,0,0,0) */
     uint32_t edge_ids;       /* 1: also write each vertex's owning edge ID (default 0) */
-    uint32_t case_indices;   /* 1: also write the case index of every cell of every active
-                                block (default 0) */
+    uint32_t case_indices;   /* reserved (the case indices are always kept, see
+                                mtx_extraction_output.case_indices) */
     uint32_t voxel_positions;/* 1: also write float3 positions in voxel units (default 0) */
     uint32_t reuse;          /* 1: reuse the output's existing buffers when they fit the
                                 problem: no allocation, no host synchronisation.  For
@@ -160,7 +160,9 @@ typedef struct mtx_extraction_output {
     uint64_t* vertex_edge_ids;   /* optional (edge_ids): 3*linear(owner corner) + axis */
     float* vertex_t;             /* optional (edge_ids): t along the owning edge, in [0,1] */
     float* vertex_voxel_pos;     /* optional: float3 per vertex, voxel units */
-    uint8_t* case_indices;       /* optional: B^3 bytes per active block */
+    uint8_t* case_indices;       /* case index of every cell of every active block, B^3 bytes per
+                                    block in list order, x-fastest; 0 outside the volume (a view
+                                    into the workspace) */
     void* workspace;             /* per-block topology kept for the adjoint */
     uint64_t workspace_bytes;
     uint64_t vertex_capacity;    /* in vertices */

@@ -106,7 +106,7 @@ uint32_t pyramid_layout(const uint32_t G[3], uint32_t (*ext)[3], uint64_t* off)
 
 // Extraction workspace layout (offsets in bytes).
 struct WsLayout {
-    uint64_t counts, vbase, tbase, mbase, table, totals, scan, bytes;
+    uint64_t counts, vbase, tbase, mbase, table, cases, totals, scan, bytes;
     uint32_t scan_tiles;
 };
 
@@ -119,6 +119,7 @@ WsLayout ws_layout(uint64_t max_blocks, uint32_t B)
     w.tbase = o;  o = align256(o + 8 * max_blocks);
     w.mbase = o;  o = align256(o + 8 * max_blocks);
     w.table = o;  o = align256(o + corner_entry_bytes(B) * (uint64_t)B * B * B * max_blocks);
+    w.cases = o;  o = align256(o + (uint64_t)B * B * B * max_blocks);
     w.totals = o; o = align256(o + 8 * 4);
     uint64_t tiles = (max_blocks + scan_tile_items() - 1) / scan_tile_items();
     if (tiles == 0) tiles = 1;
@@ -449,7 +450,7 @@ static ExtractArgs make_extract_args(const mtx_minmax_output* mm, const mtx_filt
     a.blk_mbase = reinterpret_cast<uint64_t*>(ws + w.mbase);
     a.corner_table = ws + w.table;
     a.totals = reinterpret_cast<uint64_t*>(ws + w.totals);
-    a.case_out = out->case_indices;
+    a.cases = reinterpret_cast<uint8_t*>(ws + w.cases);
     a.vertices = reinterpret_cast<float*>(out->vertices);
     a.indices = out->indices;
     a.edge_ids = out->vertex_edge_ids;
@@ -486,7 +487,7 @@ mtx_status mtx_extract(mtx_context* ctx, const mtx_minmax_output* mm,
     const uint32_t B = mm->block_dim;
     const uint64_t nb = (uint64_t)mm->grid[0] * mm->grid[1] * mm->grid[2];
     const uint32_t want = (o.normals ? 1u : 0u) | (o.edge_ids ? 2u : 0u) |
-                          (o.case_indices ? 4u : 0u) | (o.voxel_positions ? 8u : 0u);
+                          (o.voxel_positions ? 8u : 0u);
     const WsLayout w = ws_layout(nb, B);
 
     std::lock_guard<std::mutex> lk(ctx->mu);
@@ -505,8 +506,6 @@ mtx_status mtx_extract(mtx_context* ctx, const mtx_minmax_output* mm,
             a.edge_t = nullptr;
         }
         if (!o.voxel_positions) a.vox_pos = nullptr;
-        if (!o.case_indices) a.case_out = nullptr;
-        if (a.case_out && out->case_capacity < nb * B * B * B) a.case_out = nullptr;
         uint64_t* scan = reinterpret_cast<uint64_t*>(static_cast<char*>(out->workspace) + w.scan);
         ScanState st{ scan, reinterpret_cast<uint32_t*>(scan + 3 * w.scan_tiles) };
         hipError_t e = launch_zero64(reinterpret_cast<uint64_t*>(static_cast<char*>(out->workspace) + w.totals),
@@ -527,9 +526,9 @@ mtx_status mtx_extract(mtx_context* ctx, const mtx_minmax_output* mm,
     out->workspace_bytes = w.bytes;
     hipError_t e = hipMalloc(&out->workspace, w.bytes);
     if (e == hipSuccess) e = hipMalloc(&out->counts, 256);
-    if (e == hipSuccess && o.case_indices) {
+    if (e == hipSuccess) {
+        out->case_indices = static_cast<uint8_t*>(out->workspace) + w.cases;
         out->case_capacity = nb * B * B * B;
-        e = hipMalloc(&out->case_indices, out->case_capacity ? out->case_capacity : 1);
     }
     if (e != hipSuccess) {
         mtx_extraction_release(ctx, out);
@@ -605,8 +604,9 @@ void mtx_extraction_release(mtx_context* ctx, mtx_extraction_output* out)
 {
     if (!out) return;
     if (ctx) hipSetDevice(ctx->device);
+    // case_indices points into the workspace
     void* ptrs[] = { out->vertices, out->counts, out->indices, out->meshlets, out->vertex_edge_ids,
-                     out->vertex_t, out->vertex_voxel_pos, out->case_indices, out->workspace };
+                     out->vertex_t, out->vertex_voxel_pos, out->workspace };
     for (void* p : ptrs)
         if (p) hipFree(p);
     std::memset(out, 0, sizeof *out);

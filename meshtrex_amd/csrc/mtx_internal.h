@@ -84,7 +84,7 @@ struct ExtractArgs {
     uint64_t* blk_tbase;        // per active block triangle base
     uint64_t* blk_mbase;        // per active block meshlet base
     uint64_t* totals;           // [0]=vertices [1]=triangles [2]=error flags [3]=meshlets
-    uint8_t* case_out;          // optional: B^3 case bytes per active block
+    uint8_t* cases;             // per active block: B^3 case bytes (0 for cells outside the volume)
     // outputs
     float* vertices;            // 8 floats per vertex (position xyzw, normal xyzw)
     uint32_t* indices;          // 3 per triangle

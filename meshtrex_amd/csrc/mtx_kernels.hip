@@ -394,9 +394,9 @@ static hipError_t launch_mm_b8_t(const VolDesc& v, const Grid& g, uint32_t* pair
     const uint32_t ntx = (g.gx + 31u) / 32u;
     static int resident[3] = { 0, 0, 0 };
     if (!resident[0]) {
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident[0], k_minmax_b8<T, true, MASK, 1>, 256, 0);
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident[1], k_minmax_b8<T, true, MASK, 2>, 256, 0);
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident[2], k_minmax_b8<T, true, MASK, 4>, 256, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident[0], k_minmax_b8<T, true, MASK, 1>, 256, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident[1], k_minmax_b8<T, true, MASK, 2>, 256, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident[2], k_minmax_b8<T, true, MASK, 4>, 256, 0);
         for (int& r : resident)
             if (r < 1) r = 1;
     }
@@ -853,7 +853,7 @@ __global__ __launch_bounds__(256) void k_extract_count(ExtractArgs a)
                         nt += s_ntri[cs];
                     }
                 }
-                if (a.case_out) a.case_out[(uint64_t)i * NC + c] = (uint8_t)cs;
+                a.cases[(uint64_t)i * NC + c] = (uint8_t)cs;
             }
         }
         uint32_t tv, tt;
@@ -874,6 +874,114 @@ __global__ __launch_bounds__(256) void k_extract_count(ExtractArgs a)
             for (int j = 0; j < CPL; ++j)
                 if (lane * CPL + j < (uint32_t)NC) dst[j] = ent[j];
         }
+        if (lane == 0) {
+            a.blk_counts[2 * (uint64_t)i] = tv;
+            a.blk_counts[2 * (uint64_t)i + 1] = tt;
+        }
+        wave_lds_sync();
+    }
+}
+
+// Count pass specialised for B = 8: the below-isovalue bits of the 9^3 tile are
+// gathered with wave ballots (12 x 64 bits, in LDS), each lane owns one x-row
+// of 8 corners (ly = lane & 7, lz = lane >> 3) and classifies it with bit
+// operations on its four 9-bit rows (y, y+1) x (z, z+1).  Interior blocks load
+// their tile through per-lane offsets computed once.
+template <typename T>
+__global__ __launch_bounds__(256) void k_extract_count_b8(ExtractArgs a)
+{
+    constexpr int B = 8, E = 9, NTILE = 729, LPL = 12, NC = 512;
+    __shared__ uint64_t s_bits_all[4][LPL];
+    __shared__ uint8_t s_ntri[256];
+    for (uint32_t c = threadIdx.x; c < 256u; c += blockDim.x) s_ntri[c] = (uint8_t)mtx_word_ntri(mtx_tri_words[c]);
+    __syncthreads();
+    const uint32_t lane = lane_id();
+    uint64_t* s_bits = s_bits_all[threadIdx.x >> 6];
+    const T* __restrict__ vol = static_cast<const T*>(a.vol.data);
+    const uint32_t X = a.vol.X, Y = a.vol.Y, Z = a.vol.Z;
+    const bool degen = X < 2 || Y < 2 || Z < 2;
+    const bool off32 = (uint64_t)X * Y * E < 0x80000000ull;
+    uint32_t toff[LPL];
+#pragma unroll
+    for (int k = 0; k < LPL; ++k) {
+        const uint32_t j = min(lane + 64u * k, (uint32_t)NTILE - 1u);
+        toff[k] = off32 ? j % E + X * ((j / E) % E + Y * (j / (E * E))) : 0u;
+    }
+    const uint32_t na = *a.count;
+    uint16_t* __restrict__ table = static_cast<uint16_t*>(a.corner_table);
+    const uint32_t ly = lane & 7u, lz = lane >> 3;
+    const WaveItems it = wave_items(na, 4);
+    for (uint32_t i = it.first; i < it.end; i += it.stride) {
+        const uint32_t id = a.ids[i];
+        const uint32_t bx = id % a.grid.gx, by = (id / a.grid.gx) % a.grid.gy;
+        const uint32_t bz = id / (a.grid.gx * a.grid.gy);
+        const uint32_t ox = bx * B, oy = by * B, oz = bz * B;
+        const uint64_t base = ox + (uint64_t)X * (oy + (uint64_t)Y * oz);
+        const bool interior = off32 && ox + B < X && oy + B < Y && oz + B < Z;
+        T v[LPL];
+        bool in[LPL];
+        if (interior) {
+#pragma unroll
+            for (int k = 0; k < LPL; ++k) {
+                v[k] = vol[base + toff[k]];
+                in[k] = lane + 64u * k < (uint32_t)NTILE;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < LPL; ++k) {
+                const uint32_t j = lane + 64u * k;
+                const uint32_t x = ox + j % E, y = oy + (j / E) % E, z = oz + j / (E * E);
+                in[k] = j < (uint32_t)NTILE && x < X && y < Y && z < Z;
+                v[k] = vol[in[k] ? x + (uint64_t)X * (y + (uint64_t)Y * z) : 0];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) {
+            const uint64_t bal = __ballot(in[k] && vfloat(v[k]) <= a.iso);
+            if (lane == 0) s_bits[k] = bal;
+        }
+        wave_lds_sync();
+        // the four 9-bit rows of this lane: (y, z), (y+1, z), (y, z+1), (y+1, z+1)
+        auto row = [&](uint32_t r) -> uint32_t {
+            const uint32_t o = 9u * r;
+            return (uint32_t)(s_bits[o >> 6] >> (o & 63u)) & 0x1FFu |
+                   (((o & 63u) > 55u) ? (uint32_t)(s_bits[(o >> 6) + 1] << (64u - (o & 63u))) & 0x1FFu : 0u);
+        };
+        const uint32_t r0 = ly + 9u * lz;
+        const uint32_t b00 = row(r0), b10 = row(r0 + 1u), b01 = row(r0 + 9u), b11 = row(r0 + 10u);
+        const uint32_t nx = X - ox;  // >= 1
+        const uint32_t ex_x = nx >= 8u ? 0xFFu : ((1u << nx) - 1u);
+        const uint32_t ed_x = nx >= 9u ? 0xFFu : ((1u << (nx - 1u)) - 1u);
+        const bool rowok = !degen && oy + ly < Y && oz + lz < Z;
+        const bool yok = oy + ly + 1u < Y, zok = oz + lz + 1u < Z;
+        const uint32_t fx = rowok ? (b00 ^ (b00 >> 1)) & ed_x : 0u;
+        const uint32_t fy = (rowok && yok) ? (b00 ^ b10) & ex_x : 0u;
+        const uint32_t fz = (rowok && zok) ? (b00 ^ b01) & ex_x : 0u;
+        const uint32_t cellok = (rowok && yok && zok) ? ed_x : 0u;
+        uint32_t nt = 0, cases0 = 0, cases1 = 0;
+#pragma unroll
+        for (uint32_t lx = 0; lx < 8u; ++lx) {
+            const uint32_t cs = ((b00 >> lx) & 1u) | ((b00 >> lx) & 2u) | (((b10 >> lx) & 2u) << 1) |
+                                (((b10 >> lx) & 1u) << 3) | (((b01 >> lx) & 1u) << 4) | (((b01 >> lx) & 2u) << 4) |
+                                (((b11 >> lx) & 2u) << 5) | (((b11 >> lx) & 1u) << 7);
+            const uint32_t c = ((cellok >> lx) & 1u) ? cs : 0u;
+            nt += (cellok >> lx) & 1u ? s_ntri[c] : 0u;
+            if (lx < 4) cases0 |= c << (8 * lx);
+            else cases1 |= c << (8 * (lx - 4));
+        }
+        const uint32_t nv = __popc(fx) + __popc(fy) + __popc(fz);
+        uint32_t tv;
+        uint32_t run = wave_excl_scan<uint32_t>(nv, tv);
+        const uint32_t tt = wave_sum<uint32_t>(nt);
+        uint16_t ent[8];
+#pragma unroll
+        for (uint32_t lx = 0; lx < 8u; ++lx) {
+            const uint32_t f = ((fx >> lx) & 1u) | (((fy >> lx) & 1u) << 1) | (((fz >> lx) & 1u) << 2);
+            ent[lx] = (uint16_t)((run << 3) | f);
+            run += __popc(f);
+        }
+        *reinterpret_cast<uint4*>(table + (uint64_t)i * NC + lane * 8u) = *reinterpret_cast<const uint4*>(ent);
+        *reinterpret_cast<uint2*>(a.cases + (uint64_t)i * NC + lane * 8u) = make_uint2(cases0, cases1);
         if (lane == 0) {
             a.blk_counts[2 * (uint64_t)i] = tv;
             a.blk_counts[2 * (uint64_t)i + 1] = tt;
@@ -993,214 +1101,155 @@ __device__ __forceinline__ void grad_normal(const float* s, uint32_t t, uint32_t
     n[0] = gx * r; n[1] = gy * r; n[2] = gz * r;
 }
 
-template <int B, bool NORMALS> struct EmitCfg {
+template <int B, bool NORMALS> struct VertCfg {
     static constexpr int A = NORMALS ? 1 : 0;  // apron for central differences
     static constexpr int E = B + 1 + 2 * A;
     static constexpr int NTILE = E * E * E;
-    static constexpr int F = B + 1;             // corner table edge: owner corners [0, B]^3
-    static constexpr int NF = F * F * F;
-    static constexpr int LIST = 256;            // work-list chunk (vertices, triangles)
-    static constexpr int WAVE_LDS = NTILE * 4 + NF * 5 + LIST * 6 + 64;
-    // waves per workgroup: the most resident waves per CU (160 KiB of LDS on
-    // gfx950, 2 KiB of case table per workgroup, at most 64 KiB per workgroup)
-    static constexpr int waves_per_cu(int wpg)
-    {
-        return wpg * WAVE_LDS + 2048 > 65536 ? 0 : wpg * (163840 / (wpg * WAVE_LDS + 2048));
-    }
-    static constexpr int WPG = waves_per_cu(4) >= waves_per_cu(2) && waves_per_cu(4) >= waves_per_cu(1)
-                                   ? 4 : (waves_per_cu(2) >= waves_per_cu(1) ? 2 : 1);
+    static constexpr int LIST = 256;            // vertices listed at a time
+    static constexpr int WAVE_LDS = NTILE * 4 + LIST * 2 + 16;
+    static constexpr int WPG = WAVE_LDS * 4 <= 65536 ? 4 : (WAVE_LDS * 2 <= 65536 ? 2 : 1);
 };
 
-// Per wave and block, the emit pass builds in LDS the global index of the
-// first vertex at every owner corner of [0, B]^3 and its 3 edge flags: its own
-// corners from its scan, the corners on the +x/+y/+z faces from the neighbour
-// blocks' corner tables (one batched gather per block).  It then lists the
-// block's vertices (corner, axis) and triangles (cell, case, k) in LDS and
-// lets the 64 lanes take consecutive list entries: the work is balanced across
-// lanes whatever the shape of the surface, and consecutive lanes write
-// consecutive vertices and triangles.
+// This lane's corner entries (prefix << 3 | flags) and cell cases from the
+// count pass, packed: 16-bit entries two per register, cases four per register.
+template <int B>
+struct LaneCorners {
+    using Entry = typename EB<B>::Entry;
+    static constexpr int CPL = EB<B>::CPL, NC = EB<B>::NC;
+    static constexpr bool E16 = sizeof(Entry) == 2;
+    static constexpr int NEW = E16 ? (CPL + 1) / 2 : CPL, NCW = (CPL + 3) / 4;
+    uint32_t entw[NEW], csw[NCW];
+
+    __device__ __forceinline__ void load(const Entry* __restrict__ table, const uint8_t* __restrict__ cases,
+                                         uint64_t i, uint32_t lane, bool want_cases)
+    {
+        if constexpr (CPL == 8 && E16) {
+            const uint4 e4 = *reinterpret_cast<const uint4*>(table + i * NC + lane * 8u);
+            entw[0] = e4.x; entw[1] = e4.y; entw[2] = e4.z; entw[3] = e4.w;
+            if (want_cases) {
+                const uint2 c2 = *reinterpret_cast<const uint2*>(cases + i * NC + lane * 8u);
+                csw[0] = c2.x; csw[1] = c2.y;
+            } else {
+                csw[0] = csw[1] = 0;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < NEW; ++k) entw[k] = 0;
+#pragma unroll
+            for (int k = 0; k < NCW; ++k) csw[k] = 0;
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) {
+                const uint32_t c = lane * CPL + j;
+                const uint32_t e = c < (uint32_t)NC ? (uint32_t)table[i * NC + c] : 0u;
+                const uint32_t k = (want_cases && c < (uint32_t)NC) ? (uint32_t)cases[i * NC + c] : 0u;
+                if (E16) entw[j / 2] |= e << (16 * (j & 1));
+                else entw[j] = e;
+                csw[j / 4] |= k << (8 * (j & 3));
+            }
+        }
+    }
+    __device__ __forceinline__ uint32_t ent(int j) const
+    {
+        return E16 ? (entw[j / 2] >> (16 * (j & 1))) & 0xFFFFu : entw[j];
+    }
+    __device__ __forceinline__ uint32_t cs(int j) const { return (csw[j / 4] >> (8 * (j & 3))) & 0xFFu; }
+};
+
+// Vertex pass: per wave and active block, stage the voxel tile [-A, B+A]^3
+// (clamped to the volume: clamp-to-edge samples for the normals) in LDS, list
+// the block's vertices (corner, axis) in prefix order and let the 64 lanes
+// take consecutive entries: balanced work and coalesced writes.
 template <int B, typename T, bool NORMALS>
-__global__ __launch_bounds__((64 * EmitCfg<B, NORMALS>::WPG)) __attribute__((amdgpu_waves_per_eu(4)))
-void k_extract_emit(ExtractArgs a)
+__global__ __launch_bounds__((64 * VertCfg<B, NORMALS>::WPG)) void k_extract_vertices(ExtractArgs a)
 {
-    using C = EmitCfg<B, NORMALS>;
-    constexpr int A = C::A, E = C::E, NTILE = C::NTILE, WPG = C::WPG, F = C::F, NF = C::NF;
+    using C = VertCfg<B, NORMALS>;
+    constexpr int A = C::A, E = C::E, NTILE = C::NTILE, WPG = C::WPG;
     constexpr uint32_t LIST = C::LIST;
-    constexpr int NC = EB<B>::NC;
     constexpr int CPL = EB<B>::CPL;
     using Entry = typename EB<B>::Entry;
     __shared__ float s_val_all[WPG][NTILE];
-    __shared__ uint32_t s_gidx_all[WPG][NF];
-    __shared__ uint8_t s_gfl_all[WPG][NF];
     __shared__ uint16_t s_vl_all[WPG][LIST];
-    __shared__ uint32_t s_tl_all[WPG][LIST];
-    __shared__ int32_t s_nbr_all[WPG][8];
-    __shared__ uint32_t s_nbase_all[WPG][8];
-    __shared__ uint64_t s_words[256];
-    for (uint32_t c = threadIdx.x; c < 256u; c += blockDim.x) s_words[c] = mtx_tri_words[c];
-    __syncthreads();
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
     float* s_val = s_val_all[w];
-    uint32_t* s_gidx = s_gidx_all[w];
-    uint8_t* s_gfl = s_gfl_all[w];
     uint16_t* s_vl = s_vl_all[w];
-    uint32_t* s_tl = s_tl_all[w];
-    int32_t* s_nbr = s_nbr_all[w];
-    uint32_t* s_nbase = s_nbase_all[w];
     const T* __restrict__ vol = static_cast<const T*>(a.vol.data);
     const uint32_t X = a.vol.X, Y = a.vol.Y, Z = a.vol.Z;
     const float iso = a.iso;
     const uint32_t na = *a.count;
     const Entry* __restrict__ table = static_cast<const Entry*>(a.corner_table);
     const float inv2x = 2.0f / (float)X, inv2y = 2.0f / (float)Y, inv2z = 2.0f / (float)Z;
-    const bool degen = X < 2 || Y < 2 || Z < 2;
+    constexpr int LPL = (NTILE + 63) / 64;
+    const bool off32 = (uint64_t)X * Y * E < 0x80000000ull;
     const WaveItems it = wave_items(na, WPG);
     for (uint32_t i = it.first; i < it.end; i += it.stride) {
+        const uint32_t bnv = a.blk_counts[2 * (uint64_t)i];
+        if (bnv == 0) continue;
+        const uint64_t vb = a.blk_vbase[i];
+        if (vb + bnv > a.vcap) {
+            if (lane == 0) atomicOr(reinterpret_cast<unsigned long long*>(&a.totals[2]), 1ull);
+            continue;
+        }
         const uint32_t id = a.ids[i];
         const uint32_t bx = id % a.grid.gx, by = (id / a.grid.gx) % a.grid.gy;
         const uint32_t bz = id / (a.grid.gx * a.grid.gy);
         const uint32_t ox = bx * B, oy = by * B, oz = bz * B;
-        const uint64_t vb = a.blk_vbase[i], tb = a.blk_tbase[i], mb = a.blk_mbase[i];
-        const uint32_t bnv = a.blk_counts[2 * (uint64_t)i], bnt = a.blk_counts[2 * (uint64_t)i + 1];
-        const uint32_t bnm = (bnt + kMeshletPrims - 1) / kMeshletPrims;
-        const bool fits = vb + bnv <= a.vcap && 3 * (tb + bnt) <= a.icap &&
-                          (!a.meshlets || mb + bnm <= a.mcap);
-        if (!fits) {
-            if (lane == 0) atomicOr(reinterpret_cast<unsigned long long*>(&a.totals[2]), 1ull);
-            continue;
-        }
-        if (lane < 8) {
-            int32_t r = -1;
-            uint32_t base = 0;
-            const uint32_t nx = bx + (lane & 1u), ny = by + ((lane >> 1) & 1u), nz = bz + (lane >> 2);
-            if (lane != 0 && lane != 7 && nx < a.grid.gx && ny < a.grid.gy && nz < a.grid.gz) {
-                r = (int32_t)a.blk_map[nx + a.grid.gx * (ny + (uint64_t)a.grid.gy * nz)];
-                if (r >= 0) base = (uint32_t)a.blk_vbase[r];
-            }
-            s_nbr[lane] = r;
-            s_nbase[lane] = base;
-        }
-        // voxel tile, all of a batch's loads in flight together
-        constexpr int LPL = (NTILE + 63) / 64, BATCH = LPL < 8 ? LPL : 8;
+        LaneCorners<B> lc;
+        lc.load(table, nullptr, i, lane, false);
+        const bool interior = off32 && ox >= (uint32_t)A && oy >= (uint32_t)A && oz >= (uint32_t)A &&
+                              ox + B + A < X && oy + B + A < Y && oz + B + A < Z;
+        constexpr int BATCH = LPL < 11 ? LPL : 11;
+        if (interior) {
+            const T* __restrict__ t0p = vol + ((ox - A) + (uint64_t)X * ((oy - A) + (uint64_t)Y * (oz - A)));
 #pragma unroll 1
-        for (int k0 = 0; k0 < LPL; k0 += BATCH) {
-            T v[BATCH];
+            for (int k0 = 0; k0 < LPL; k0 += BATCH) {
+                T v[BATCH];
 #pragma unroll
-            for (int k = 0; k < BATCH; ++k) {
-                const uint32_t j = min(lane + 64u * (k0 + k), (uint32_t)NTILE - 1u);
-                int32_t x = (int32_t)(ox + j % E) - A, y = (int32_t)(oy + (j / E) % E) - A;
-                int32_t z = (int32_t)(oz + j / (E * E)) - A;
-                x = min(max(x, 0), (int32_t)X - 1);
-                y = min(max(y, 0), (int32_t)Y - 1);
-                z = min(max(z, 0), (int32_t)Z - 1);
-                v[k] = vol[(uint32_t)x + (uint64_t)X * ((uint32_t)y + (uint64_t)Y * (uint32_t)z)];
-            }
-#pragma unroll
-            for (int k = 0; k < BATCH; ++k) {
-                const uint32_t j = lane + 64u * (k0 + k);
-                if (j < (uint32_t)NTILE) s_val[j] = vfloat(v[k]);
-            }
-        }
-        wave_lds_sync();
-        // neighbour-owned corners (a coordinate equal to B): one batched gather.
-        // Face corner f enumerates x = B (F*F corners), then y = B with x < B
-        // (B*F), then z = B with x, y < B (B*B): F^3 - B^3 in all.
-        {
-            constexpr int NFACE = NF - NC;
-            constexpr int NPL = (NFACE + 63) / 64;
-            Entry ent[NPL];
-            int32_t jn[NPL];
-            uint32_t qs[NPL], nsel[NPL];
-#pragma unroll
-            for (int k = 0; k < NPL; ++k) {
-                const uint32_t f = lane + 64u * k;
-                uint32_t qx, qy, qz;
-                if (f < (uint32_t)(F * F)) {
-                    qx = B; qy = f % F; qz = f / F;
-                } else if (f < (uint32_t)(F * F + B * F)) {
-                    const uint32_t g = f - F * F;
-                    qx = g % B; qy = B; qz = g / B;
-                } else {
-                    const uint32_t g = f - F * F - B * F;
-                    qx = g % B; qy = g / B; qz = B;
+                for (int k = 0; k < BATCH; ++k) {
+                    const uint32_t j = min(lane + 64u * (k0 + k), (uint32_t)NTILE - 1u);
+                    v[k] = t0p[j % E + X * ((j / E) % E + Y * (j / (E * E)))];
                 }
-                const uint32_t nx = qx == (uint32_t)B, ny = qy == (uint32_t)B, nz = qz == (uint32_t)B;
-                nsel[k] = nx | (ny << 1) | (nz << 2);
-                qs[k] = qx + F * (qy + F * qz);
-                jn[k] = f < (uint32_t)NFACE ? s_nbr[nsel[k]] : -1;
-                ent[k] = 0;
-                if (jn[k] >= 0)
-                    ent[k] = table[(uint64_t)jn[k] * NC + (qx - B * nx) + B * ((qy - B * ny) + B * (qz - B * nz))];
-            }
 #pragma unroll
-            for (int k = 0; k < NPL; ++k) {
-                const uint32_t f = lane + 64u * k;
-                if (f < (uint32_t)NFACE) {
-                    s_gidx[qs[k]] = jn[k] >= 0 ? s_nbase[nsel[k]] + (uint32_t)(ent[k] >> 3) : 0u;
-                    s_gfl[qs[k]] = jn[k] >= 0 ? (uint8_t)(ent[k] & 7u) : (uint8_t)0xF0u;
+                for (int k = 0; k < BATCH; ++k) {
+                    const uint32_t j = lane + 64u * (k0 + k);
+                    if (j < (uint32_t)NTILE) s_val[j] = vfloat(v[k]);
+                }
+            }
+        } else {
+#pragma unroll 1
+            for (int k0 = 0; k0 < LPL; k0 += BATCH) {
+                T v[BATCH];
+#pragma unroll
+                for (int k = 0; k < BATCH; ++k) {
+                    const uint32_t j = min(lane + 64u * (k0 + k), (uint32_t)NTILE - 1u);
+                    int32_t x = (int32_t)(ox + j % E) - A, y = (int32_t)(oy + (j / E) % E) - A;
+                    int32_t z = (int32_t)(oz + j / (E * E)) - A;
+                    x = min(max(x, 0), (int32_t)X - 1);
+                    y = min(max(y, 0), (int32_t)Y - 1);
+                    z = min(max(z, 0), (int32_t)Z - 1);
+                    v[k] = vol[(uint32_t)x + (uint64_t)X * ((uint32_t)y + (uint64_t)Y * (uint32_t)z)];
+                }
+#pragma unroll
+                for (int k = 0; k < BATCH; ++k) {
+                    const uint32_t j = lane + 64u * (k0 + k);
+                    if (j < (uint32_t)NTILE) s_val[j] = vfloat(v[k]);
                 }
             }
         }
-        // classify my corners/cells
-        uint32_t fl[CPL], cs[CPL];
-        uint32_t nv = 0, nt = 0;
-#pragma unroll
-        for (int j = 0; j < CPL; ++j) {
-            const uint32_t c = lane * CPL + j;
-            fl[j] = 0;
-            cs[j] = 0xFFFFFFFFu;
-            if (c >= (uint32_t)NC) continue;
-            const uint32_t lx = c % B, ly = (c / B) % B, lz = c / (B * B);
-            const uint32_t x = ox + lx, y = oy + ly, z = oz + lz;
-            if (degen || !(x < X && y < Y && z < Z)) continue;
-            const uint32_t t0 = (lx + A) + E * ((ly + A) + E * (lz + A));
-            const uint32_t b0 = s_val[t0] <= iso;
-            uint32_t f = 0;
-            if (x + 1 < X) f |= b0 ^ (uint32_t)(s_val[t0 + 1] <= iso);
-            if (y + 1 < Y) f |= (b0 ^ (uint32_t)(s_val[t0 + E] <= iso)) << 1;
-            if (z + 1 < Z) f |= (b0 ^ (uint32_t)(s_val[t0 + E * E] <= iso)) << 2;
-            fl[j] = f;
-            nv += __popc(f);
-            if (x + 1 < X && y + 1 < Y && z + 1 < Z) {
-                uint32_t q8 = 0;
-#pragma unroll
-                for (uint32_t q = 0; q < 8; ++q)
-                    q8 |= (uint32_t)(s_val[t0 + mtx_corner_x(q) + E * (mtx_corner_y(q) + E * mtx_corner_z(q))] <= iso) << q;
-                cs[j] = q8;
-                nt += mtx_word_ntri(s_words[q8]);
-            }
-        }
-        uint32_t tv, tt;
-        const uint32_t ev = wave_excl_scan<uint32_t>(nv, tv);
-        const uint32_t et = wave_excl_scan<uint32_t>(nt, tt);
-        {
-            uint32_t run = (uint32_t)vb + ev;
+        for (uint32_t c0 = 0; c0 < bnv; c0 += LIST) {
 #pragma unroll
             for (int j = 0; j < CPL; ++j) {
                 const uint32_t c = lane * CPL + j;
-                if (c < (uint32_t)NC) {
-                    const uint32_t q = c % B + F * ((c / B) % B + F * (c / (B * B)));
-                    s_gidx[q] = run;
-                    s_gfl[q] = (uint8_t)fl[j];
-                }
-                run += __popc(fl[j]);
-            }
-        }
-        // vertices, LIST at a time
-        for (uint32_t c0 = 0; c0 < tv; c0 += LIST) {
-            uint32_t run = ev;
-#pragma unroll
-            for (int j = 0; j < CPL; ++j) {
-                const uint32_t c = lane * CPL + j;
+                const uint32_t e = lc.ent(j);
+                uint32_t run = e >> 3;
 #pragma unroll
                 for (uint32_t ax = 0; ax < 3; ++ax)
-                    if ((fl[j] >> ax) & 1u) {
+                    if ((e >> ax) & 1u) {
                         if (run - c0 < LIST) s_vl[run - c0] = (uint16_t)(c | (ax << 12));
                         ++run;
                     }
             }
             wave_lds_sync();
-            const uint32_t n = min(LIST, tv - c0);
+            const uint32_t n = min(LIST, bnv - c0);
             for (uint32_t v = lane; v < n; v += 64u) {
                 const uint32_t e = s_vl[v];
                 const uint32_t c = e & 0xFFFu, ax = e >> 12;
@@ -1242,16 +1291,146 @@ void k_extract_emit(ExtractArgs a)
             }
             wave_lds_sync();
         }
-        // triangles, LIST at a time
+    }
+}
+
+template <int B> struct TriCfg {
+    static constexpr int F = B + 1;             // owner corners [0, B]^3
+    static constexpr int NF = F * F * F;
+    static constexpr int LIST = 256;            // triangles listed at a time
+    static constexpr int WAVE_LDS = NF * 5 + LIST * 4 + 64;
+    static constexpr int WPG = WAVE_LDS * 4 + 2048 <= 65536 ? 4 : (WAVE_LDS * 2 + 2048 <= 65536 ? 2 : 1);
+};
+
+// Triangle pass: per wave and active block, the global index of the first
+// vertex at every owner corner of [0, B]^3 and its 3 edge flags go to LDS --
+// the block's own corners from its count-pass entries, the corners on the
+// +x/+y/+z faces from the neighbours' entries (one batched gather) -- then the
+// block's triangles are listed (cell, case, k) and resolved three LDS reads
+// each by consecutive lanes.  No voxel is read.
+template <int B>
+__global__ __launch_bounds__((64 * TriCfg<B>::WPG)) void k_extract_triangles(ExtractArgs a)
+{
+    using C = TriCfg<B>;
+    constexpr int WPG = C::WPG, F = C::F, NF = C::NF;
+    constexpr uint32_t LIST = C::LIST;
+    constexpr int NC = EB<B>::NC;
+    constexpr int CPL = EB<B>::CPL;
+    using Entry = typename EB<B>::Entry;
+    __shared__ uint32_t s_gidx_all[WPG][NF];
+    __shared__ uint8_t s_gfl_all[WPG][NF];
+    __shared__ uint32_t s_tl_all[WPG][LIST];
+    __shared__ int32_t s_nbr_all[WPG][8];
+    __shared__ uint32_t s_nbase_all[WPG][8];
+    __shared__ uint64_t s_words[256];
+    for (uint32_t c = threadIdx.x; c < 256u; c += blockDim.x) s_words[c] = mtx_tri_words[c];
+    __syncthreads();
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    uint32_t* s_gidx = s_gidx_all[w];
+    uint8_t* s_gfl = s_gfl_all[w];
+    uint32_t* s_tl = s_tl_all[w];
+    int32_t* s_nbr = s_nbr_all[w];
+    uint32_t* s_nbase = s_nbase_all[w];
+    const uint32_t na = *a.count;
+    const Entry* __restrict__ table = static_cast<const Entry*>(a.corner_table);
+    const WaveItems it = wave_items(na, WPG);
+    for (uint32_t i = it.first; i < it.end; i += it.stride) {
+        const uint32_t bnt = a.blk_counts[2 * (uint64_t)i + 1];
+        if (bnt == 0) continue;
+        const uint32_t bnv = a.blk_counts[2 * (uint64_t)i];
+        const uint64_t vb = a.blk_vbase[i], tb = a.blk_tbase[i], mb = a.blk_mbase[i];
+        const uint32_t bnm = (bnt + kMeshletPrims - 1) / kMeshletPrims;
+        if (3 * (tb + bnt) > a.icap || (a.meshlets && mb + bnm > a.mcap)) {
+            if (lane == 0) atomicOr(reinterpret_cast<unsigned long long*>(&a.totals[2]), 1ull);
+            continue;
+        }
+        const uint32_t id = a.ids[i];
+        const uint32_t bx = id % a.grid.gx, by = (id / a.grid.gx) % a.grid.gy;
+        const uint32_t bz = id / (a.grid.gx * a.grid.gy);
+        LaneCorners<B> lc;
+        lc.load(table, a.cases, i, lane, true);
+        if (lane < 8) {
+            int32_t r = -1;
+            uint32_t base = 0;
+            const uint32_t nx = bx + (lane & 1u), ny = by + ((lane >> 1) & 1u), nz = bz + (lane >> 2);
+            if (lane != 0 && lane != 7 && nx < a.grid.gx && ny < a.grid.gy && nz < a.grid.gz) {
+                r = (int32_t)a.blk_map[nx + a.grid.gx * (ny + (uint64_t)a.grid.gy * nz)];
+                if (r >= 0) base = (uint32_t)a.blk_vbase[r];
+            }
+            s_nbr[lane] = r;
+            s_nbase[lane] = base;
+        }
+        // own corners: vertex index = block base + prefix from the count pass
+        uint32_t nt = 0;
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
+            const uint32_t c = lane * CPL + j;
+            if (c < (uint32_t)NC) {
+                const uint32_t q = c % B + F * ((c / B) % B + F * (c / (B * B)));
+                s_gidx[q] = (uint32_t)vb + (lc.ent(j) >> 3);
+                s_gfl[q] = (uint8_t)(lc.ent(j) & 7u);
+            }
+            nt += mtx_word_ntri(s_words[lc.cs(j)]);
+        }
+        wave_lds_sync();
+        // neighbour-owned corners (a coordinate equal to B): one batched gather.
+        // Face corner f enumerates x = B (F*F corners), then y = B with x < B
+        // (B*F), then z = B with x, y < B (B*B): F^3 - B^3 in all.
+        {
+            constexpr int NFACE = NF - NC;
+            constexpr int NPL = (NFACE + 63) / 64;
+            Entry fent[NPL];
+            int32_t jn[NPL];
+#pragma unroll
+            for (int k = 0; k < NPL; ++k) {
+                const uint32_t f = lane + 64u * k;
+                uint32_t qx, qy, qz;
+                if (f < (uint32_t)(F * F)) {
+                    qx = B; qy = f % F; qz = f / F;
+                } else if (f < (uint32_t)(F * F + B * F)) {
+                    const uint32_t g = f - F * F;
+                    qx = g % B; qy = B; qz = g / B;
+                } else {
+                    const uint32_t g = f - F * F - B * F;
+                    qx = g % B; qy = g / B; qz = B;
+                }
+                const uint32_t nx = qx == (uint32_t)B, ny = qy == (uint32_t)B, nz = qz == (uint32_t)B;
+                jn[k] = f < (uint32_t)NFACE ? s_nbr[nx | (ny << 1) | (nz << 2)] : -1;
+                fent[k] = 0;
+                if (jn[k] >= 0)
+                    fent[k] = table[(uint64_t)jn[k] * NC + (qx - B * nx) + B * ((qy - B * ny) + B * (qz - B * nz))];
+            }
+#pragma unroll
+            for (int k = 0; k < NPL; ++k) {
+                const uint32_t f = lane + 64u * k;
+                if (f >= (uint32_t)NFACE) continue;
+                uint32_t qx, qy, qz;
+                if (f < (uint32_t)(F * F)) {
+                    qx = B; qy = f % F; qz = f / F;
+                } else if (f < (uint32_t)(F * F + B * F)) {
+                    const uint32_t g = f - F * F;
+                    qx = g % B; qy = B; qz = g / B;
+                } else {
+                    const uint32_t g = f - F * F - B * F;
+                    qx = g % B; qy = g / B; qz = B;
+                }
+                const uint32_t nsel = (qx == (uint32_t)B) | ((qy == (uint32_t)B) << 1) | ((qz == (uint32_t)B) << 2);
+                const uint32_t q = qx + F * (qy + F * qz);
+                s_gidx[q] = jn[k] >= 0 ? s_nbase[nsel] + (uint32_t)(fent[k] >> 3) : 0u;
+                s_gfl[q] = jn[k] >= 0 ? (uint8_t)(fent[k] & 7u) : (uint8_t)0xF0u;
+            }
+        }
+        uint32_t tt;
+        const uint32_t et = wave_excl_scan<uint32_t>(nt, tt);
         for (uint32_t c0 = 0; c0 < tt; c0 += LIST) {
             uint32_t run = et;
 #pragma unroll
             for (int j = 0; j < CPL; ++j) {
-                if (cs[j] == 0xFFFFFFFFu) continue;
                 const uint32_t c = lane * CPL + j;
-                const uint32_t ntri = mtx_word_ntri(s_words[cs[j]]);
+                const uint32_t cs = lc.cs(j);
+                const uint32_t ntri = mtx_word_ntri(s_words[cs]);
                 for (uint32_t k = 0; k < ntri; ++k, ++run)
-                    if (run - c0 < LIST) s_tl[run - c0] = c | (k << 12) | (cs[j] << 16);
+                    if (run - c0 < LIST) s_tl[run - c0] = c | (k << 12) | (cs << 16);
             }
             wave_lds_sync();
             const uint32_t n = min(LIST, tt - c0);
@@ -1292,6 +1471,13 @@ void k_extract_emit(ExtractArgs a)
 template <int B>
 static hipError_t launch_count_b(const ExtractArgs& a, hipStream_t s, uint32_t nwg)
 {
+    if (B == 8) {
+        if (a.vol.dtype == kF32)
+            hipLaunchKernelGGL(k_extract_count_b8<float>, dim3(nwg), dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL(k_extract_count_b8<uint8_t>, dim3(nwg), dim3(256), 0, s, a);
+        return hipGetLastError();
+    }
     if (a.vol.dtype == kF32)
         hipLaunchKernelGGL((k_extract_count<B, float>), dim3(nwg), dim3(256), 0, s, a);
     else
@@ -1302,15 +1488,19 @@ static hipError_t launch_count_b(const ExtractArgs& a, hipStream_t s, uint32_t n
 template <int B>
 static hipError_t launch_emit_b(const ExtractArgs& a, hipStream_t s, uint32_t nwg)
 {
-    const uint32_t tn = 64 * EmitCfg<B, true>::WPG, tf = 64 * EmitCfg<B, false>::WPG;
-    const uint32_t gn = nwg * (4 / EmitCfg<B, true>::WPG), gf = nwg * (4 / EmitCfg<B, false>::WPG);
+    const uint32_t tn = 64 * VertCfg<B, true>::WPG, tf = 64 * VertCfg<B, false>::WPG;
+    const uint32_t gn = nwg * (4 / VertCfg<B, true>::WPG), gf = nwg * (4 / VertCfg<B, false>::WPG);
     if (a.vol.dtype == kF32) {
-        if (a.normals) hipLaunchKernelGGL((k_extract_emit<B, float, true>), dim3(gn), dim3(tn), 0, s, a);
-        else hipLaunchKernelGGL((k_extract_emit<B, float, false>), dim3(gf), dim3(tf), 0, s, a);
+        if (a.normals) hipLaunchKernelGGL((k_extract_vertices<B, float, true>), dim3(gn), dim3(tn), 0, s, a);
+        else hipLaunchKernelGGL((k_extract_vertices<B, float, false>), dim3(gf), dim3(tf), 0, s, a);
     } else {
-        if (a.normals) hipLaunchKernelGGL((k_extract_emit<B, uint8_t, true>), dim3(gn), dim3(tn), 0, s, a);
-        else hipLaunchKernelGGL((k_extract_emit<B, uint8_t, false>), dim3(gf), dim3(tf), 0, s, a);
+        if (a.normals) hipLaunchKernelGGL((k_extract_vertices<B, uint8_t, true>), dim3(gn), dim3(tn), 0, s, a);
+        else hipLaunchKernelGGL((k_extract_vertices<B, uint8_t, false>), dim3(gf), dim3(tf), 0, s, a);
     }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const uint32_t tt = 64 * TriCfg<B>::WPG, gt = nwg * (4 / TriCfg<B>::WPG);
+    hipLaunchKernelGGL((k_extract_triangles<B>), dim3(gt), dim3(tt), 0, s, a);
     return hipGetLastError();
 }
 
