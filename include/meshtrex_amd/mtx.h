@@ -221,6 +221,46 @@ mtx_status mtx_extract(mtx_context* ctx, const mtx_minmax_output* mm,
 mtx_status mtx_read_extraction_counts(mtx_context* ctx, mtx_extraction_output* out);
 void mtx_extraction_release(mtx_context* ctx, mtx_extraction_output* out);
 
+/* ---- differentiation ------------------------------------------------------ */
+
+/* Adjoint of the extraction with its topology frozen: from dL/d(vertex
+ * position) -- `dl_dpos`, 3 floats per vertex in voxel units, in the vertex
+ * order of `ext` -- to dL/d(voxel value) (`dl_dvol`, one float per voxel,
+ * overwritten everywhere, zero where no vertex depends on the voxel) and
+ * dL/d(isovalue) (`dl_diso`, one float).  Positions are p = owner + t * e_axis
+ * with t = (iso - v0) / (v1 - v0) (src/shaders/marching_cubes_pmb.mesh.glsl:122-141);
+ * t is recomputed from the volume `mm` describes, which must be the one `ext`
+ * was extracted from.  No reference counterpart: the reference has no
+ * differentiation.  Deterministic (no atomics). */
+mtx_status mtx_adjoint(mtx_context* ctx, const mtx_minmax_output* mm, const mtx_filtering_output* filt,
+                       const mtx_extraction_output* ext, const mtx_push_constants* pc,
+                       const float* dl_dpos, float* dl_dvol, float* dl_diso, void* stream);
+
+/* ---- profiling (the reference's GPUProfiler regions, src/gpuProfiler.h:44-107) ---- */
+
+/* Named GPU-time regions recorded with HIP events on the stream a stage runs
+ * on; the managers use the reference's region names MinMax_Leaf_Pass,
+ * MinMax_Octree_Reduction, Active_Block_Filtering and Mesh_Extraction
+ * (src/minMaxManager.cpp:130,141, src/filteringManager.cpp:85,
+ * src/extractionManager.cpp:635). */
+typedef struct mtx_profiler mtx_profiler;
+mtx_status mtx_profiler_create(mtx_profiler** out);
+void mtx_profiler_destroy(mtx_profiler* p);
+mtx_status mtx_profiler_begin(mtx_profiler* p, const char* name, void* stream);
+mtx_status mtx_profiler_end(mtx_profiler* p, void* stream);
+/* Blocking: milliseconds of region `index` (in begin order); name copied into
+ * `name` (capacity `cap`).  Returns MTX_EINVAL past the last region. */
+mtx_status mtx_profiler_region(mtx_profiler* p, uint32_t index, char* name, uint32_t cap, float* ms);
+void mtx_profiler_reset(mtx_profiler* p);
+
+/* ---- on-disk output -------------------------------------------------------- */
+
+/* Wavefront OBJ of an extraction, as the reference's writeGPUExtractionToOBJ
+ * writes it (src/extractionTestUtils.cpp:328-366): "v x y z" per vertex (the
+ * NDC position), "vn x y z" per vertex, "f a//a b//b c//c" per triangle,
+ * 1-based.  Blocking. */
+mtx_status mtx_write_obj(mtx_context* ctx, mtx_extraction_output* ext, const char* path);
+
 /* ---- device memory plumbing (for bindings without a device allocator) --- */
 
 mtx_status mtx_device_malloc(void** ptr, uint64_t bytes);
@@ -228,6 +268,7 @@ mtx_status mtx_device_free(void* ptr);
 mtx_status mtx_memcpy_h2d(void* dst, const void* src, uint64_t bytes);
 mtx_status mtx_memcpy_d2h(void* dst, const void* src, uint64_t bytes);
 mtx_status mtx_memcpy_d2d(void* dst, const void* src, uint64_t bytes);
+mtx_status mtx_memcpy_d2d_async(void* dst, const void* src, uint64_t bytes, void* stream);
 mtx_status mtx_stream_synchronize(void* stream);
 mtx_status mtx_device_synchronize(void);
 uint32_t mtx_key_f32(float f);

@@ -8,5 +8,5 @@ the kernels (csrc/), the in-tree build (build.py), the ctypes view of the ABI
 from .managers import (  # noqa: F401
     Context, DeviceBuffer, Volume, MinMaxOutput, FilteringOutput, ExtractionOutput, MtxError,
     computeMinMaxMip, filterActiveBlocks, readActiveBlockCount, extractMeshletDescriptors,
-    loadVolume, makePushConstants, makeOptions, read_device,
+    loadVolume, makePushConstants, makeOptions, read_device, MtxProfiler, writeGPUExtractionToOBJ,
 )

@@ -90,11 +90,22 @@ _SIGS = {
     "mtx_memcpy_h2d": (ctypes.c_int, [vp, vp, u64]),
     "mtx_memcpy_d2h": (ctypes.c_int, [vp, vp, u64]),
     "mtx_memcpy_d2d": (ctypes.c_int, [vp, vp, u64]),
+    "mtx_memcpy_d2d_async": (ctypes.c_int, [vp, vp, u64, vp]),
     "mtx_stream_synchronize": (ctypes.c_int, [vp]),
     "mtx_device_synchronize": (ctypes.c_int, []),
     "mtx_key_f32": (u32, [f32]),
     "mtx_unkey_f32": (f32, [u32]),
     "mtx_struct_size": (u64, [ctypes.c_char_p]),
+    "mtx_profiler_create": (ctypes.c_int, [ctypes.POINTER(vp)]),
+    "mtx_profiler_destroy": (None, [vp]),
+    "mtx_profiler_begin": (ctypes.c_int, [vp, ctypes.c_char_p, vp]),
+    "mtx_profiler_end": (ctypes.c_int, [vp, vp]),
+    "mtx_profiler_region": (ctypes.c_int, [vp, u32, ctypes.c_char_p, u32, ctypes.POINTER(f32)]),
+    "mtx_profiler_reset": (None, [vp]),
+    "mtx_write_obj": (ctypes.c_int, [vp, ctypes.POINTER(CExtractionOutput), ctypes.c_char_p]),
+    "mtx_adjoint": (ctypes.c_int, [vp, ctypes.POINTER(CMinMaxOutput), ctypes.POINTER(CFilteringOutput),
+                                   ctypes.POINTER(CExtractionOutput), ctypes.POINTER(PushConstants),
+                                   vp, vp, vp, vp]),
 }
 
 SYNTH_SIGS = {

@@ -43,6 +43,18 @@ def _compile(src):
     return obj
 
 
+def build_demo(verbose=False):
+    """The C++ host-API demo (tests/cpp/managers_demo.cpp): host code only."""
+    src = os.path.join(ROOT, "tests", "cpp", "managers_demo.cpp")
+    if not os.path.exists(src):
+        return
+    out = os.path.join(ROOT, "tests", "cpp", "managers_demo")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"), src, "-o", out,
+          "-L" + HERE, "-lmeshtrex_amd", "-Wl,-rpath,$ORIGIN/../../meshtrex_amd"])
+    if verbose:
+        print("built", out)
+
+
 def build(verbose=False):
     os.makedirs(OBJ, exist_ok=True)
     srcs = [s for v in LIBS.values() for s in v]
@@ -56,6 +68,7 @@ def build(verbose=False):
         os.replace(tmp, out)
         if verbose:
             print("built", out)
+    build_demo(verbose)
 
 
 if __name__ == "__main__":

@@ -8,6 +8,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <vector>
 
 struct mtx_context {
     int device = 0;
@@ -106,7 +107,7 @@ uint32_t pyramid_layout(const uint32_t G[3], uint32_t (*ext)[3], uint64_t* off)
 
 // Extraction workspace layout (offsets in bytes).
 struct WsLayout {
-    uint64_t counts, vbase, tbase, mbase, table, cases, totals, scan, bytes;
+    uint64_t counts, vbase, tbase, mbase, table, cases, adj, totals, scan, bytes;
     uint32_t scan_tiles;
 };
 
@@ -120,6 +121,7 @@ WsLayout ws_layout(uint64_t max_blocks, uint32_t B)
     w.mbase = o;  o = align256(o + 8 * max_blocks);
     w.table = o;  o = align256(o + corner_entry_bytes(B) * (uint64_t)B * B * B * max_blocks);
     w.cases = o;  o = align256(o + (uint64_t)B * B * B * max_blocks);
+    w.adj = o;    o = align256(o + 8ull * 4 * adjoint_workgroups(max_blocks));
     w.totals = o; o = align256(o + 8 * 4);
     uint64_t tiles = (max_blocks + scan_tile_items() - 1) / scan_tile_items();
     if (tiles == 0) tiles = 1;
@@ -579,6 +581,36 @@ mtx_status mtx_extract(mtx_context* ctx, const mtx_minmax_output* mm,
     return finish(bd);
 }
 
+mtx_status mtx_adjoint(mtx_context* ctx, const mtx_minmax_output* mm, const mtx_filtering_output* filt,
+                       const mtx_extraction_output* ext, const mtx_push_constants* pc,
+                       const float* dl_dpos, float* dl_dvol, float* dl_diso, void* stream)
+{
+    if (!ctx) return MTX_EINVAL;
+    if (!mm || !filt || !ext || !pc || !dl_dvol || !dl_diso) return fail(ctx, MTX_EINVAL, "null argument");
+    if (!ext->workspace || !mm->volume || !filt->block_to_active)
+        return fail(ctx, MTX_ESTATE, "adjoint needs the forward pass's outputs");
+    if (ext->vertexCount && !dl_dpos) return fail(ctx, MTX_EINVAL, "dl_dpos is null");
+    const uint32_t B = mm->block_dim;
+    const uint64_t nb = (uint64_t)mm->grid[0] * mm->grid[1] * mm->grid[2];
+    if (ext->block_dim != B || ext->max_blocks < nb)
+        return fail(ctx, MTX_ESTATE, "extraction output does not belong to this min/max output");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    Bound bd;
+    if (mtx_status st = bind(ctx, stream, bd)) return st;
+    const WsLayout w = ws_layout(ext->max_blocks, B);
+    mtx_extraction_output tmp = *ext;
+    AdjointArgs a{};
+    a.fwd = make_extract_args(mm, filt, pc, &tmp, w);
+    a.dl_dpos = dl_dpos;
+    a.dl_dvol = dl_dvol;
+    a.dl_diso = dl_diso;
+    a.iso_partials = reinterpret_cast<double*>(static_cast<char*>(ext->workspace) + w.adj);
+    a.n_wg = adjoint_workgroups(nb);
+    hipError_t e = launch_adjoint(a, bd.s);
+    if (e != hipSuccess) return fail(ctx, MTX_EDEVICE, "adjoint launch: %s", hipGetErrorString(e));
+    return finish(bd);
+}
+
 mtx_status mtx_read_extraction_counts(mtx_context* ctx, mtx_extraction_output* out)
 {
     if (!ctx) return MTX_EINVAL;
@@ -642,9 +674,114 @@ mtx_status mtx_memcpy_d2d(void* dst, const void* src, uint64_t bytes)
     return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice) == hipSuccess ? MTX_OK : MTX_EDEVICE;
 }
 
+mtx_status mtx_memcpy_d2d_async(void* dst, const void* src, uint64_t bytes, void* stream)
+{
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, static_cast<hipStream_t>(stream)) == hipSuccess
+               ? MTX_OK : MTX_EDEVICE;
+}
+
 mtx_status mtx_stream_synchronize(void* stream)
 {
     return hipStreamSynchronize(static_cast<hipStream_t>(stream)) == hipSuccess ? MTX_OK : MTX_EDEVICE;
+}
+
+// ---- profiler -----------------------------------------------------------------
+
+struct mtx_profiler {
+    struct Region {
+        std::string name;
+        hipEvent_t begin = nullptr, end = nullptr;
+    };
+    std::vector<Region> regions;
+    std::vector<size_t> open;
+};
+
+mtx_status mtx_profiler_create(mtx_profiler** out)
+{
+    if (!out) return MTX_EINVAL;
+    *out = new (std::nothrow) mtx_profiler;
+    return *out ? MTX_OK : MTX_ENOMEM;
+}
+
+void mtx_profiler_reset(mtx_profiler* p)
+{
+    if (!p) return;
+    for (auto& r : p->regions) {
+        if (r.begin) hipEventDestroy(r.begin);
+        if (r.end) hipEventDestroy(r.end);
+    }
+    p->regions.clear();
+    p->open.clear();
+}
+
+void mtx_profiler_destroy(mtx_profiler* p)
+{
+    mtx_profiler_reset(p);
+    delete p;
+}
+
+mtx_status mtx_profiler_begin(mtx_profiler* p, const char* name, void* stream)
+{
+    if (!p || !name) return MTX_EINVAL;
+    mtx_profiler::Region r;
+    r.name = name;
+    if (hipEventCreate(&r.begin) != hipSuccess || hipEventCreate(&r.end) != hipSuccess) return MTX_EDEVICE;
+    if (hipEventRecord(r.begin, static_cast<hipStream_t>(stream)) != hipSuccess) return MTX_EDEVICE;
+    p->regions.push_back(r);
+    p->open.push_back(p->regions.size() - 1);
+    return MTX_OK;
+}
+
+mtx_status mtx_profiler_end(mtx_profiler* p, void* stream)
+{
+    if (!p || p->open.empty()) return MTX_ESTATE;
+    const size_t i = p->open.back();
+    p->open.pop_back();
+    return hipEventRecord(p->regions[i].end, static_cast<hipStream_t>(stream)) == hipSuccess ? MTX_OK : MTX_EDEVICE;
+}
+
+mtx_status mtx_profiler_region(mtx_profiler* p, uint32_t index, char* name, uint32_t cap, float* ms)
+{
+    if (!p || index >= p->regions.size()) return MTX_EINVAL;
+    auto& r = p->regions[index];
+    if (hipEventSynchronize(r.end) != hipSuccess) return MTX_EDEVICE;
+    float t = 0.f;
+    if (hipEventElapsedTime(&t, r.begin, r.end) != hipSuccess) return MTX_EDEVICE;
+    if (ms) *ms = t;
+    if (name && cap) {
+        std::strncpy(name, r.name.c_str(), cap - 1);
+        name[cap - 1] = 0;
+    }
+    return MTX_OK;
+}
+
+// ---- OBJ --------------------------------------------------------------------
+
+mtx_status mtx_write_obj(mtx_context* ctx, mtx_extraction_output* ext, const char* path)
+{
+    if (!ctx) return MTX_EINVAL;
+    if (!ext || !path) return fail(ctx, MTX_EINVAL, "null argument");
+    if (mtx_status st = mtx_read_extraction_counts(ctx, ext)) return st;
+    const uint64_t nv = ext->vertexCount, ni = ext->indexCount;
+    std::vector<mtx_vertex> v(nv);
+    std::vector<uint32_t> idx(ni);
+    if (nv) MTX_HIP(ctx, hipMemcpy(v.data(), ext->vertices, nv * sizeof(mtx_vertex), hipMemcpyDeviceToHost));
+    if (ni) MTX_HIP(ctx, hipMemcpy(idx.data(), ext->indices, ni * 4, hipMemcpyDeviceToHost));
+    FILE* f = std::fopen(path, "w");
+    if (!f) return fail(ctx, MTX_EINVAL, "cannot open %s for writing", path);
+    std::fprintf(f, "# isosurface: %llu vertices (one per active edge), %llu triangles\n\n",
+                 (unsigned long long)nv, (unsigned long long)(ni / 3));
+    std::fprintf(f, "# Vertex positions\n");
+    for (const auto& p : v) std::fprintf(f, "v %g %g %g\n", p.position[0], p.position[1], p.position[2]);
+    std::fprintf(f, "\n# Vertex normals\n");
+    for (const auto& p : v) std::fprintf(f, "vn %g %g %g\n", p.normal[0], p.normal[1], p.normal[2]);
+    std::fprintf(f, "\n# Faces\n");
+    for (uint64_t i = 0; i + 2 < ni; i += 3) {
+        const uint32_t a = idx[i] + 1, b = idx[i + 1] + 1, c = idx[i + 2] + 1;
+        std::fprintf(f, "f %u//%u %u//%u %u//%u\n", a, a, b, b, c, c);
+    }
+    const bool ok = std::fclose(f) == 0;
+    return ok ? MTX_OK : fail(ctx, MTX_EDEVICE, "write of %s failed", path);
 }
 
 uint64_t mtx_struct_size(const char* name)

@@ -109,16 +109,14 @@ hipError_t launch_extract_emit(const ExtractArgs& a, hipStream_t s);
 // Adjoint: dL/dvoxel (dense, overwritten) and dL/diso (one float, overwritten)
 // from dL/d(vertex position in voxel units), topology of the forward pass.
 struct AdjointArgs {
-    ExtractArgs fwd;            // forward topology (ids, count, corner tables, bases)
-    const int32_t* blk_to_active; // dense map block -> active index or -1
+    ExtractArgs fwd;            // forward topology (map, corner tables, vertex bases)
     const float* dl_dpos;       // 3 per vertex
     float* dl_dvol;             // one per voxel
     float* dl_diso;             // one float
-    double* iso_partials;       // scratch: one double per adjoint workgroup
+    double* iso_partials;       // one double per adjoint wave
     uint32_t n_wg;
 };
-hipError_t launch_block_to_active(const uint32_t* ids, const uint32_t* count, const Grid& g,
-                                  int32_t* map, hipStream_t s);
+uint32_t adjoint_workgroups(uint64_t nb);
 hipError_t launch_adjoint(const AdjointArgs& a, hipStream_t s);
 
 }  // namespace mtx

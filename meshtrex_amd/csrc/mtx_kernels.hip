@@ -1558,4 +1558,175 @@ hipError_t launch_extract_scan(const ExtractArgs& a, ScanState st, hipStream_t s
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// adjoint of the vertex positions
+//
+// p = owner + t * e_axis (voxel units), t = (iso - v0) / (v1 - v0) with the
+// topology of the forward pass frozen:
+//   dt/dv0 = (t - 1) / (v1 - v0),  dt/dv1 = -t / (v1 - v0),  dt/diso = 1 / (v1 - v0).
+// Every voxel belongs to one home block (the block holding it as a corner); a
+// wave per home block gathers, for each of its voxels, the contributions of
+// the voxel's owned edges (it is v0) and of the three edges owned by its -x,
+// -y, -z neighbours (it is v1), looking the vertices up in the forward pass's
+// corner tables and recomputing t from the volume.  Each voxel is written once
+// (zeros where no edge touches it): no atomics, a deterministic result.
+// dL/diso is summed per wave in a fixed order and the per-wave partials are
+// added in order by a second kernel.
+// ---------------------------------------------------------------------------
+
+template <int B, typename T>
+__global__ __launch_bounds__(256) void k_adjoint(AdjointArgs a)
+{
+    constexpr int E = B + 2;  // voxels [-1, B] per axis
+    constexpr int NTILE = E * E * E;
+    constexpr int NC = EB<B>::NC;
+    constexpr int CPL = EB<B>::CPL;
+    using Entry = typename EB<B>::Entry;
+    __shared__ float s_all[4][NTILE];
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    float* s_val = s_all[w];
+    const ExtractArgs& f = a.fwd;
+    const T* __restrict__ vol = static_cast<const T*>(f.vol.data);
+    const uint32_t X = f.vol.X, Y = f.vol.Y, Z = f.vol.Z;
+    const float iso = f.iso;
+    const Entry* __restrict__ table = static_cast<const Entry*>(f.corner_table);
+    const uint32_t gx = f.grid.gx, gy = f.grid.gy, gz = f.grid.gz;
+    const uint32_t nb = gx * gy * gz;
+    const WaveItems it = wave_items(nb, 4);
+    double diso = 0.0;
+    for (uint32_t b = it.first; b < it.end; b += it.stride) {
+        const uint32_t bx = b % gx, by = (b / gx) % gy, bz = b / (gx * gy);
+        const uint32_t ox = bx * B, oy = by * B, oz = bz * B;
+        const int32_t s = (int32_t)f.blk_map[b];
+        const int32_t nxm = bx > 0 ? (int32_t)f.blk_map[b - 1] : -1;
+        const int32_t nym = by > 0 ? (int32_t)f.blk_map[b - gx] : -1;
+        const int32_t nzm = bz > 0 ? (int32_t)f.blk_map[b - gx * gy] : -1;
+        const bool work = s >= 0 || nxm >= 0 || nym >= 0 || nzm >= 0;
+        if (work) {
+            for (uint32_t j = lane; j < (uint32_t)NTILE; j += 64u) {
+                const int32_t x = min(max((int32_t)(ox + j % E) - 1, 0), (int32_t)X - 1);
+                const int32_t y = min(max((int32_t)(oy + (j / E) % E) - 1, 0), (int32_t)Y - 1);
+                const int32_t z = min(max((int32_t)(oz + j / (E * E)) - 1, 0), (int32_t)Z - 1);
+                s_val[j] = vfloat(vol[(uint32_t)x + (uint64_t)X * ((uint32_t)y + (uint64_t)Y * (uint32_t)z)]);
+            }
+            wave_lds_sync();
+        }
+        const uint64_t vbs = s >= 0 ? f.blk_vbase[s] : 0;
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
+            const uint32_t c = lane * CPL + j;
+            if (c >= (uint32_t)NC) continue;
+            const uint32_t lx = c % B, ly = (c / B) % B, lz = c / (B * B);
+            const uint32_t x = ox + lx, y = oy + ly, z = oz + lz;
+            if (!(x < X && y < Y && z < Z)) continue;
+            float g = 0.0f;
+            if (work) {
+                const uint32_t t0 = (lx + 1) + E * ((ly + 1) + E * (lz + 1));
+                const float vg = s_val[t0];
+                if (s >= 0) {
+                    const uint32_t e = table[(uint64_t)s * NC + c];
+#pragma unroll
+                    for (uint32_t ax = 0; ax < 3; ++ax) {
+                        if (!((e >> ax) & 1u)) continue;
+                        const uint64_t vi = vbs + (e >> 3) + __popc(e & 7u & ((1u << ax) - 1u));
+                        const float G = a.dl_dpos[3 * vi + ax];
+                        const uint32_t st = ax == 0 ? 1u : (ax == 1 ? (uint32_t)E : (uint32_t)(E * E));
+                        const float d = s_val[t0 + st] - vg;
+                        const float t = (iso - vg) / d;
+                        g += G * (t - 1.0f) / d;
+                        diso += (double)(G / d);
+                    }
+                }
+#pragma unroll
+                for (uint32_t ax = 0; ax < 3; ++ax) {
+                    const uint32_t l = ax == 0 ? lx : (ax == 1 ? ly : lz);
+                    const uint32_t gcoord = ax == 0 ? x : (ax == 1 ? y : z);
+                    if (gcoord == 0) continue;  // no voxel at -1
+                    int32_t n;
+                    uint32_t oc;
+                    if (l == 0) {
+                        n = ax == 0 ? nxm : (ax == 1 ? nym : nzm);
+                        oc = ax == 0 ? (B - 1) + B * (ly + B * lz)
+                                     : (ax == 1 ? lx + B * ((B - 1) + B * lz) : lx + B * (ly + B * (B - 1)));
+                    } else {
+                        n = s;
+                        oc = c - (ax == 0 ? 1u : (ax == 1 ? (uint32_t)B : (uint32_t)(B * B)));
+                    }
+                    if (n < 0) continue;
+                    const uint32_t e = table[(uint64_t)n * NC + oc];
+                    if (!((e >> ax) & 1u)) continue;
+                    const uint64_t vi = f.blk_vbase[n] + (e >> 3) + __popc(e & 7u & ((1u << ax) - 1u));
+                    const float G = a.dl_dpos[3 * vi + ax];
+                    const uint32_t st = ax == 0 ? 1u : (ax == 1 ? (uint32_t)E : (uint32_t)(E * E));
+                    const float v0 = s_val[t0 - st];
+                    const float d = vg - v0;
+                    const float t = (iso - v0) / d;
+                    g += -G * t / d;
+                }
+            }
+            a.dl_dvol[x + (uint64_t)X * (y + (uint64_t)Y * z)] = g;
+        }
+        if (work) wave_lds_sync();
+    }
+    // fixed-order wave reduction, one partial per wave
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) diso += __shfl_xor(diso, d, 64);
+    if (lane == 0) a.iso_partials[blockIdx.x * 4 + w] = diso;
+}
+
+__global__ __launch_bounds__(64) void k_adjoint_iso(const double* __restrict__ partials, uint32_t n,
+                                                    float* __restrict__ out)
+{
+    // lane-strided sums, then a fixed shuffle tree: same order on every run
+    double s = 0.0;
+    for (uint32_t i = threadIdx.x; i < n; i += 64u) s += partials[i];
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) s += __shfl_xor(s, d, 64);
+    if (threadIdx.x == 0) *out = (float)s;
+}
+
+template <int B>
+static hipError_t launch_adjoint_b(const AdjointArgs& a, hipStream_t s)
+{
+    if (a.fwd.vol.dtype == kF32)
+        hipLaunchKernelGGL((k_adjoint<B, float>), dim3(a.n_wg), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_adjoint<B, uint8_t>), dim3(a.n_wg), dim3(256), 0, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_adjoint_iso, dim3(1), dim3(64), 0, s, a.iso_partials, a.n_wg * 4, a.dl_diso);
+    return hipGetLastError();
+}
+
+uint32_t adjoint_workgroups(uint64_t nb)
+{
+    uint64_t need = (nb + 3) / 4;
+    uint32_t n = 2048;
+    if (need < n) n = need < 8 ? 8 : (uint32_t)(((need + 7) / 8) * 8);
+    return n;
+}
+
+hipError_t launch_adjoint(const AdjointArgs& a, hipStream_t s)
+{
+    switch (a.fwd.grid.B) {
+    case 1: return launch_adjoint_b<1>(a, s);
+    case 2: return launch_adjoint_b<2>(a, s);
+    case 3: return launch_adjoint_b<3>(a, s);
+    case 4: return launch_adjoint_b<4>(a, s);
+    case 5: return launch_adjoint_b<5>(a, s);
+    case 6: return launch_adjoint_b<6>(a, s);
+    case 7: return launch_adjoint_b<7>(a, s);
+    case 8: return launch_adjoint_b<8>(a, s);
+    case 9: return launch_adjoint_b<9>(a, s);
+    case 10: return launch_adjoint_b<10>(a, s);
+    case 11: return launch_adjoint_b<11>(a, s);
+    case 12: return launch_adjoint_b<12>(a, s);
+    case 13: return launch_adjoint_b<13>(a, s);
+    case 14: return launch_adjoint_b<14>(a, s);
+    case 15: return launch_adjoint_b<15>(a, s);
+    case 16: return launch_adjoint_b<16>(a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 }  // namespace mtx

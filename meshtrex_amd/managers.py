@@ -306,13 +306,63 @@ class ExtractionOutput:
             pass
 
 
+class MtxProfiler:
+    """GPU-time regions (the reference's GPUProfiler, src/gpuProfiler.h:44-107),
+    recorded with HIP events on the stream each stage runs on."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        if L.lib().mtx_profiler_create(ctypes.byref(h)) != L.MTX_OK:
+            raise MtxError(L.MTX_ENOMEM, "profiler")
+        self.handle = h
+
+    def begin(self, name, stream=None):
+        L.lib().mtx_profiler_begin(self.handle, name.encode(), _stream(stream))
+
+    def end(self, stream=None):
+        L.lib().mtx_profiler_end(self.handle, _stream(stream))
+
+    def results(self):
+        """[(region name, milliseconds)] in begin order (blocks on the GPU)."""
+        out, i = [], 0
+        buf = ctypes.create_string_buffer(128)
+        ms = ctypes.c_float()
+        while L.lib().mtx_profiler_region(self.handle, i, buf, 128, ctypes.byref(ms)) == L.MTX_OK:
+            out.append((buf.value.decode(), ms.value))
+            i += 1
+        return out
+
+    def reset(self):
+        L.lib().mtx_profiler_reset(self.handle)
+
+    def __del__(self):
+        try:
+            L.lib().mtx_profiler_destroy(self.handle)
+        except Exception:
+            pass
+
+
 def computeMinMaxMip(context, volume, pushConstants, stream=None, profiler=None, options=None):
     out = MinMaxOutput(context)
     v = volume._c()
+    o = options if options is not None else makeOptions()
+    build = o.build_pyramid
+    leaf = L.Options.from_buffer_copy(o)
+    leaf.build_pyramid = 0
+    if profiler:
+        profiler.begin("MinMax_Leaf_Pass", stream)
     st = L.lib().mtx_compute_min_max(context.handle, ctypes.byref(v), ctypes.byref(pushConstants),
-                                     ctypes.byref(options) if options is not None else None,
-                                     _stream(stream), ctypes.byref(out.c))
+                                     ctypes.byref(leaf), _stream(stream), ctypes.byref(out.c))
+    if profiler:
+        profiler.end(stream)
     _check(context, st)
+    if build and out.c.num_levels > 1:
+        if profiler:
+            profiler.begin("MinMax_Octree_Reduction", stream)
+        st = L.lib().mtx_build_pyramid(context.handle, ctypes.byref(out.c), _stream(stream))
+        if profiler:
+            profiler.end(stream)
+        _check(context, st)
     out._volume_ref = volume
     return out
 
@@ -320,10 +370,14 @@ def computeMinMaxMip(context, volume, pushConstants, stream=None, profiler=None,
 def filterActiveBlocks(context, minMaxOutput, pushConstants, stream=None, profiler=None,
                        options=None, out=None):
     out = out if out is not None else FilteringOutput(context)
+    if profiler:
+        profiler.begin("Active_Block_Filtering", stream)
     st = L.lib().mtx_filter_active_blocks(context.handle, ctypes.byref(minMaxOutput.c),
                                           ctypes.byref(pushConstants),
                                           ctypes.byref(options) if options is not None else None,
                                           _stream(stream), ctypes.byref(out.c))
+    if profiler:
+        profiler.end(stream)
     _check(context, st)
     return out
 
@@ -335,12 +389,22 @@ def readActiveBlockCount(context, filteringOutput):
 def extractMeshletDescriptors(context, minMaxOutput, filterOutput, pushConstants, stream=None,
                               profiler=None, options=None, out=None):
     out = out if out is not None else ExtractionOutput(context)
+    if profiler:
+        profiler.begin("Mesh_Extraction", stream)
     st = L.lib().mtx_extract(context.handle, ctypes.byref(minMaxOutput.c), ctypes.byref(filterOutput.c),
                              ctypes.byref(pushConstants),
                              ctypes.byref(options) if options is not None else None,
                              _stream(stream), ctypes.byref(out.c))
+    if profiler:
+        profiler.end(stream)
     _check(context, st)
     return out
+
+
+def writeGPUExtractionToOBJ(context, extractionResult, filePath):
+    """OBJ as src/extractionTestUtils.cpp:328-366 writes it (v, vn, f a//a, 1-based)."""
+    _check(context, L.lib().mtx_write_obj(context.handle, ctypes.byref(extractionResult.c),
+                                          str(filePath).encode()))
 
 
 def key_f32(x):

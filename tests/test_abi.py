@@ -78,3 +78,30 @@ def test_context_create_fails_cleanly_without_gpu():
         pytest.skip("a GPU is present")
     h = ctypes.c_void_p()
     assert L.lib().mtx_context_create(0, ctypes.byref(h)) == L.MTX_EDEVICE
+
+
+def test_load_volume_naming_contract(tmp_path):
+    import numpy as np
+    from meshtrex_amd import loadVolume
+    p = tmp_path / "bonsai_4x3x2_uint8.raw"
+    np.arange(24, dtype=np.uint8).tofile(p)
+    v = loadVolume(p)
+    assert v.volume_dims == (4, 3, 2) and v.volume_type == "uint8" and v.volume_data[5] == 5
+    q = tmp_path / "sdf_2x2x2_float32.raw"
+    np.linspace(-1, 1, 8, dtype=np.float32).tofile(q)
+    assert loadVolume(q).volume_data.dtype == np.float32
+    with pytest.raises(RuntimeError, match="naming scheme"):
+        loadVolume(tmp_path / "noname.raw")
+    short = tmp_path / "s_4x4x4_uint8.raw"
+    short.write_bytes(b"\0" * 10)
+    with pytest.raises(RuntimeError, match="Failed to read"):
+        loadVolume(short)
+
+
+def test_cpp_header_compiles_standalone(tmp_path):
+    # managers.hpp + mtx.h need only a C++17 host compiler (no HIP headers)
+    import subprocess
+    src = tmp_path / "t.cpp"
+    src.write_text('#include "meshtrex_amd/managers.hpp"\nint main(){ meshtrex::PushConstants pc; return (int)sizeof(pc) - 64; }\n')
+    subprocess.check_call(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only",
+                           "-I" + os.path.join(ROOT, "include"), str(src)])
