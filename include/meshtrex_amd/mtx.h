@@ -99,7 +99,9 @@ typedef struct mtx_options {
                                 problem: no allocation, no host synchronisation.  For
                                 mtx_extract an overflow is then reported by
                                 mtx_read_extraction_counts (default 0) */
-    uint32_t _reserved[7];
+    uint32_t minmax_rows;    /* B = 8 min-max: y-blocks per wavefront (1, 2 or 4); 0 = chosen
+                                from the volume's shape (default) */
+    uint32_t _reserved[6];
 } mtx_options;
 
 /* Min/max of every block (level 0) and the min/max pyramid, as uint32 key
@@ -235,6 +237,100 @@ void mtx_extraction_release(mtx_context* ctx, mtx_extraction_output* out);
 mtx_status mtx_adjoint(mtx_context* ctx, const mtx_minmax_output* mm, const mtx_filtering_output* filt,
                        const mtx_extraction_output* ext, const mtx_push_constants* pc,
                        const float* dl_dpos, float* dl_dvol, float* dl_diso, void* stream);
+
+/* ---- batches ----------------------------------------------------------------
+ *
+ * Many volumes of any dimensions, each with one or more isovalues, extracted by
+ * one launch sequence: every (volume, isovalue) pair is a "unit".  Units are
+ * numbered volume by volume, isovalues in the order given.  mtx_batch_create
+ * plans the batch once -- it picks block edge and min/max wave shape per dtype
+ * (autotuned by timing the candidates on the batch itself unless fixed in the
+ * options), sizes the outputs from a planning run and, by default, captures the
+ * whole sequence (min/max, compaction, count, scan, vertices, triangles,
+ * per-unit ranges for every dtype class) in one hipGraph.  mtx_batch_run then
+ * replays it: no allocation, no host synchronisation.  Each unit's slice of
+ * the outputs equals what mtx_extract gives for that volume and isovalue at
+ * the unit's block edge, with indices and meshlet offsets relative to the
+ * unit's first vertex and index.  Volumes must stay device-resident at the
+ * same addresses; their contents may change between runs (a run whose
+ * outputs exceed the planned capacity flags the units concerned).  No
+ * reference counterpart: the reference extracts one volume at one isovalue
+ * per call (src/meshtrex.cpp:107-132). */
+
+typedef struct mtx_batch mtx_batch;
+
+typedef struct mtx_batch_volume {
+    mtx_volume volume;           /* device-resident: volume.on_device must be 1 */
+    const float* isovalues;      /* host array of num_isovalues, read by mtx_batch_create */
+    uint32_t num_isovalues;
+    uint32_t _pad;
+} mtx_batch_volume;
+
+typedef struct mtx_batch_options {
+    uint32_t block_dim;          /* 4, 8 or 16; 0 = autotuned per dtype (default) */
+    uint32_t minmax_rows;        /* y-blocks per min/max wavefront 1, 2, 4; 0 = autotuned (default) */
+    uint32_t rule;               /* activity rule (default MTX_RULE_CPU) */
+    uint32_t normals;            /* default 1 */
+    uint32_t edge_ids;           /* vertex_edge_ids output (default 0) */
+    uint32_t graph;              /* 1: replay one captured hipGraph (default); 0: launch directly */
+    uint32_t tune_iters;         /* timed runs per autotuner candidate (default 3) */
+    uint32_t headroom_pct;       /* output capacity above the planning run's counts (default 0) */
+    uint32_t _reserved[8];
+} mtx_batch_options;
+
+/* One unit's slice of the batch outputs, written on the device by every run. */
+typedef struct mtx_batch_range {
+    uint64_t vertex_offset;      /* into vertices (and vertex_edge_ids) */
+    uint64_t vertex_count;
+    uint64_t index_offset;       /* into indices; index values are relative to vertex_offset */
+    uint64_t index_count;
+    uint64_t meshlet_offset;     /* into meshlets; their offsets are relative to the unit */
+    uint64_t meshlet_count;
+    uint32_t active_offset;      /* into active_block_ids (block IDs within the unit's volume) */
+    uint32_t active_count;
+    uint32_t block_dim;          /* the unit's block edge */
+    uint32_t flags;              /* bit 0: capacity exceeded (output incomplete); bit 1: inconsistent list */
+} mtx_batch_range;
+
+typedef struct mtx_batch_output {
+    mtx_vertex* vertices;        /* device */
+    uint32_t* indices;           /* device */
+    mtx_meshlet* meshlets;       /* device */
+    uint64_t* vertex_edge_ids;   /* device, or NULL without the edge_ids option */
+    uint32_t* active_block_ids;  /* device */
+    mtx_batch_range* ranges;     /* device, one per unit */
+    uint64_t vertex_capacity;
+    uint64_t index_capacity;
+    uint64_t meshlet_capacity;
+    uint32_t num_units;
+    uint32_t num_classes;        /* (dtype, block edge) classes the batch runs */
+} mtx_batch_output;
+
+/* What the planner chose for one dtype class. */
+typedef struct mtx_batch_class_info {
+    uint32_t dtype;
+    uint32_t block_dim;
+    uint32_t minmax_rows;
+    uint32_t num_volumes;
+    uint32_t num_units;
+    uint32_t active_blocks;      /* planning run */
+    uint64_t blocks;             /* sum of blocks over the class's units */
+    uint64_t voxels;             /* sum of voxels over the class's volumes */
+    float tuned_ms[3];           /* autotuner: class pipeline at B = 4, 8, 16 (0 = not timed) */
+    float tuned_rows_ms[3];      /* autotuner: min/max at 1, 2, 4 y-blocks per wave (0 = not timed) */
+} mtx_batch_class_info;
+
+void mtx_batch_options_default(mtx_batch_options* o);
+mtx_status mtx_batch_create(mtx_context* ctx, const mtx_batch_volume* volumes, uint32_t num_volumes,
+                            const mtx_batch_options* opt, mtx_batch** out);
+/* Enqueue one run on `stream` (NULL: the context's stream, and wait for it). */
+mtx_status mtx_batch_run(mtx_batch* b, void* stream);
+mtx_status mtx_batch_output_get(const mtx_batch* b, mtx_batch_output* out);
+/* Blocking copy of the first n unit ranges of the last run to host memory;
+ * returns MTX_ECAPACITY if any unit's flags has bit 0 set, MTX_ESTATE for bit 1. */
+mtx_status mtx_batch_read_ranges(mtx_batch* b, mtx_batch_range* host, uint32_t n);
+mtx_status mtx_batch_class_info_get(const mtx_batch* b, uint32_t index, mtx_batch_class_info* out);
+void mtx_batch_destroy(mtx_batch* b);
 
 /* ---- profiling (the reference's GPUProfiler regions, src/gpuProfiler.h:44-107) ---- */
 

@@ -34,7 +34,7 @@ class CVolume(ctypes.Structure):
 class Options(ctypes.Structure):
     _fields_ = [("overlap", u32), ("rule", u32), ("build_pyramid", u32), ("fuse_filter", u32),
                 ("normals", u32), ("edge_ids", u32), ("case_indices", u32),
-                ("voxel_positions", u32), ("reuse", u32), ("_reserved", u32 * 7)]
+                ("voxel_positions", u32), ("reuse", u32), ("minmax_rows", u32), ("_reserved", u32 * 6)]
 
 
 class CMinMaxOutput(ctypes.Structure):
@@ -60,6 +60,36 @@ class CExtractionOutput(ctypes.Structure):
                 ("meshletCount", u32),
                 ("vertexCount", u32), ("indexCount", u32), ("block_dim", u32),
                 ("max_blocks", u32), ("flags", u32)]
+
+
+class CBatchVolume(ctypes.Structure):
+    _fields_ = [("volume", CVolume), ("isovalues", ctypes.POINTER(f32)), ("num_isovalues", u32),
+                ("_pad", u32)]
+
+
+class BatchOptions(ctypes.Structure):
+    _fields_ = [("block_dim", u32), ("minmax_rows", u32), ("rule", u32), ("normals", u32),
+                ("edge_ids", u32), ("graph", u32), ("tune_iters", u32), ("headroom_pct", u32),
+                ("_reserved", u32 * 8)]
+
+
+class BatchRange(ctypes.Structure):
+    _fields_ = [("vertex_offset", u64), ("vertex_count", u64), ("index_offset", u64),
+                ("index_count", u64), ("meshlet_offset", u64), ("meshlet_count", u64),
+                ("active_offset", u32), ("active_count", u32), ("block_dim", u32), ("flags", u32)]
+
+
+class CBatchOutput(ctypes.Structure):
+    _fields_ = [("vertices", vp), ("indices", vp), ("meshlets", vp), ("vertex_edge_ids", vp),
+                ("active_block_ids", vp), ("ranges", vp), ("vertex_capacity", u64),
+                ("index_capacity", u64), ("meshlet_capacity", u64), ("num_units", u32),
+                ("num_classes", u32)]
+
+
+class BatchClassInfo(ctypes.Structure):
+    _fields_ = [("dtype", u32), ("block_dim", u32), ("minmax_rows", u32), ("num_volumes", u32),
+                ("num_units", u32), ("active_blocks", u32), ("blocks", u64), ("voxels", u64),
+                ("tuned_ms", f32 * 3), ("tuned_rows_ms", f32 * 3)]
 
 
 # name -> (restype, argtypes)
@@ -106,6 +136,14 @@ _SIGS = {
     "mtx_adjoint": (ctypes.c_int, [vp, ctypes.POINTER(CMinMaxOutput), ctypes.POINTER(CFilteringOutput),
                                    ctypes.POINTER(CExtractionOutput), ctypes.POINTER(PushConstants),
                                    vp, vp, vp, vp]),
+    "mtx_batch_options_default": (None, [ctypes.POINTER(BatchOptions)]),
+    "mtx_batch_create": (ctypes.c_int, [vp, ctypes.POINTER(CBatchVolume), u32, ctypes.POINTER(BatchOptions),
+                                        ctypes.POINTER(vp)]),
+    "mtx_batch_run": (ctypes.c_int, [vp, vp]),
+    "mtx_batch_output_get": (ctypes.c_int, [vp, ctypes.POINTER(CBatchOutput)]),
+    "mtx_batch_read_ranges": (ctypes.c_int, [vp, ctypes.POINTER(BatchRange), u32]),
+    "mtx_batch_class_info_get": (ctypes.c_int, [vp, u32, ctypes.POINTER(BatchClassInfo)]),
+    "mtx_batch_destroy": (None, [vp]),
 }
 
 SYNTH_SIGS = {

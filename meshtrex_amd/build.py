@@ -25,7 +25,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=" + ARCH, "-Wall",
           "-Wno-unused-function", "-I" + os.path.join(ROOT, "include")] + DEVICE_FLAGS
 
 LIBS = {
-    "libmeshtrex_amd.so": ["mtx_kernels.hip", "mtx_api.hip"],
+    "libmeshtrex_amd.so": ["mtx_kernels.hip", "mtx_batch.hip", "mtx_api.hip", "mtx_batch_api.hip"],
     "libmtx_synth.so": ["synth.hip"],
 }
 

@@ -1,6 +1,5 @@
 // C-ABI host layer: argument checking, device memory, launch sequencing.
-#include "../../include/meshtrex_amd/mtx.h"
-#include "mtx_internal.h"
+#include "mtx_host.h"
 
 #include <cmath>
 #include <cstdarg>
@@ -10,37 +9,9 @@
 #include <string>
 #include <vector>
 
-struct mtx_context {
-    int device = 0;
-    hipStream_t own = nullptr;
-    std::string last_error;
-    std::mutex mu;
-};
-
 namespace {
 
 using namespace mtx;
-
-mtx_status fail(mtx_context* ctx, mtx_status st, const char* fmt, ...)
-{
-    if (ctx) {
-        char buf[512];
-        va_list ap;
-        va_start(ap, fmt);
-        vsnprintf(buf, sizeof buf, fmt, ap);
-        va_end(ap);
-        ctx->last_error = buf;
-    }
-    return st;
-}
-
-#define MTX_HIP(ctx, call)                                                                 \
-    do {                                                                                   \
-        hipError_t e_ = (call);                                                            \
-        if (e_ != hipSuccess)                                                              \
-            return fail((ctx), e_ == hipErrorOutOfMemory ? MTX_ENOMEM : MTX_EDEVICE,       \
-                        "%s: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__); \
-    } while (0)
 
 mtx_options defaults()
 {
@@ -48,41 +19,6 @@ mtx_options defaults()
     mtx_options_default(&o);
     return o;
 }
-
-struct Bound {
-    mtx_context* ctx;
-    hipStream_t s;
-    bool sync;
-};
-
-mtx_status bind(mtx_context* ctx, void* stream, Bound& b)
-{
-    b.ctx = ctx;
-    b.s = stream ? static_cast<hipStream_t>(stream) : ctx->own;
-    b.sync = stream == nullptr;
-    if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, MTX_EDEVICE, "hipSetDevice failed");
-    return MTX_OK;
-}
-
-mtx_status finish(const Bound& b)
-{
-    if (b.sync) MTX_HIP(b.ctx, hipStreamSynchronize(b.s));
-    return MTX_OK;
-}
-
-ActParams make_act(int dtype, float iso, int rule)
-{
-    ActParams a{};
-    a.iso = iso;
-    a.dtype = dtype;
-    a.rule = rule;
-    float r = std::roundf(iso);
-    a.iso_r_valid = (r >= 0.0f && r < 4294967296.0f) ? 1 : 0;
-    a.iso_r = a.iso_r_valid ? (uint32_t)r : 0u;
-    return a;
-}
-
-uint64_t align256(uint64_t x) { return (x + 255) & ~255ull; }
 
 // Pyramid layout, src/minMaxManager.cpp:85-92,147-151.
 uint32_t pyramid_layout(const uint32_t G[3], uint32_t (*ext)[3], uint64_t* off)
@@ -105,31 +41,6 @@ uint32_t pyramid_layout(const uint32_t G[3], uint32_t (*ext)[3], uint64_t* off)
     return L;
 }
 
-// Extraction workspace layout (offsets in bytes).
-struct WsLayout {
-    uint64_t counts, vbase, tbase, mbase, table, cases, adj, totals, scan, bytes;
-    uint32_t scan_tiles;
-};
-
-WsLayout ws_layout(uint64_t max_blocks, uint32_t B)
-{
-    WsLayout w{};
-    uint64_t o = 0;
-    w.counts = o; o = align256(o + 8 * max_blocks);
-    w.vbase = o;  o = align256(o + 8 * max_blocks);
-    w.tbase = o;  o = align256(o + 8 * max_blocks);
-    w.mbase = o;  o = align256(o + 8 * max_blocks);
-    w.table = o;  o = align256(o + corner_entry_bytes(B) * (uint64_t)B * B * B * max_blocks);
-    w.cases = o;  o = align256(o + (uint64_t)B * B * B * max_blocks);
-    w.adj = o;    o = align256(o + 8ull * 4 * adjoint_workgroups(max_blocks));
-    w.totals = o; o = align256(o + 8 * 4);
-    uint64_t tiles = (max_blocks + scan_tile_items() - 1) / scan_tile_items();
-    if (tiles == 0) tiles = 1;
-    w.scan_tiles = (uint32_t)tiles;
-    w.scan = o;   o = align256(o + 8 * (3 * tiles + 1));
-    w.bytes = o;
-    return w;
-}
 
 }  // namespace
 
@@ -253,6 +164,8 @@ mtx_status mtx_compute_min_max(mtx_context* ctx, const mtx_volume* volume,
     if (pc->volumeDim[0] != X || pc->volumeDim[1] != Y || pc->volumeDim[2] != Z)
         return fail(ctx, MTX_EINVAL, "push constants volumeDim does not match the volume");
     if (o.overlap > 1) return fail(ctx, MTX_EINVAL, "overlap must be 0 or 1");
+    if (o.minmax_rows != 0 && o.minmax_rows != 1 && o.minmax_rows != 2 && o.minmax_rows != 4)
+        return fail(ctx, MTX_EINVAL, "minmax_rows must be 0, 1, 2 or 4");
     Grid g{ B, (X + B - 1) / B, (Y + B - 1) / B, (Z + B - 1) / B };
     if (pc->blockGridDim[0] != g.gx || pc->blockGridDim[1] != g.gy || pc->blockGridDim[2] != g.gz)
         return fail(ctx, MTX_EINVAL, "blockGridDim must be ceil(volumeDim / blockDim)");
@@ -304,7 +217,7 @@ mtx_status mtx_compute_min_max(mtx_context* ctx, const mtx_volume* volume,
     ActParams act = make_act((int)volume->dtype, pc->isovalue, (int)o.rule);
     hipError_t e;
     if (fast)
-        e = launch_minmax_b8(v, g, out->minmax, fuse ? out->active_mask : nullptr, &act, bd.s);
+        e = launch_minmax_b8(v, g, out->minmax, fuse ? out->active_mask : nullptr, &act, o.minmax_rows, bd.s);
     else
         e = launch_minmax_generic(v, g, (int)o.overlap, out->minmax, bd.s);
     if (e == hipSuccess && o.build_pyramid && out->num_levels > 1) {
@@ -797,6 +710,11 @@ uint64_t mtx_struct_size(const char* name)
     MTX_SZ(mtx_vertex)
     MTX_SZ(mtx_meshlet)
     MTX_SZ(mtx_extraction_output)
+    MTX_SZ(mtx_batch_volume)
+    MTX_SZ(mtx_batch_options)
+    MTX_SZ(mtx_batch_range)
+    MTX_SZ(mtx_batch_output)
+    MTX_SZ(mtx_batch_class_info)
 #undef MTX_SZ
     return 0;
 }

@@ -1,0 +1,106 @@
+// Host-side internals shared by the C-ABI translation units: the context,
+// error reporting, stream binding.  Not installed; not part of the ABI.
+#pragma once
+
+#include "../../include/meshtrex_amd/mtx.h"
+#include "mtx_internal.h"
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <mutex>
+#include <string>
+
+struct mtx_context {
+    int device = 0;
+    hipStream_t own = nullptr;
+    std::string last_error;
+    std::mutex mu;
+};
+
+namespace mtx {
+
+inline mtx_status fail(mtx_context* ctx, mtx_status st, const char* fmt, ...)
+{
+    if (ctx) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        ctx->last_error = buf;
+    }
+    return st;
+}
+
+#define MTX_HIP(ctx, call)                                                                 \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return ::mtx::fail((ctx), e_ == hipErrorOutOfMemory ? MTX_ENOMEM : MTX_EDEVICE, \
+                               "%s: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__); \
+    } while (0)
+
+struct Bound {
+    mtx_context* ctx;
+    hipStream_t s;
+    bool sync;
+};
+
+// A null stream means the context's own stream, and a synchronous call.
+inline mtx_status bind(mtx_context* ctx, void* stream, Bound& b)
+{
+    b.ctx = ctx;
+    b.s = stream ? static_cast<hipStream_t>(stream) : ctx->own;
+    b.sync = stream == nullptr;
+    if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, MTX_EDEVICE, "hipSetDevice failed");
+    return MTX_OK;
+}
+
+inline mtx_status finish(const Bound& b)
+{
+    if (b.sync) MTX_HIP(b.ctx, hipStreamSynchronize(b.s));
+    return MTX_OK;
+}
+
+inline ActParams make_act(int dtype, float iso, int rule)
+{
+    ActParams a{};
+    a.iso = iso;
+    a.dtype = dtype;
+    a.rule = rule;
+    float r = std::roundf(iso);
+    a.iso_r_valid = (r >= 0.0f && r < 4294967296.0f) ? 1 : 0;
+    a.iso_r = a.iso_r_valid ? (uint32_t)r : 0u;
+    return a;
+}
+
+inline uint64_t align256(uint64_t x) { return (x + 255) & ~255ull; }
+
+// Extraction workspace layout (offsets in bytes).
+struct WsLayout {
+    uint64_t counts, vbase, tbase, mbase, table, cases, adj, totals, scan, bytes;
+    uint32_t scan_tiles;
+};
+
+inline WsLayout ws_layout(uint64_t max_blocks, uint32_t B)
+{
+    WsLayout w{};
+    uint64_t o = 0;
+    w.counts = o; o = align256(o + 8 * max_blocks);
+    w.vbase = o;  o = align256(o + 8 * max_blocks);
+    w.tbase = o;  o = align256(o + 8 * max_blocks);
+    w.mbase = o;  o = align256(o + 8 * max_blocks);
+    w.table = o;  o = align256(o + corner_entry_bytes(B) * (uint64_t)B * B * B * max_blocks);
+    w.cases = o;  o = align256(o + (uint64_t)B * B * B * max_blocks);
+    w.adj = o;    o = align256(o + 8ull * 4 * adjoint_workgroups(max_blocks));
+    w.totals = o; o = align256(o + 8 * 4);
+    uint64_t tiles = (max_blocks + scan_tile_items() - 1) / scan_tile_items();
+    if (tiles == 0) tiles = 1;
+    w.scan_tiles = (uint32_t)tiles;
+    w.scan = o;   o = align256(o + 8 * (3 * tiles + 1));
+    w.bytes = o;
+    return w;
+}
+
+}  // namespace mtx

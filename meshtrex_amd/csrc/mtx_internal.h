@@ -41,8 +41,9 @@ struct ScanState {
 
 // Block min/max, B=8, overlap 1, coalesced wave64 kernel.  Optionally writes
 // the activity bit mask (one 32-bit word per 32 x-blocks) for `act`.
+// rows: y-blocks per wavefront (1, 2, 4) or 0 to choose from the shape.
 hipError_t launch_minmax_b8(const VolDesc& v, const Grid& g, uint32_t* pairs, uint32_t* mask,
-                            const ActParams* act, hipStream_t s);
+                            const ActParams* act, uint32_t rows, hipStream_t s);
 
 // Block min/max for any B, overlap 0 or 1 (one wave per block).
 hipError_t launch_minmax_generic(const VolDesc& v, const Grid& g, int overlap, uint32_t* pairs,
@@ -68,6 +69,65 @@ hipError_t launch_filter_pairs(const uint32_t* pairs, const Grid& g, const ActPa
 
 // Clear n 64-bit words (a small fill kernel, safe under graph capture).
 hipError_t launch_zero64(uint64_t* p, uint64_t n, hipStream_t s);
+
+// ---- batches -----------------------------------------------------------------
+
+// One (volume, isovalue) unit of a batch, as the batched kernels read it.  The
+// active lists of all units of a batch class are concatenated in unit order.
+struct BatchUnit {
+    const void* data;           // the unit's volume (x fastest)
+    const uint32_t* pairs;      // the volume's leaf min/max key pairs, 2 words per block
+    uint32_t* map;              // this unit's dense block ID -> active index map
+    uint32_t X, Y, Z;
+    uint32_t gx, gy, gz;
+    uint32_t tile0;             // first compaction tile of this unit
+    uint32_t nb;                // gx * gy * gz
+    ActParams act;              // isovalue and activity rule
+    uint32_t _pad[3];
+};
+
+// One volume of a batch class for the batched min/max (B = 4, 8 or 16, overlap 1).
+struct BatchVolume {
+    const void* data;
+    uint32_t* pairs;            // 2 words per block
+    uint32_t X, Y, Z;
+    uint32_t gx, gy, gz;
+    uint32_t aligned;           // X % 4 == 0 and data 16-byte aligned
+    uint32_t ntx, wx, ntxg, nygrp, zc;   // launch plan of this volume
+    uint32_t wg0;               // first workgroup of this volume
+};
+
+// Fill the launch plan of every volume (block edge B, M y-blocks per wave)
+// and return the workgroup count (0: not supported).  wg0 of each volume must
+// then be copied to a separate device array for the workgroup -> volume search.
+uint64_t plan_minmax_batch(BatchVolume* vols, uint32_t nvol, int dtype, uint32_t B, uint32_t M);
+hipError_t launch_minmax_batch(const BatchVolume* vols, const uint32_t* wg0, uint32_t nvol, int dtype,
+                               uint32_t B, uint32_t M, uint64_t nwg, hipStream_t s);
+
+// Ordered compaction over every unit of a batch class: active block IDs (local
+// to their unit), each active block's unit, the first active index of every
+// unit (unit_first[nunits] = total) and the total count; writes each unit's map.
+uint32_t batch_filter_tiles(uint32_t nb);   // compaction tiles of a unit of nb blocks
+hipError_t launch_filter_batch(const BatchUnit* units, const uint32_t* tile0, uint32_t nunits,
+                               uint32_t ntiles, uint32_t* ids, uint32_t* item_unit,
+                               uint32_t* unit_first, uint32_t* count, ScanState st, hipStream_t s);
+
+// Per-unit output ranges after the extraction scan (see mtx_batch_range).
+struct BatchRangeArgs {
+    const uint32_t* unit_first;
+    const uint32_t* unit_global;   // class-local unit -> batch unit index
+    uint32_t nunits;
+    const uint32_t* count;
+    const uint64_t* vbase;
+    const uint64_t* tbase;
+    const uint64_t* mbase;
+    const uint64_t* totals;
+    uint64_t voff, toff, moff, aoff;   // the class's base in the batch buffers
+    uint64_t vcap, icap, mcap;         // the class's capacities (relative to its base)
+    uint32_t block_dim;
+    void* ranges;                      // mtx_batch_range[batch units]
+};
+hipError_t launch_batch_ranges(const BatchRangeArgs& a, hipStream_t s);
 
 // ---- extraction --------------------------------------------------------------
 
@@ -96,9 +156,16 @@ struct ExtractArgs {
     uint32_t* out_counts;       // optional device uint32[3]: vertexCount, indexCount, meshletCount
     int normals;
     uint32_t max_blocks;        // capacity of the per-block arrays
+    // batched form (units != nullptr): vol, iso and blk_map come per active
+    // block from units[item_unit[i]]; indices and meshlet offsets are relative
+    // to the unit's first vertex / index (unit_first: first active index per unit)
+    const BatchUnit* units;
+    const uint32_t* item_unit;
+    const uint32_t* unit_first;
 };
 
 bool extract_supported_block(uint32_t B);
+bool batch_supported_block(uint32_t B);  // batched form: B = 4, 8, 16
 size_t corner_entry_bytes(uint32_t B);
 uint32_t scan_tile_items();
 

@@ -2,129 +2,13 @@
 // compaction, and PMB marching cubes (count, scan, emit).  Written for CDNA4:
 // wave64, LDS tiles, XCD-aware work order; no matrix cores (nothing here is a
 // dense contraction).
-#include "mtx_internal.h"
+#include "mtx_device.h"
 #include "mc_cases.h"
 
 #include <cstdlib>
 #include <type_traits>
 
 namespace mtx {
-
-// ---------------------------------------------------------------------------
-// small helpers
-// ---------------------------------------------------------------------------
-
-__device__ __forceinline__ uint32_t vkey(float f)
-{
-    // order-preserving map of IEEE-754 binary32 onto uint32
-    uint32_t b = __float_as_uint(f);
-    uint32_t m = (uint32_t)((int32_t)b >> 31) | 0x80000000u;
-    return b ^ m;
-}
-__device__ __forceinline__ uint32_t vkey(uint8_t v) { return v; }
-
-__device__ __forceinline__ float unkey(uint32_t k)
-{
-    uint32_t b = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
-    return __uint_as_float(b);
-}
-
-__device__ __forceinline__ float vfloat(float f) { return f; }
-__device__ __forceinline__ float vfloat(uint8_t v) { return (float)v; }
-
-// Activity of a block from its key pair (CPU rule of src/blockFilteringTestUtils.h:456-468,
-// or the live shader's rule of src/shaders/occupiedBlockPrefixSum.comp.glsl:84).
-__device__ __forceinline__ bool block_active(uint32_t lo, uint32_t hi, const ActParams& a)
-{
-    float flo = a.dtype == kU8 ? (float)lo : unkey(lo);
-    float fhi = a.dtype == kU8 ? (float)hi : unkey(hi);
-    if (a.rule == kRuleGpu) return a.iso >= flo && a.iso <= fhi;
-    if (lo == hi) return false;
-    if (a.dtype == kU8) return a.iso_r_valid && a.iso_r >= lo && a.iso_r <= hi;
-    return a.iso >= flo && a.iso <= fhi;
-}
-
-__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
-
-template <typename V>
-__device__ __forceinline__ V wave_incl_scan(V v)
-{
-    const uint32_t lane = lane_id();
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        V t = __shfl_up(v, d, 64);
-        if (lane >= (uint32_t)d) v += t;
-    }
-    return v;
-}
-
-template <typename V>
-__device__ __forceinline__ V wave_sum(V v)
-{
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
-    return v;
-}
-
-// XCD-aware remap: the dispatcher deals consecutive workgroup IDs round-robin
-// over the 8 XCDs; give each XCD a contiguous range of logical work so that
-// neighbouring tiles share one L2.
-__device__ __forceinline__ uint32_t xcd_remap(uint32_t wg, uint32_t nwg)
-{
-    if (nwg % 8u) return wg;
-    return (wg % 8u) * (nwg / 8u) + wg / 8u;
-}
-
-// ---------------------------------------------------------------------------
-// decoupled look-back (single-pass scan across workgroups)
-// status word: bits 63:62 = 1 aggregate ready, 2 inclusive prefix ready
-// ---------------------------------------------------------------------------
-
-constexpr uint64_t kFlagAgg = 1ull << 62;
-constexpr uint64_t kFlagInc = 2ull << 62;
-constexpr uint64_t kValMask = (1ull << 62) - 1;
-
-__device__ __forceinline__ uint64_t ld_acq(const uint64_t* p)
-{
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_rel(uint64_t* p, uint64_t v)
-{
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Called by one whole wave; returns the exclusive prefix of `tile` (same in all lanes).
-__device__ uint64_t lookback(uint64_t* status, uint32_t tile, uint64_t aggregate)
-{
-    const uint32_t lane = lane_id();
-    if (tile == 0) {
-        if (lane == 0) st_rel(&status[0], kFlagInc | aggregate);
-        return 0;
-    }
-    if (lane == 0) st_rel(&status[tile], kFlagAgg | aggregate);
-    uint64_t excl = 0;
-    int64_t end = (int64_t)tile - 1;
-    for (;;) {
-        int64_t idx = end - (int64_t)lane;
-        uint64_t s = idx >= 0 ? ld_acq(&status[idx]) : kFlagInc;
-        uint64_t flag = s >> 62;
-        if (__any(flag == 0)) {
-            __builtin_amdgcn_s_sleep(1);
-            continue;
-        }
-        uint64_t inc = __ballot(flag == 2);
-        uint64_t v = s & kValMask;
-        if (inc) {
-            uint32_t first = __builtin_ctzll(inc);
-            excl += wave_sum<uint64_t>(lane <= first ? v : 0ull);
-            break;
-        }
-        excl += wave_sum<uint64_t>(v);
-        end -= 64;
-    }
-    if (lane == 0) st_rel(&status[tile], kFlagInc | (excl + aggregate));
-    return excl;
-}
 
 __global__ void k_zero64(uint64_t* __restrict__ p, uint64_t n)
 {
@@ -218,8 +102,9 @@ __device__ __forceinline__ MMAcc mm_row(const uint32_t (&k)[4], uint32_t kx, uin
     return a;
 }
 
-// One plane for M y-blocks starting at row y0: out[k] = (merge ? out[k] (+) : ) rows of block k.
-template <typename T, bool ALIGNED, int M>
+// One plane for M y-blocks of NB rows starting at row y0:
+// out[k] = (merge ? out[k] (+) : ) rows of block k.
+template <typename T, bool ALIGNED, int M, int NB = 8>
 __device__ __forceinline__ void mm_plane(MMAcc (&out)[M], bool merge, const T* __restrict__ vol,
                                          uint64_t zoff, uint32_t X, uint32_t Y, uint32_t y0,
                                          uint32_t x0, bool lane_ok, uint32_t xe, bool xe_ok)
@@ -229,24 +114,24 @@ __device__ __forceinline__ void mm_plane(MMAcc (&out)[M], bool merge, const T* _
     MMAcc carry = mm_row<ALIGNED>(k0, kx0, X, x0, lane_ok, xe_ok);
 #pragma unroll
     for (int b = 0; b < M; ++b) {
-        const uint32_t yb = y0 + 8u * b;
-        const uint32_t ylast = min(yb + 8u, Y - 1u);
-        uint32_t k[8][4], kx[8];
+        const uint32_t yb = y0 + (uint32_t)NB * b;
+        const uint32_t ylast = min(yb + (uint32_t)NB, Y - 1u);
+        uint32_t k[NB][4], kx[NB];
 #pragma unroll
-        for (int r = 0; r < 8; ++r) {
+        for (int r = 0; r < NB; ++r) {
             const uint32_t y = min(yb + 1u + r, ylast);
             mm_load<T, ALIGNED>(vol + zoff + (uint64_t)y * X, X, x0, lane_ok, xe, xe_ok, k[r], kx[r]);
         }
         MMAcc a = carry;
 #pragma unroll
-        for (int r = 0; r < 8; ++r) {
+        for (int r = 0; r < NB; ++r) {
             const MMAcc row = mm_row<ALIGNED>(k[r], kx[r], X, x0, lane_ok, xe_ok);
             mm_merge(a, row);
-            if (r == 7) carry = row;
+            if (r == NB - 1) carry = row;
         }
         if (merge) mm_merge(out[b], a);
         else out[b] = a;
-        // one batch of 8 rows in flight at a time: keeps the register budget at
+        // one batch of NB rows in flight at a time: keeps the register budget at
         // 4+ waves/SIMD instead of hoisting every block's loads
         __builtin_amdgcn_sched_barrier(0);
     }
@@ -343,7 +228,7 @@ struct MMPlan {
 
 // Pick y-blocks per wave (M) and the z chunk: every workgroup resident in one
 // round (no tail), the fewest voxels read twice (1/(8M) + 1/(8zc)).
-static MMPlan plan_minmax(const Grid& g, uint32_t ntx, const int (&resident)[3])
+static MMPlan plan_minmax(const Grid& g, uint32_t ntx, const int (&resident)[3], uint32_t rows)
 {
     const uint32_t wx = ntx >= 4 ? 4u : (ntx >= 2 ? 2u : 1u);
     MMPlan best{};
@@ -351,8 +236,7 @@ static MMPlan plan_minmax(const Grid& g, uint32_t ntx, const int (&resident)[3])
     const uint32_t ms[3] = { 1, 2, 4 };
     for (int mi = 0; mi < 3; ++mi) {
         const uint32_t M = ms[mi];
-        if (const char* env = getenv("MTX_MINMAX_M"))
-            if ((uint32_t)atoi(env) != M) continue;
+        if (rows && rows != M) continue;
         const uint64_t target = (uint64_t)device_cus() * (uint64_t)(resident[mi] > 0 ? resident[mi] : 1);
         MMPlan p{};
         p.M = M;
@@ -362,7 +246,6 @@ static MMPlan plan_minmax(const Grid& g, uint32_t ntx, const int (&resident)[3])
         p.nygrp = (ygroups + (4 / wx) - 1) / (4 / wx);
         const uint64_t cols = (uint64_t)p.ntxg * p.nygrp;
         uint64_t zc = (cols * g.gz + target - 1) / target;
-        if (const char* env = getenv("MTX_MINMAX_ZCHUNK")) zc = strtoull(env, nullptr, 10);
         if (zc < 1) zc = 1;
         if (zc > g.gz) zc = g.gz;
         p.zc = (uint32_t)zc;
@@ -389,7 +272,7 @@ static hipError_t launch_mm_m(const VolDesc& v, const Grid& g, const MMPlan& p, 
 
 template <typename T, bool MASK>
 static hipError_t launch_mm_b8_t(const VolDesc& v, const Grid& g, uint32_t* pairs, uint32_t* mask,
-                                 const ActParams* act, hipStream_t s)
+                                 const ActParams* act, uint32_t rows, hipStream_t s)
 {
     const uint32_t ntx = (g.gx + 31u) / 32u;
     static int resident[3] = { 0, 0, 0 };
@@ -400,7 +283,7 @@ static hipError_t launch_mm_b8_t(const VolDesc& v, const Grid& g, uint32_t* pair
         for (int& r : resident)
             if (r < 1) r = 1;
     }
-    const MMPlan p = plan_minmax(g, ntx, resident);
+    const MMPlan p = plan_minmax(g, ntx, resident, rows);
     if (p.nwg == 0 || p.nwg > 0x7FFFFFFFull) return hipErrorInvalidValue;
     const bool aligned = (v.X % 4u) == 0 && (((uintptr_t)v.data) % (4 * sizeof(T))) == 0;
     ActParams a{};
@@ -417,14 +300,187 @@ static hipError_t launch_mm_b8_t(const VolDesc& v, const Grid& g, uint32_t* pair
 }
 
 hipError_t launch_minmax_b8(const VolDesc& v, const Grid& g, uint32_t* pairs, uint32_t* mask,
-                            const ActParams* act, hipStream_t s)
+                            const ActParams* act, uint32_t rows, hipStream_t s)
 {
+    if (rows != 0 && rows != 1 && rows != 2 && rows != 4) return hipErrorInvalidValue;
     const bool m = mask != nullptr;
     if (v.dtype == kF32)
-        return m ? launch_mm_b8_t<float, true>(v, g, pairs, mask, act, s)
-                 : launch_mm_b8_t<float, false>(v, g, pairs, mask, act, s);
-    return m ? launch_mm_b8_t<uint8_t, true>(v, g, pairs, mask, act, s)
-             : launch_mm_b8_t<uint8_t, false>(v, g, pairs, mask, act, s);
+        return m ? launch_mm_b8_t<float, true>(v, g, pairs, mask, act, rows, s)
+                 : launch_mm_b8_t<float, false>(v, g, pairs, mask, act, rows, s);
+    return m ? launch_mm_b8_t<uint8_t, true>(v, g, pairs, mask, act, rows, s)
+             : launch_mm_b8_t<uint8_t, false>(v, g, pairs, mask, act, rows, s);
+}
+
+// ---------------------------------------------------------------------------
+// batched block min/max, B = 4, 8 or 16, overlap 1
+//
+// The row walk of k_minmax_b8 generalised to B/4 lanes per x-block (a wave
+// segment of 256 voxels holds 64, 32 or 16 x-blocks) and to many volumes per
+// launch: each volume has its own launch plan and a contiguous range of
+// workgroups; a workgroup finds its volume with one wave ballot per 64
+// volumes.  Volumes whose rows are not 16-byte aligned take scalar loads.
+// ---------------------------------------------------------------------------
+
+template <typename T, bool ALIGNED, int B, int M>
+__device__ __forceinline__ void mm_batch_body(const BatchVolume& V, uint32_t tile)
+{
+    constexpr uint32_t L = B / 4;       // lanes per x-block
+    constexpr uint32_t BPW = 64 / L;    // x-blocks per wave segment
+    const T* __restrict__ vol = static_cast<const T*>(V.data);
+    const uint32_t X = V.X, Y = V.Y, Z = V.Z;
+    const uint32_t txg = tile % V.ntxg;
+    const uint32_t rest = tile / V.ntxg;
+    const uint32_t ygrp = rest % V.nygrp;
+    const uint32_t cz = rest / V.nygrp;
+    const uint32_t lane = lane_id();
+    const uint32_t w = threadIdx.x >> 6;
+    const uint32_t wy = 4u / V.wx;
+    const uint32_t tx = txg * V.wx + w % V.wx;
+    const uint32_t by0 = (ygrp * wy + w / V.wx) * (uint32_t)M;
+    if (tx >= V.ntx || by0 >= V.gy) return;
+    const uint32_t bz0 = cz * V.zc;
+    if (bz0 >= V.gz) return;
+    const uint32_t bz1 = min(bz0 + V.zc, V.gz);
+
+    const uint32_t x0 = tx * 256u + lane * 4u;
+    const bool lane_ok = x0 < X;
+    const uint32_t xe = tx * 256u + 256u;
+    const bool xe_ok = xe < X;
+    const uint32_t y0 = by0 * (uint32_t)B;
+    const uint64_t XY = (uint64_t)X * Y;
+    uint2* __restrict__ pairs = reinterpret_cast<uint2*>(V.pairs);
+
+    MMAcc first[M];
+    mm_plane<T, ALIGNED, M, B>(first, false, vol, (uint64_t)(bz0 * B) * XY, X, Y, y0, x0, lane_ok, xe, xe_ok);
+    const uint32_t bx = tx * BPW + lane / L;
+    for (uint32_t bz = bz0; bz < bz1; ++bz) {
+        MMAcc acc[M];
+#pragma unroll
+        for (int b = 0; b < M; ++b) acc[b] = first[b];
+        const uint32_t zl = min(bz * B + B, Z - 1u);
+#pragma unroll 1
+        for (uint32_t dz = 1; dz < (uint32_t)B; ++dz)
+            mm_plane<T, ALIGNED, M, B>(acc, true, vol, (uint64_t)min(bz * B + dz, zl) * XY, X, Y, y0, x0,
+                                       lane_ok, xe, xe_ok);
+        mm_plane<T, ALIGNED, M, B>(first, false, vol, (uint64_t)zl * XY, X, Y, y0, x0, lane_ok, xe, xe_ok);
+#pragma unroll
+        for (int b = 0; b < M; ++b) {
+            mm_merge(acc[b], first[b]);
+            const uint32_t by = by0 + b;
+            // x combine: the L lanes of the block plus the first voxel of lane + L
+            uint32_t bmin = acc[b].amin, bmax = acc[b].amax;
+#pragma unroll
+            for (uint32_t d = 1; d < L; d <<= 1) {
+                bmin = min(bmin, (uint32_t)__shfl_xor(bmin, (int)d, 64));
+                bmax = max(bmax, (uint32_t)__shfl_xor(bmax, (int)d, 64));
+            }
+            const uint32_t src = min(lane + L, 63u);
+            uint32_t nmin = __shfl(acc[b].emin, (int)src, 64);
+            uint32_t nmax = __shfl(acc[b].emax, (int)src, 64);
+            if (lane >= 64u - L) { nmin = acc[b].xmin; nmax = acc[b].xmax; }
+            bmin = min(bmin, nmin);
+            bmax = max(bmax, nmax);
+            if (lane % L == 0u && bx < V.gx && by < V.gy)
+                pairs[bx + (uint64_t)V.gx * (by + (uint64_t)V.gy * bz)] = make_uint2(bmin, bmax);
+        }
+    }
+}
+
+template <typename T, int B, int M>
+__global__ __launch_bounds__(256) void k_minmax_batch(const BatchVolume* __restrict__ vols,
+                                                      const uint32_t* __restrict__ wg0, uint32_t nvol)
+{
+    __shared__ uint32_t s_vol;
+    const uint32_t wg = xcd_remap(blockIdx.x, gridDim.x);
+    if (threadIdx.x < 64) {
+        // the last volume whose first workgroup is <= wg (wg0 ascending, wg0[0] = 0)
+        uint32_t n = 0;
+        for (uint32_t c = 0; c < nvol; c += 64) {
+            const uint32_t k = c + threadIdx.x;
+            n += (uint32_t)__popcll(__ballot(k < nvol && wg0[k] <= wg));
+        }
+        if (threadIdx.x == 0) s_vol = n - 1;
+    }
+    __syncthreads();
+    const uint32_t v = __builtin_amdgcn_readfirstlane(s_vol);
+    const BatchVolume& V = vols[v];
+    const uint32_t tile = wg - V.wg0;
+    if (V.aligned) mm_batch_body<T, true, B, M>(V, tile);
+    else mm_batch_body<T, false, B, M>(V, tile);
+}
+
+template <typename T, int B, int M>
+static int mm_batch_resident()
+{
+    static int r = 0;
+    if (!r) {
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&r, k_minmax_batch<T, B, M>, 256, 0);
+        if (r < 1) r = 1;
+    }
+    return r;
+}
+
+static int mm_batch_resident_for(int dtype, uint32_t B, uint32_t M)
+{
+#define MTX_RES(BB, MM)                                                                          \
+    if (B == BB && M == MM)                                                                      \
+        return dtype == kF32 ? mm_batch_resident<float, BB, MM>() : mm_batch_resident<uint8_t, BB, MM>();
+    MTX_RES(4, 1) MTX_RES(4, 2) MTX_RES(4, 4)
+    MTX_RES(8, 1) MTX_RES(8, 2) MTX_RES(8, 4)
+    MTX_RES(16, 1) MTX_RES(16, 2) MTX_RES(16, 4)
+#undef MTX_RES
+    return 1;
+}
+
+uint64_t plan_minmax_batch(BatchVolume* vols, uint32_t nvol, int dtype, uint32_t B, uint32_t M)
+{
+    if (!batch_supported_block(B) || (M != 1 && M != 2 && M != 4) || nvol == 0) return 0;
+    const uint64_t target = (uint64_t)device_cus() * (uint64_t)mm_batch_resident_for(dtype, B, M);
+    uint32_t maxgz = 1;
+    for (uint32_t i = 0; i < nvol; ++i) {
+        BatchVolume& v = vols[i];
+        v.ntx = (v.X + 255u) / 256u;
+        v.wx = v.ntx >= 4 ? 4u : (v.ntx >= 2 ? 2u : 1u);
+        v.ntxg = (v.ntx + v.wx - 1) / v.wx;
+        const uint32_t ygroups = (v.gy + M - 1) / M;
+        v.nygrp = (ygroups + (4 / v.wx) - 1) / (4 / v.wx);
+        maxgz = v.gz > maxgz ? v.gz : maxgz;
+    }
+    // the smallest z chunk that keeps every workgroup resident in one round
+    uint32_t zc = 1;
+    for (; zc < maxgz; ++zc) {
+        uint64_t n = 0;
+        for (uint32_t i = 0; i < nvol; ++i)
+            n += (uint64_t)vols[i].ntxg * vols[i].nygrp * ((vols[i].gz + zc - 1) / zc);
+        if (n <= target) break;
+    }
+    uint64_t wg = 0;
+    for (uint32_t i = 0; i < nvol; ++i) {
+        BatchVolume& v = vols[i];
+        v.zc = zc < v.gz ? zc : v.gz;
+        v.wg0 = (uint32_t)wg;
+        wg += (uint64_t)v.ntxg * v.nygrp * ((v.gz + v.zc - 1) / v.zc);
+    }
+    return wg > 0x7FFFFFFFull ? 0 : wg;
+}
+
+hipError_t launch_minmax_batch(const BatchVolume* vols, const uint32_t* wg0, uint32_t nvol, int dtype,
+                               uint32_t B, uint32_t M, uint64_t nwg, hipStream_t s)
+{
+    if (nwg == 0 || nwg > 0x7FFFFFFFull) return hipErrorInvalidValue;
+#define MTX_MMB(BB, MM)                                                                          \
+    if (B == BB && M == MM) {                                                                    \
+        if (dtype == kF32)                                                                       \
+            hipLaunchKernelGGL((k_minmax_batch<float, BB, MM>), dim3((uint32_t)nwg), dim3(256), 0, s, vols, wg0, nvol); \
+        else                                                                                     \
+            hipLaunchKernelGGL((k_minmax_batch<uint8_t, BB, MM>), dim3((uint32_t)nwg), dim3(256), 0, s, vols, wg0, nvol); \
+        return hipGetLastError();                                                                \
+    }
+    MTX_MMB(4, 1) MTX_MMB(4, 2) MTX_MMB(4, 4)
+    MTX_MMB(8, 1) MTX_MMB(8, 2) MTX_MMB(8, 4)
+    MTX_MMB(16, 1) MTX_MMB(16, 2) MTX_MMB(16, 4)
+#undef MTX_MMB
+    return hipErrorInvalidValue;
 }
 
 // ---------------------------------------------------------------------------
@@ -780,7 +836,39 @@ __device__ __forceinline__ V wave_excl_scan(V v, V& total)
     return inc - v;
 }
 
-template <int B, typename T>
+// The volume an active block belongs to: the launch's own, or in the batched
+// form the block's unit (wave-uniform: one unit per active block).
+struct ItemVol {
+    const void* data;
+    uint32_t X, Y, Z, gx, gy, gz;
+    float iso;
+    const uint32_t* map;
+};
+
+template <bool BATCH>
+__device__ __forceinline__ ItemVol item_vol(const ExtractArgs& a, uint32_t i, uint32_t& unit)
+{
+    ItemVol r;
+    if constexpr (BATCH) {
+        unit = __builtin_amdgcn_readfirstlane(a.item_unit[i]);
+        const BatchUnit& U = a.units[unit];
+        r.data = U.data;
+        r.X = U.X; r.Y = U.Y; r.Z = U.Z;
+        r.gx = U.gx; r.gy = U.gy; r.gz = U.gz;
+        r.iso = U.act.iso;
+        r.map = U.map;
+    } else {
+        unit = 0;
+        r.data = a.vol.data;
+        r.X = a.vol.X; r.Y = a.vol.Y; r.Z = a.vol.Z;
+        r.gx = a.grid.gx; r.gy = a.grid.gy; r.gz = a.grid.gz;
+        r.iso = a.iso;
+        r.map = a.blk_map;
+    }
+    return r;
+}
+
+template <int B, typename T, bool BATCH>
 __global__ __launch_bounds__(256) void k_extract_count(ExtractArgs a)
 {
     constexpr int E = B + 1;
@@ -794,36 +882,38 @@ __global__ __launch_bounds__(256) void k_extract_count(ExtractArgs a)
     __syncthreads();
     const uint32_t lane = lane_id();
     uint8_t* s_below = s_all[threadIdx.x >> 6];
-    const T* __restrict__ vol = static_cast<const T*>(a.vol.data);
-    const uint32_t X = a.vol.X, Y = a.vol.Y, Z = a.vol.Z;
-    // a volume thinner than 2 voxels on any axis has no cells, hence no triangles
-    // and no vertices (every vertex is a corner of some triangle)
-    const bool degen = X < 2 || Y < 2 || Z < 2;
     const uint32_t na = *a.count;
     Entry* __restrict__ table = static_cast<Entry*>(a.corner_table);
     const WaveItems it = wave_items(na, 4);
     for (uint32_t i = it.first; i < it.end; i += it.stride) {
+        uint32_t unit;
+        const ItemVol V = item_vol<BATCH>(a, i, unit);
+        const T* __restrict__ vol = static_cast<const T*>(V.data);
+        const uint32_t X = V.X, Y = V.Y, Z = V.Z;
+        // a volume thinner than 2 voxels on any axis has no cells, hence no
+        // triangles and no vertices (every vertex is a corner of some triangle)
+        const bool degen = X < 2 || Y < 2 || Z < 2;
         const uint32_t id = a.ids[i];
-        const uint32_t bx = id % a.grid.gx, by = (id / a.grid.gx) % a.grid.gy;
-        const uint32_t bz = id / (a.grid.gx * a.grid.gy);
+        const uint32_t bx = id % V.gx, by = (id / V.gx) % V.gy;
+        const uint32_t bz = id / (V.gx * V.gy);
         const uint32_t ox = bx * B, oy = by * B, oz = bz * B;
         // all of a batch's loads are issued before the first is consumed
-        constexpr int LPL = (NTILE + 63) / 64, BATCH = LPL < 16 ? LPL : 16;
+        constexpr int LPL = (NTILE + 63) / 64, NLD = LPL < 16 ? LPL : 16;
 #pragma unroll
-        for (int k0 = 0; k0 < LPL; k0 += BATCH) {
-            T v[BATCH];
-            bool in[BATCH];
+        for (int k0 = 0; k0 < LPL; k0 += NLD) {
+            T v[NLD];
+            bool in[NLD];
 #pragma unroll
-            for (int k = 0; k < BATCH; ++k) {
+            for (int k = 0; k < NLD; ++k) {
                 const uint32_t j = lane + 64u * (k0 + k);
                 const uint32_t x = ox + j % E, y = oy + (j / E) % E, z = oz + j / (E * E);
                 in[k] = j < (uint32_t)NTILE && x < X && y < Y && z < Z;
                 v[k] = vol[in[k] ? x + (uint64_t)X * (y + (uint64_t)Y * z) : 0];
             }
 #pragma unroll
-            for (int k = 0; k < BATCH; ++k) {
+            for (int k = 0; k < NLD; ++k) {
                 const uint32_t j = lane + 64u * (k0 + k);
-                if (j < (uint32_t)NTILE) s_below[j] = (uint8_t)(in[k] && vfloat(v[k]) <= a.iso);
+                if (j < (uint32_t)NTILE) s_below[j] = (uint8_t)(in[k] && vfloat(v[k]) <= V.iso);
             }
         }
         wave_lds_sync();
@@ -887,7 +977,7 @@ __global__ __launch_bounds__(256) void k_extract_count(ExtractArgs a)
 // of 8 corners (ly = lane & 7, lz = lane >> 3) and classifies it with bit
 // operations on its four 9-bit rows (y, y+1) x (z, z+1).  Interior blocks load
 // their tile through per-lane offsets computed once.
-template <typename T>
+template <typename T, bool BATCH>
 __global__ __launch_bounds__(256) void k_extract_count_b8(ExtractArgs a)
 {
     constexpr int B = 8, E = 9, NTILE = 729, LPL = 12, NC = 512;
@@ -897,24 +987,30 @@ __global__ __launch_bounds__(256) void k_extract_count_b8(ExtractArgs a)
     __syncthreads();
     const uint32_t lane = lane_id();
     uint64_t* s_bits = s_bits_all[threadIdx.x >> 6];
-    const T* __restrict__ vol = static_cast<const T*>(a.vol.data);
-    const uint32_t X = a.vol.X, Y = a.vol.Y, Z = a.vol.Z;
-    const bool degen = X < 2 || Y < 2 || Z < 2;
-    const bool off32 = (uint64_t)X * Y * E < 0x80000000ull;
     uint32_t toff[LPL];
+    auto tile_offsets = [&](uint32_t X, uint32_t Y, bool off32) {
 #pragma unroll
-    for (int k = 0; k < LPL; ++k) {
-        const uint32_t j = min(lane + 64u * k, (uint32_t)NTILE - 1u);
-        toff[k] = off32 ? j % E + X * ((j / E) % E + Y * (j / (E * E))) : 0u;
-    }
+        for (int k = 0; k < LPL; ++k) {
+            const uint32_t j = min(lane + 64u * k, (uint32_t)NTILE - 1u);
+            toff[k] = off32 ? j % E + X * ((j / E) % E + Y * (j / (E * E))) : 0u;
+        }
+    };
+    if constexpr (!BATCH) tile_offsets(a.vol.X, a.vol.Y, (uint64_t)a.vol.X * a.vol.Y * E < 0x80000000ull);
     const uint32_t na = *a.count;
     uint16_t* __restrict__ table = static_cast<uint16_t*>(a.corner_table);
     const uint32_t ly = lane & 7u, lz = lane >> 3;
     const WaveItems it = wave_items(na, 4);
     for (uint32_t i = it.first; i < it.end; i += it.stride) {
+        uint32_t unit;
+        const ItemVol V = item_vol<BATCH>(a, i, unit);
+        const T* __restrict__ vol = static_cast<const T*>(V.data);
+        const uint32_t X = V.X, Y = V.Y, Z = V.Z;
+        const bool degen = X < 2 || Y < 2 || Z < 2;
+        const bool off32 = (uint64_t)X * Y * E < 0x80000000ull;
+        if constexpr (BATCH) tile_offsets(X, Y, off32);
         const uint32_t id = a.ids[i];
-        const uint32_t bx = id % a.grid.gx, by = (id / a.grid.gx) % a.grid.gy;
-        const uint32_t bz = id / (a.grid.gx * a.grid.gy);
+        const uint32_t bx = id % V.gx, by = (id / V.gx) % V.gy;
+        const uint32_t bz = id / (V.gx * V.gy);
         const uint32_t ox = bx * B, oy = by * B, oz = bz * B;
         const uint64_t base = ox + (uint64_t)X * (oy + (uint64_t)Y * oz);
         const bool interior = off32 && ox + B < X && oy + B < Y && oz + B < Z;
@@ -937,14 +1033,14 @@ __global__ __launch_bounds__(256) void k_extract_count_b8(ExtractArgs a)
         }
 #pragma unroll
         for (int k = 0; k < LPL; ++k) {
-            const uint64_t bal = __ballot(in[k] && vfloat(v[k]) <= a.iso);
+            const uint64_t bal = __ballot(in[k] && vfloat(v[k]) <= V.iso);
             if (lane == 0) s_bits[k] = bal;
         }
         wave_lds_sync();
         // the four 9-bit rows of this lane: (y, z), (y+1, z), (y, z+1), (y+1, z+1)
         auto row = [&](uint32_t r) -> uint32_t {
             const uint32_t o = 9u * r;
-            return (uint32_t)(s_bits[o >> 6] >> (o & 63u)) & 0x1FFu |
+            return ((uint32_t)(s_bits[o >> 6] >> (o & 63u)) & 0x1FFu) |
                    (((o & 63u) > 55u) ? (uint32_t)(s_bits[(o >> 6) + 1] << (64u - (o & 63u))) & 0x1FFu : 0u);
         };
         const uint32_t r0 = ly + 9u * lz;
@@ -1159,7 +1255,7 @@ struct LaneCorners {
 // (clamped to the volume: clamp-to-edge samples for the normals) in LDS, list
 // the block's vertices (corner, axis) in prefix order and let the 64 lanes
 // take consecutive entries: balanced work and coalesced writes.
-template <int B, typename T, bool NORMALS>
+template <int B, typename T, bool NORMALS, bool BATCH>
 __global__ __launch_bounds__((64 * VertCfg<B, NORMALS>::WPG)) void k_extract_vertices(ExtractArgs a)
 {
     using C = VertCfg<B, NORMALS>;
@@ -1172,14 +1268,9 @@ __global__ __launch_bounds__((64 * VertCfg<B, NORMALS>::WPG)) void k_extract_ver
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
     float* s_val = s_val_all[w];
     uint16_t* s_vl = s_vl_all[w];
-    const T* __restrict__ vol = static_cast<const T*>(a.vol.data);
-    const uint32_t X = a.vol.X, Y = a.vol.Y, Z = a.vol.Z;
-    const float iso = a.iso;
     const uint32_t na = *a.count;
     const Entry* __restrict__ table = static_cast<const Entry*>(a.corner_table);
-    const float inv2x = 2.0f / (float)X, inv2y = 2.0f / (float)Y, inv2z = 2.0f / (float)Z;
     constexpr int LPL = (NTILE + 63) / 64;
-    const bool off32 = (uint64_t)X * Y * E < 0x80000000ull;
     const WaveItems it = wave_items(na, WPG);
     for (uint32_t i = it.first; i < it.end; i += it.stride) {
         const uint32_t bnv = a.blk_counts[2 * (uint64_t)i];
@@ -1189,37 +1280,44 @@ __global__ __launch_bounds__((64 * VertCfg<B, NORMALS>::WPG)) void k_extract_ver
             if (lane == 0) atomicOr(reinterpret_cast<unsigned long long*>(&a.totals[2]), 1ull);
             continue;
         }
+        uint32_t unit;
+        const ItemVol V = item_vol<BATCH>(a, i, unit);
+        const T* __restrict__ vol = static_cast<const T*>(V.data);
+        const uint32_t X = V.X, Y = V.Y, Z = V.Z;
+        const float iso = V.iso;
+        const float inv2x = 2.0f / (float)X, inv2y = 2.0f / (float)Y, inv2z = 2.0f / (float)Z;
+        const bool off32 = (uint64_t)X * Y * E < 0x80000000ull;
         const uint32_t id = a.ids[i];
-        const uint32_t bx = id % a.grid.gx, by = (id / a.grid.gx) % a.grid.gy;
-        const uint32_t bz = id / (a.grid.gx * a.grid.gy);
+        const uint32_t bx = id % V.gx, by = (id / V.gx) % V.gy;
+        const uint32_t bz = id / (V.gx * V.gy);
         const uint32_t ox = bx * B, oy = by * B, oz = bz * B;
         LaneCorners<B> lc;
         lc.load(table, nullptr, i, lane, false);
         const bool interior = off32 && ox >= (uint32_t)A && oy >= (uint32_t)A && oz >= (uint32_t)A &&
                               ox + B + A < X && oy + B + A < Y && oz + B + A < Z;
-        constexpr int BATCH = LPL < 11 ? LPL : 11;
+        constexpr int NLD = LPL < 11 ? LPL : 11;
         if (interior) {
             const T* __restrict__ t0p = vol + ((ox - A) + (uint64_t)X * ((oy - A) + (uint64_t)Y * (oz - A)));
 #pragma unroll 1
-            for (int k0 = 0; k0 < LPL; k0 += BATCH) {
-                T v[BATCH];
+            for (int k0 = 0; k0 < LPL; k0 += NLD) {
+                T v[NLD];
 #pragma unroll
-                for (int k = 0; k < BATCH; ++k) {
+                for (int k = 0; k < NLD; ++k) {
                     const uint32_t j = min(lane + 64u * (k0 + k), (uint32_t)NTILE - 1u);
                     v[k] = t0p[j % E + X * ((j / E) % E + Y * (j / (E * E)))];
                 }
 #pragma unroll
-                for (int k = 0; k < BATCH; ++k) {
+                for (int k = 0; k < NLD; ++k) {
                     const uint32_t j = lane + 64u * (k0 + k);
                     if (j < (uint32_t)NTILE) s_val[j] = vfloat(v[k]);
                 }
             }
         } else {
 #pragma unroll 1
-            for (int k0 = 0; k0 < LPL; k0 += BATCH) {
-                T v[BATCH];
+            for (int k0 = 0; k0 < LPL; k0 += NLD) {
+                T v[NLD];
 #pragma unroll
-                for (int k = 0; k < BATCH; ++k) {
+                for (int k = 0; k < NLD; ++k) {
                     const uint32_t j = min(lane + 64u * (k0 + k), (uint32_t)NTILE - 1u);
                     int32_t x = (int32_t)(ox + j % E) - A, y = (int32_t)(oy + (j / E) % E) - A;
                     int32_t z = (int32_t)(oz + j / (E * E)) - A;
@@ -1229,7 +1327,7 @@ __global__ __launch_bounds__((64 * VertCfg<B, NORMALS>::WPG)) void k_extract_ver
                     v[k] = vol[(uint32_t)x + (uint64_t)X * ((uint32_t)y + (uint64_t)Y * (uint32_t)z)];
                 }
 #pragma unroll
-                for (int k = 0; k < BATCH; ++k) {
+                for (int k = 0; k < NLD; ++k) {
                     const uint32_t j = lane + 64u * (k0 + k);
                     if (j < (uint32_t)NTILE) s_val[j] = vfloat(v[k]);
                 }
@@ -1308,7 +1406,7 @@ template <int B> struct TriCfg {
 // +x/+y/+z faces from the neighbours' entries (one batched gather) -- then the
 // block's triangles are listed (cell, case, k) and resolved three LDS reads
 // each by consecutive lanes.  No voxel is read.
-template <int B>
+template <int B, bool BATCH>
 __global__ __launch_bounds__((64 * TriCfg<B>::WPG)) void k_extract_triangles(ExtractArgs a)
 {
     using C = TriCfg<B>;
@@ -1344,18 +1442,27 @@ __global__ __launch_bounds__((64 * TriCfg<B>::WPG)) void k_extract_triangles(Ext
             if (lane == 0) atomicOr(reinterpret_cast<unsigned long long*>(&a.totals[2]), 1ull);
             continue;
         }
+        uint32_t unit;
+        const ItemVol V = item_vol<BATCH>(a, i, unit);
+        // batched: indices and meshlet offsets relative to the unit's first vertex / triangle
+        uint64_t uv = 0, ut = 0;
+        if constexpr (BATCH) {
+            const uint32_t f = a.unit_first[unit];
+            uv = a.blk_vbase[f];
+            ut = a.blk_tbase[f];
+        }
         const uint32_t id = a.ids[i];
-        const uint32_t bx = id % a.grid.gx, by = (id / a.grid.gx) % a.grid.gy;
-        const uint32_t bz = id / (a.grid.gx * a.grid.gy);
+        const uint32_t bx = id % V.gx, by = (id / V.gx) % V.gy;
+        const uint32_t bz = id / (V.gx * V.gy);
         LaneCorners<B> lc;
         lc.load(table, a.cases, i, lane, true);
         if (lane < 8) {
             int32_t r = -1;
             uint32_t base = 0;
             const uint32_t nx = bx + (lane & 1u), ny = by + ((lane >> 1) & 1u), nz = bz + (lane >> 2);
-            if (lane != 0 && lane != 7 && nx < a.grid.gx && ny < a.grid.gy && nz < a.grid.gz) {
-                r = (int32_t)a.blk_map[nx + a.grid.gx * (ny + (uint64_t)a.grid.gy * nz)];
-                if (r >= 0) base = (uint32_t)a.blk_vbase[r];
+            if (lane != 0 && lane != 7 && nx < V.gx && ny < V.gy && nz < V.gz) {
+                r = (int32_t)V.map[nx + V.gx * (ny + (uint64_t)V.gy * nz)];
+                if (r >= 0) base = (uint32_t)(a.blk_vbase[r] - uv);
             }
             s_nbr[lane] = r;
             s_nbase[lane] = base;
@@ -1367,7 +1474,7 @@ __global__ __launch_bounds__((64 * TriCfg<B>::WPG)) void k_extract_triangles(Ext
             const uint32_t c = lane * CPL + j;
             if (c < (uint32_t)NC) {
                 const uint32_t q = c % B + F * ((c / B) % B + F * (c / (B * B)));
-                s_gidx[q] = (uint32_t)vb + (lc.ent(j) >> 3);
+                s_gidx[q] = (uint32_t)(vb - uv) + (lc.ent(j) >> 3);
                 s_gfl[q] = (uint8_t)(lc.ent(j) & 7u);
             }
             nt += mtx_word_ntri(s_words[lc.cs(j)]);
@@ -1460,47 +1567,47 @@ __global__ __launch_bounds__((64 * TriCfg<B>::WPG)) void k_extract_triangles(Ext
         if (a.meshlets) {
             for (uint32_t k = lane; k < bnm; k += 64u) {
                 const uint32_t prims = min(kMeshletPrims, bnt - k * kMeshletPrims);
-                a.meshlets[mb + k] = make_uint4((uint32_t)vb, (uint32_t)(3 * (tb + (uint64_t)k * kMeshletPrims)),
-                                                bnv, prims);
+                a.meshlets[mb + k] = make_uint4((uint32_t)(vb - uv),
+                                                (uint32_t)(3 * (tb - ut + (uint64_t)k * kMeshletPrims)), bnv, prims);
             }
         }
         wave_lds_sync();
     }
 }
 
-template <int B>
+template <int B, bool BATCH>
 static hipError_t launch_count_b(const ExtractArgs& a, hipStream_t s, uint32_t nwg)
 {
     if (B == 8) {
         if (a.vol.dtype == kF32)
-            hipLaunchKernelGGL(k_extract_count_b8<float>, dim3(nwg), dim3(256), 0, s, a);
+            hipLaunchKernelGGL((k_extract_count_b8<float, BATCH>), dim3(nwg), dim3(256), 0, s, a);
         else
-            hipLaunchKernelGGL(k_extract_count_b8<uint8_t>, dim3(nwg), dim3(256), 0, s, a);
+            hipLaunchKernelGGL((k_extract_count_b8<uint8_t, BATCH>), dim3(nwg), dim3(256), 0, s, a);
         return hipGetLastError();
     }
     if (a.vol.dtype == kF32)
-        hipLaunchKernelGGL((k_extract_count<B, float>), dim3(nwg), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_extract_count<B, float, BATCH>), dim3(nwg), dim3(256), 0, s, a);
     else
-        hipLaunchKernelGGL((k_extract_count<B, uint8_t>), dim3(nwg), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_extract_count<B, uint8_t, BATCH>), dim3(nwg), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
-template <int B>
+template <int B, bool BATCH>
 static hipError_t launch_emit_b(const ExtractArgs& a, hipStream_t s, uint32_t nwg)
 {
     const uint32_t tn = 64 * VertCfg<B, true>::WPG, tf = 64 * VertCfg<B, false>::WPG;
     const uint32_t gn = nwg * (4 / VertCfg<B, true>::WPG), gf = nwg * (4 / VertCfg<B, false>::WPG);
     if (a.vol.dtype == kF32) {
-        if (a.normals) hipLaunchKernelGGL((k_extract_vertices<B, float, true>), dim3(gn), dim3(tn), 0, s, a);
-        else hipLaunchKernelGGL((k_extract_vertices<B, float, false>), dim3(gf), dim3(tf), 0, s, a);
+        if (a.normals) hipLaunchKernelGGL((k_extract_vertices<B, float, true, BATCH>), dim3(gn), dim3(tn), 0, s, a);
+        else hipLaunchKernelGGL((k_extract_vertices<B, float, false, BATCH>), dim3(gf), dim3(tf), 0, s, a);
     } else {
-        if (a.normals) hipLaunchKernelGGL((k_extract_vertices<B, uint8_t, true>), dim3(gn), dim3(tn), 0, s, a);
-        else hipLaunchKernelGGL((k_extract_vertices<B, uint8_t, false>), dim3(gf), dim3(tf), 0, s, a);
+        if (a.normals) hipLaunchKernelGGL((k_extract_vertices<B, uint8_t, true, BATCH>), dim3(gn), dim3(tn), 0, s, a);
+        else hipLaunchKernelGGL((k_extract_vertices<B, uint8_t, false, BATCH>), dim3(gf), dim3(tf), 0, s, a);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const uint32_t tt = 64 * TriCfg<B>::WPG, gt = nwg * (4 / TriCfg<B>::WPG);
-    hipLaunchKernelGGL((k_extract_triangles<B>), dim3(gt), dim3(tt), 0, s, a);
+    hipLaunchKernelGGL((k_extract_triangles<B, BATCH>), dim3(gt), dim3(tt), 0, s, a);
     return hipGetLastError();
 }
 
@@ -1516,34 +1623,47 @@ static uint32_t persistent_wgs(uint32_t max_blocks)
 
 #define MTX_DISPATCH_B(FN, ...)                                   \
     switch (a.grid.B) {                                           \
-    case 1: return FN<1>(__VA_ARGS__);                            \
-    case 2: return FN<2>(__VA_ARGS__);                            \
-    case 3: return FN<3>(__VA_ARGS__);                            \
-    case 4: return FN<4>(__VA_ARGS__);                            \
-    case 5: return FN<5>(__VA_ARGS__);                            \
-    case 6: return FN<6>(__VA_ARGS__);                            \
-    case 7: return FN<7>(__VA_ARGS__);                            \
-    case 8: return FN<8>(__VA_ARGS__);                            \
-    case 9: return FN<9>(__VA_ARGS__);                            \
-    case 10: return FN<10>(__VA_ARGS__);                          \
-    case 11: return FN<11>(__VA_ARGS__);                          \
-    case 12: return FN<12>(__VA_ARGS__);                          \
-    case 13: return FN<13>(__VA_ARGS__);                          \
-    case 14: return FN<14>(__VA_ARGS__);                          \
-    case 15: return FN<15>(__VA_ARGS__);                          \
-    case 16: return FN<16>(__VA_ARGS__);                          \
+    case 1: return FN<1, false>(__VA_ARGS__);                     \
+    case 2: return FN<2, false>(__VA_ARGS__);                     \
+    case 3: return FN<3, false>(__VA_ARGS__);                     \
+    case 4: return FN<4, false>(__VA_ARGS__);                     \
+    case 5: return FN<5, false>(__VA_ARGS__);                     \
+    case 6: return FN<6, false>(__VA_ARGS__);                     \
+    case 7: return FN<7, false>(__VA_ARGS__);                     \
+    case 8: return FN<8, false>(__VA_ARGS__);                     \
+    case 9: return FN<9, false>(__VA_ARGS__);                     \
+    case 10: return FN<10, false>(__VA_ARGS__);                   \
+    case 11: return FN<11, false>(__VA_ARGS__);                   \
+    case 12: return FN<12, false>(__VA_ARGS__);                   \
+    case 13: return FN<13, false>(__VA_ARGS__);                   \
+    case 14: return FN<14, false>(__VA_ARGS__);                   \
+    case 15: return FN<15, false>(__VA_ARGS__);                   \
+    case 16: return FN<16, false>(__VA_ARGS__);                   \
     default: return hipErrorInvalidValue;                         \
     }
+
+// The batched form exists for the block edges a batch can be tuned to.
+#define MTX_DISPATCH_BATCH_B(FN, ...)                             \
+    switch (a.grid.B) {                                           \
+    case 4: return FN<4, true>(__VA_ARGS__);                      \
+    case 8: return FN<8, true>(__VA_ARGS__);                      \
+    case 16: return FN<16, true>(__VA_ARGS__);                    \
+    default: return hipErrorInvalidValue;                         \
+    }
+
+bool batch_supported_block(uint32_t B) { return B == 4 || B == 8 || B == 16; }
 
 hipError_t launch_extract_count(const ExtractArgs& a, hipStream_t s)
 {
     const uint32_t nwg = persistent_wgs(a.max_blocks);
+    if (a.units) { MTX_DISPATCH_BATCH_B(launch_count_b, a, s, nwg) }
     MTX_DISPATCH_B(launch_count_b, a, s, nwg)
 }
 
 hipError_t launch_extract_emit(const ExtractArgs& a, hipStream_t s)
 {
     const uint32_t nwg = persistent_wgs(a.max_blocks);
+    if (a.units) { MTX_DISPATCH_BATCH_B(launch_emit_b, a, s, nwg) }
     MTX_DISPATCH_B(launch_emit_b, a, s, nwg)
 }
 

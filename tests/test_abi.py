@@ -44,7 +44,10 @@ def test_bindings_cover_every_declared_symbol():
 @pytest.mark.parametrize("name,cls", [
     ("mtx_push_constants", L.PushConstants), ("mtx_volume", L.CVolume),
     ("mtx_options", L.Options), ("mtx_minmax_output", L.CMinMaxOutput),
-    ("mtx_filtering_output", L.CFilteringOutput), ("mtx_extraction_output", L.CExtractionOutput)])
+    ("mtx_filtering_output", L.CFilteringOutput), ("mtx_extraction_output", L.CExtractionOutput),
+    ("mtx_batch_volume", L.CBatchVolume), ("mtx_batch_options", L.BatchOptions),
+    ("mtx_batch_range", L.BatchRange), ("mtx_batch_output", L.CBatchOutput),
+    ("mtx_batch_class_info", L.BatchClassInfo)])
 def test_struct_layouts(name, cls):
     assert L.lib().mtx_struct_size(name.encode()) == ctypes.sizeof(cls)
 
