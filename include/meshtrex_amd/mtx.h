@@ -316,6 +316,8 @@ typedef struct mtx_batch_class_info {
     uint32_t active_blocks;      /* planning run */
     uint64_t blocks;             /* sum of blocks over the class's units */
     uint64_t voxels;             /* sum of voxels over the class's volumes */
+    uint64_t vertices;           /* planning run */
+    uint64_t triangles;          /* planning run */
     float tuned_ms[3];           /* autotuner: class pipeline at B = 4, 8, 16 (0 = not timed) */
     float tuned_rows_ms[3];      /* autotuner: min/max at 1, 2, 4 y-blocks per wave (0 = not timed) */
 } mtx_batch_class_info;

@@ -89,7 +89,7 @@ class CBatchOutput(ctypes.Structure):
 class BatchClassInfo(ctypes.Structure):
     _fields_ = [("dtype", u32), ("block_dim", u32), ("minmax_rows", u32), ("num_volumes", u32),
                 ("num_units", u32), ("active_blocks", u32), ("blocks", u64), ("voxels", u64),
-                ("tuned_ms", f32 * 3), ("tuned_rows_ms", f32 * 3)]
+                ("vertices", u64), ("triangles", u64), ("tuned_ms", f32 * 3), ("tuned_rows_ms", f32 * 3)]
 
 
 # name -> (restype, argtypes)

@@ -13,6 +13,10 @@ namespace {
 
 using namespace mtx;
 
+// Bytes behind a filter's count word for the compaction's tile counts (grown
+// for grids of more than 2^17 tiles).
+constexpr uint64_t kFilterScratch = 4ull * 131072;
+
 mtx_options defaults()
 {
     mtx_options o;
@@ -282,7 +286,7 @@ mtx_status mtx_filter_active_blocks(mtx_context* ctx, const mtx_minmax_output* m
         if (out->compacted_block_ids) mtx_filtering_release(ctx, out);
         std::memset(out, 0, sizeof *out);
         MTX_HIP(ctx, hipMalloc(&out->compacted_block_ids, 4 * (g.nb() ? g.nb() : 1)));
-        hipError_t e = hipMalloc(&out->active_block_count, 256 + 8ull * 65536);
+        hipError_t e = hipMalloc(&out->active_block_count, 256 + kFilterScratch);
         if (e == hipSuccess) e = hipMalloc(&out->block_to_active, 4 * (g.nb() ? g.nb() : 1));
         if (e != hipSuccess) {
             mtx_filtering_release(ctx, out);
@@ -296,28 +300,24 @@ mtx_status mtx_filter_active_blocks(mtx_context* ctx, const mtx_minmax_output* m
     const bool use_mask = mm->active_mask && mm->mask_valid && mm->mask_rule == o.rule &&
                           (mm->mask_isovalue == pc->isovalue ||
                            (std::isnan(mm->mask_isovalue) && std::isnan(pc->isovalue)));
-    const uint32_t words = filter_scan_words(g.nb(), use_mask, mm->mask_words_per_row, g);
-    if (words > 65536 - 1) {
-        // scan state lives behind the count word; grow it for very large grids
+    const uint32_t tiles = filter_tiles(g.nb(), use_mask, mm->mask_words_per_row, g);
+    if (4ull * tiles > kFilterScratch) {
+        // the tile counts live behind the count word; grow them for very large grids
         uint32_t* big = nullptr;
-        if (hipMalloc(&big, 256 + 8ull * (words + 1)) != hipSuccess)
-            return fail(ctx, MTX_ENOMEM, "filter scan state");
+        if (hipMalloc(&big, 256 + 4ull * tiles) != hipSuccess)
+            return fail(ctx, MTX_ENOMEM, "filter tile counts");
         hipFree(out->active_block_count);
         out->active_block_count = big;
     }
-    uint64_t* scan = reinterpret_cast<uint64_t*>(reinterpret_cast<char*>(out->active_block_count) + 256);
-    ScanState st{ scan, reinterpret_cast<uint32_t*>(scan + words - 1) };
-    hipError_t e = launch_zero64(scan, words, bd.s);
-    if (e == hipSuccess) {
-        if (use_mask)
-            e = launch_compact_mask(mm->active_mask, g, mm->mask_words_per_row,
-                                    out->compacted_block_ids, out->active_block_count,
-                                    out->block_to_active, st, bd.s);
-        else
-            e = launch_filter_pairs(mm->minmax, g, make_act((int)mm->dtype, pc->isovalue, (int)o.rule),
-                                    out->compacted_block_ids, out->active_block_count,
-                                    out->block_to_active, st, bd.s);
-    }
+    TileSums ts{ reinterpret_cast<uint64_t*>(reinterpret_cast<char*>(out->active_block_count) + 256) };
+    hipError_t e;
+    if (use_mask)
+        e = launch_compact_mask(mm->active_mask, g, mm->mask_words_per_row, out->compacted_block_ids,
+                                out->active_block_count, out->block_to_active, ts, bd.s);
+    else
+        e = launch_filter_pairs(mm->minmax, g, make_act((int)mm->dtype, pc->isovalue, (int)o.rule),
+                                out->compacted_block_ids, out->active_block_count, out->block_to_active, ts,
+                                bd.s);
     if (e != hipSuccess) return fail(ctx, MTX_EDEVICE, "filter launch: %s", hipGetErrorString(e));
     return finish(bd);
 }
@@ -421,12 +421,9 @@ mtx_status mtx_extract(mtx_context* ctx, const mtx_minmax_output* mm,
             a.edge_t = nullptr;
         }
         if (!o.voxel_positions) a.vox_pos = nullptr;
-        uint64_t* scan = reinterpret_cast<uint64_t*>(static_cast<char*>(out->workspace) + w.scan);
-        ScanState st{ scan, reinterpret_cast<uint32_t*>(scan + 3 * w.scan_tiles) };
-        hipError_t e = launch_zero64(reinterpret_cast<uint64_t*>(static_cast<char*>(out->workspace) + w.totals),
-                                     (w.scan - w.totals) / 8 + 3 * w.scan_tiles + 1, bd.s);
-        if (e == hipSuccess) e = launch_extract_count(a, bd.s);
-        if (e == hipSuccess) e = launch_extract_scan(a, st, bd.s);
+        TileSums ts{ reinterpret_cast<uint64_t*>(static_cast<char*>(out->workspace) + w.scan) };
+        hipError_t e = launch_extract_count(a, bd.s);
+        if (e == hipSuccess) e = launch_extract_scan(a, ts, bd.s);
         if (e == hipSuccess) e = launch_extract_emit(a, bd.s);
         if (e != hipSuccess) return fail(ctx, MTX_EDEVICE, "extract launch: %s", hipGetErrorString(e));
         return finish(bd);
@@ -451,13 +448,10 @@ mtx_status mtx_extract(mtx_context* ctx, const mtx_minmax_output* mm,
     }
     ExtractArgs a = make_extract_args(mm, filt, pc, out, w);
     a.normals = o.normals ? 1 : 0;
-    uint64_t* scan = reinterpret_cast<uint64_t*>(static_cast<char*>(out->workspace) + w.scan);
-    ScanState st{ scan, reinterpret_cast<uint32_t*>(scan + 3 * w.scan_tiles) };
-    e = launch_zero64(reinterpret_cast<uint64_t*>(static_cast<char*>(out->workspace) + w.totals),
-                      (w.scan - w.totals) / 8 + 3 * w.scan_tiles + 1, bd.s);
-    if (e == hipSuccess) e = hipMemsetAsync(out->counts, 0, 256, bd.s);
+    TileSums ts{ reinterpret_cast<uint64_t*>(static_cast<char*>(out->workspace) + w.scan) };
+    e = hipMemsetAsync(out->counts, 0, 256, bd.s);
     if (e == hipSuccess) e = launch_extract_count(a, bd.s);
-    if (e == hipSuccess) e = launch_extract_scan(a, st, bd.s);
+    if (e == hipSuccess) e = launch_extract_scan(a, ts, bd.s);
     uint64_t tot[4] = { 0, 0, 0, 0 };
     if (e == hipSuccess) e = hipMemcpyAsync(tot, a.totals, 32, hipMemcpyDeviceToHost, bd.s);
     if (e == hipSuccess) e = hipStreamSynchronize(bd.s);

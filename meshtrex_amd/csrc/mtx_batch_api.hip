@@ -4,6 +4,7 @@
 
 #include <cstring>
 #include <new>
+#include <string>
 #include <vector>
 
 using namespace mtx;
@@ -40,8 +41,7 @@ struct ClassPlan {
     uint32_t* d_count = nullptr;
     uint32_t* d_ids = nullptr;
     char* d_ws = nullptr;           // extraction workspace (ws_layout)
-    uint64_t* d_fscan = nullptr;    // compaction scan state: ntiles status words + ticket
-    uint64_t zero_words = 0;        // words cleared before a run, from the extraction totals on
+    uint64_t* d_fscan = nullptr;    // compaction tile counts (ntiles uint32)
 
     // outputs (class-relative capacities); the batch owns the memory
     mtx_vertex* vertices = nullptr;
@@ -150,8 +150,7 @@ mtx_status class_build(mtx_context* ctx, const std::vector<VolumeIn>& vols, cons
     const uint32_t nu = (uint32_t)c.hu.size();
     c.w = ws_layout(nb_total, B);
 
-    // slab layout; the words cleared before each run (extraction totals and
-    // scan state, compaction scan state) are contiguous at its end
+    // slab layout (nothing in it needs clearing between runs)
     uint64_t off = 0;
     auto take = [&](uint64_t bytes) { uint64_t r = off; off = align256(off + bytes); return r; };
     const uint64_t o_vols = take(sizeof(BatchVolume) * nv);
@@ -165,13 +164,8 @@ mtx_status class_build(mtx_context* ctx, const std::vector<VolumeIn>& vols, cons
     const uint64_t o_ids = take(4 * nb_total);
     const uint64_t o_first = take(4ull * (nu + 1));
     const uint64_t o_count = take(8);
-    const uint64_t o_ws = take(c.w.totals);                 // workspace up to the totals
-    const uint64_t o_zero = off;                            // totals + extraction scan + compaction scan
-    const uint64_t ws_tail = c.w.bytes - c.w.totals;
-    off += ws_tail;
-    const uint64_t o_fscan = off;
-    off += 8ull * (tiles + 1);
-    c.zero_words = (off - o_zero) / 8;
+    const uint64_t o_ws = take(c.w.bytes);                  // extraction workspace
+    const uint64_t o_fscan = take(4 * tiles);               // compaction tile counts
     hipError_t e = hipMalloc(&c.slab, off);
     if (e != hipSuccess) {
         c.slab = nullptr;
@@ -188,9 +182,7 @@ mtx_status class_build(mtx_context* ctx, const std::vector<VolumeIn>& vols, cons
     c.d_ids = reinterpret_cast<uint32_t*>(c.slab + o_ids);
     c.d_unit_first = reinterpret_cast<uint32_t*>(c.slab + o_first);
     c.d_count = reinterpret_cast<uint32_t*>(c.slab + o_count);
-    // the workspace's totals must sit at c.w.totals from d_ws
-    c.d_ws = c.slab + o_zero - c.w.totals;
-    (void)o_ws;
+    c.d_ws = c.slab + o_ws;
     c.d_fscan = reinterpret_cast<uint64_t*>(c.slab + o_fscan);
 
     for (uint32_t i = 0; i < nv; ++i) c.hv[i].pairs = c.d_pairs + 2 * pair_off[i];
@@ -233,15 +225,13 @@ hipError_t class_minmax(const ClassPlan& c, hipStream_t s)
 }
 
 // The class's whole pipeline on `s`: no allocation, no host synchronisation,
-// memory cleared by kernels (capturable).
+// no memset (capturable).
 hipError_t class_enqueue(const ClassPlan& c, hipStream_t s)
 {
-    hipError_t e = launch_zero64(reinterpret_cast<uint64_t*>(c.d_ws + c.w.totals), c.zero_words, s);
-    if (e == hipSuccess) e = class_minmax(c, s);
+    hipError_t e = class_minmax(c, s);
     if (e == hipSuccess)
         e = launch_filter_batch(c.d_units, c.d_tile0, (uint32_t)c.hu.size(), c.ntiles, c.d_ids, c.d_item_unit,
-                                c.d_unit_first, c.d_count,
-                                ScanState{ c.d_fscan, reinterpret_cast<uint32_t*>(c.d_fscan + c.ntiles) }, s);
+                                c.d_unit_first, c.d_count, TileSums{ c.d_fscan }, s);
     if (e != hipSuccess) return e;
     ExtractArgs a{};
     a.vol.dtype = c.dtype;
@@ -267,9 +257,8 @@ hipError_t class_enqueue(const ClassPlan& c, hipStream_t s)
     a.units = c.d_units;
     a.item_unit = c.d_item_unit;
     a.unit_first = c.d_unit_first;
-    uint64_t* scan = reinterpret_cast<uint64_t*>(c.d_ws + c.w.scan);
     e = launch_extract_count(a, s);
-    if (e == hipSuccess) e = launch_extract_scan(a, ScanState{ scan, reinterpret_cast<uint32_t*>(scan + 3 * c.w.scan_tiles) }, s);
+    if (e == hipSuccess) e = launch_extract_scan(a, TileSums{ reinterpret_cast<uint64_t*>(c.d_ws + c.w.scan) }, s);
     if (e == hipSuccess) e = launch_extract_emit(a, s);
     if (e != hipSuccess || !c.ranges) return e;
     BatchRangeArgs r{};
@@ -511,8 +500,16 @@ mtx_status mtx_batch_create(mtx_context* ctx, const mtx_batch_volume* volumes, u
         c.mcap = with_headroom(c.tot[3], o.headroom_pct);
         V += c.vcap; T += c.icap / 3; Mt += c.mcap; A += c.nb_total;
     }
-    if (V > 0xFFFFFFFFull || 3 * T > 0xFFFFFFFFull || A > 0xFFFFFFFFull)
-        return bail(fail(ctx, MTX_ECAPACITY, "batch outputs exceed 32-bit offsets"));
+    if (V > 0xFFFFFFFFull || 3 * T > 0xFFFFFFFFull || A > 0xFFFFFFFFull) {
+        std::string d;
+        for (const ClassPlan& c : b->classes) {
+            char buf[160];
+            snprintf(buf, sizeof buf, " [dtype %d B %u: %u active blocks, %llu vertices, %llu triangles]", c.dtype,
+                     c.B, c.active, (unsigned long long)c.tot[0], (unsigned long long)c.tot[1]);
+            d += buf;
+        }
+        return bail(fail(ctx, MTX_ECAPACITY, "batch outputs exceed 32-bit offsets:%s", d.c_str()));
+    }
     uint64_t off = 0;
     auto take = [&](uint64_t bytes) { uint64_t r = off; off = align256(off + (bytes ? bytes : 1)); return r; };
     const uint64_t o_v = take(32 * V), o_i = take(12 * T), o_m = take(16 * Mt);
@@ -607,6 +604,8 @@ mtx_status mtx_batch_class_info_get(const mtx_batch* b, uint32_t index, mtx_batc
     out->active_blocks = c.active;
     out->blocks = c.nb_total;
     out->voxels = c.voxels;
+    out->vertices = c.tot[0];
+    out->triangles = c.tot[1];
     for (int k = 0; k < 3; ++k) {
         out->tuned_ms[k] = c.tuned_ms[k];
         out->tuned_rows_ms[k] = c.tuned_rows_ms[k];

@@ -1,5 +1,5 @@
 // Device helpers shared by the kernel translation units: voxel keys, the
-// activity rule, wave scans, XCD-aware work order, decoupled look-back.
+// activity rule, wave and workgroup sums, XCD-aware work order.
 #pragma once
 
 #include "mtx_internal.h"
@@ -72,54 +72,29 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t wg, uint32_t nwg)
 }
 
 // ---------------------------------------------------------------------------
-// decoupled look-back (single-pass scan across workgroups)
-// status word: bits 63:62 = 1 aggregate ready, 2 inclusive prefix ready
+// workgroup sums (256 threads)
 // ---------------------------------------------------------------------------
 
-constexpr uint64_t kFlagAgg = 1ull << 62;
-constexpr uint64_t kFlagInc = 2ull << 62;
-constexpr uint64_t kValMask = (1ull << 62) - 1;
-
-__device__ __forceinline__ uint64_t ld_acq(const uint64_t* p)
+// Sum of v over the workgroup, returned to every thread; s: 4 words of LDS.
+template <typename V, typename S>
+__device__ __forceinline__ S wg_sum256(V v, S* s)
 {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    const S w = wave_sum<S>((S)v);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = w;
+    __syncthreads();
+    const S r = s[0] + s[1] + s[2] + s[3];
+    __syncthreads();
+    return r;
 }
-__device__ __forceinline__ void st_rel(uint64_t* p, uint64_t v)
-{
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
 
-// Called by one whole wave; returns the exclusive prefix of `tile` (same in all lanes).
-__device__ inline uint64_t lookback(uint64_t* status, uint32_t tile, uint64_t aggregate)
+// Exclusive prefix of tile t from the per-tile counts of an earlier launch:
+// the workgroup adds up its predecessors' counts itself.
+template <typename V>
+__device__ __forceinline__ uint64_t tile_prefix(const V* __restrict__ counts, uint32_t t, uint64_t* s)
 {
-    const uint32_t lane = lane_id();
-    if (tile == 0) {
-        if (lane == 0) st_rel(&status[0], kFlagInc | aggregate);
-        return 0;
-    }
-    if (lane == 0) st_rel(&status[tile], kFlagAgg | aggregate);
-    uint64_t excl = 0;
-    int64_t end = (int64_t)tile - 1;
-    for (;;) {
-        int64_t idx = end - (int64_t)lane;
-        uint64_t s = idx >= 0 ? ld_acq(&status[idx]) : kFlagInc;
-        uint64_t flag = s >> 62;
-        if (__any(flag == 0)) {
-            __builtin_amdgcn_s_sleep(1);
-            continue;
-        }
-        uint64_t inc = __ballot(flag == 2);
-        uint64_t v = s & kValMask;
-        if (inc) {
-            uint32_t first = __builtin_ctzll(inc);
-            excl += wave_sum<uint64_t>(lane <= first ? v : 0ull);
-            break;
-        }
-        excl += wave_sum<uint64_t>(v);
-        end -= 64;
-    }
-    if (lane == 0) st_rel(&status[tile], kFlagInc | (excl + aggregate));
-    return excl;
+    uint64_t acc = 0;
+    for (uint32_t k = threadIdx.x; k < t; k += 256) acc += counts[k];
+    return wg_sum256<uint64_t, uint64_t>(acc, s);
 }
 
 }  // namespace mtx

@@ -31,10 +31,10 @@ struct Grid {
     uint64_t nb() const { return (uint64_t)gx * gy * gz; }
 };
 
-// Lookback scan state: one 64-bit status word per tile plus a ticket counter.
-struct ScanState {
-    uint64_t* status;   // zeroed before use
-    uint32_t* ticket;   // zeroed before use
+// Scratch of the two-launch scans: per-tile sums written by the first launch
+// and read by the second (no clearing needed).
+struct TileSums {
+    uint64_t* sums;
 };
 
 // ---- launchers (all asynchronous on `s`) ----------------------------------
@@ -53,22 +53,20 @@ hipError_t launch_minmax_generic(const VolDesc& v, const Grid& g, int overlap, u
 hipError_t launch_pyramid(uint32_t* pairs, uint32_t levels, const uint32_t (*ext)[3],
                           const uint64_t* off, hipStream_t s);
 
-// Number of 64-bit words a compaction's scan state needs for nb blocks.
-uint32_t filter_scan_words(uint64_t nb, bool from_mask, uint32_t words_per_row, const Grid& g);
+// Tiles of a compaction of nb blocks (one uint32 of TileSums each).
+uint32_t filter_tiles(uint64_t nb, bool from_mask, uint32_t words_per_row, const Grid& g);
 
 // Ordered compaction of active block IDs from the fused bit mask.  Also writes
 // the dense map block ID -> active index (0xFFFFFFFF for inactive blocks).
 hipError_t launch_compact_mask(const uint32_t* mask, const Grid& g, uint32_t words_per_row,
-                               uint32_t* ids, uint32_t* count, uint32_t* map, ScanState st,
+                               uint32_t* ids, uint32_t* count, uint32_t* map, TileSums ts,
                                hipStream_t s);
 
 // Ordered compaction of active block IDs from leaf min/max pairs (+ dense map).
 hipError_t launch_filter_pairs(const uint32_t* pairs, const Grid& g, const ActParams& act,
-                               uint32_t* ids, uint32_t* count, uint32_t* map, ScanState st,
+                               uint32_t* ids, uint32_t* count, uint32_t* map, TileSums ts,
                                hipStream_t s);
 
-// Clear n 64-bit words (a small fill kernel, safe under graph capture).
-hipError_t launch_zero64(uint64_t* p, uint64_t n, hipStream_t s);
 
 // ---- batches -----------------------------------------------------------------
 
@@ -110,7 +108,7 @@ hipError_t launch_minmax_batch(const BatchVolume* vols, const uint32_t* wg0, uin
 uint32_t batch_filter_tiles(uint32_t nb);   // compaction tiles of a unit of nb blocks
 hipError_t launch_filter_batch(const BatchUnit* units, const uint32_t* tile0, uint32_t nunits,
                                uint32_t ntiles, uint32_t* ids, uint32_t* item_unit,
-                               uint32_t* unit_first, uint32_t* count, ScanState st, hipStream_t s);
+                               uint32_t* unit_first, uint32_t* count, TileSums ts, hipStream_t s);
 
 // Per-unit output ranges after the extraction scan (see mtx_batch_range).
 struct BatchRangeArgs {
@@ -170,7 +168,8 @@ size_t corner_entry_bytes(uint32_t B);
 uint32_t scan_tile_items();
 
 hipError_t launch_extract_count(const ExtractArgs& a, hipStream_t s);
-hipError_t launch_extract_scan(const ExtractArgs& a, ScanState st, hipStream_t s);
+// Also clears the emit pass's error flags (totals[2]); TileSums: 3 words per scan tile.
+hipError_t launch_extract_scan(const ExtractArgs& a, TileSums ts, hipStream_t s);
 hipError_t launch_extract_emit(const ExtractArgs& a, hipStream_t s);
 
 // Adjoint: dL/dvoxel (dense, overwritten) and dL/diso (one float, overwritten)

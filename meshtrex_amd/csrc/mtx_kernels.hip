@@ -10,21 +10,6 @@
 
 namespace mtx {
 
-__global__ void k_zero64(uint64_t* __restrict__ p, uint64_t n)
-{
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * blockDim.x)
-        p[i] = 0;
-}
-
-hipError_t launch_zero64(uint64_t* p, uint64_t n, hipStream_t s)
-{
-    if (n == 0) return hipSuccess;
-    uint64_t g = (n + 255) / 256;
-    if (g > 1024) g = 1024;
-    hipLaunchKernelGGL(k_zero64, dim3((uint32_t)g), dim3(256), 0, s, p, n);
-    return hipGetLastError();
-}
 
 // ---------------------------------------------------------------------------
 // block min/max, B = 8, overlap 1: the roofline kernel
@@ -611,108 +596,154 @@ hipError_t launch_pyramid(uint32_t* pairs, uint32_t levels, const uint32_t (*ext
 
 // ---------------------------------------------------------------------------
 // ordered compaction of active block IDs
+//
+// Reduce-then-scatter in two launches, with no waits between workgroups: the
+// first counts the active blocks of every tile, the second lets each tile sum
+// its predecessors' counts itself (at most a few thousand cached words) and
+// scatter its IDs.  On a chip of 8 XCDs with separate L2s, polling another
+// workgroup's published prefix costs a coherent round trip per look-back
+// window; reading the finished counts of a previous launch does not.
 // ---------------------------------------------------------------------------
 
-constexpr uint32_t kFilterItems = 8;  // blocks per thread (pairs path)
+constexpr uint32_t kFilterItems = 8;                     // blocks per thread (pairs path)
+constexpr uint32_t kFilterTile = 256 * kFilterItems;     // blocks per tile (pairs path)
+constexpr uint32_t kMaskTileWords = 256;                 // mask words per tile (one per thread)
 
-uint32_t filter_scan_words(uint64_t nb, bool from_mask, uint32_t words_per_row, const Grid& g)
+uint32_t filter_tiles(uint64_t nb, bool from_mask, uint32_t words_per_row, const Grid& g)
 {
     uint64_t tiles;
-    if (from_mask) {
-        uint64_t words = (uint64_t)g.gy * g.gz * words_per_row;
-        tiles = (words + 255) / 256;
-    } else {
-        tiles = (nb + 256 * kFilterItems - 1) / (256 * kFilterItems);
-    }
-    if (tiles == 0) tiles = 1;
-    return (uint32_t)tiles + 1;  // +1: the ticket counter lives in the last word
+    if (from_mask) tiles = ((uint64_t)g.gy * g.gz * words_per_row + kMaskTileWords - 1) / kMaskTileWords;
+    else tiles = (nb + kFilterTile - 1) / kFilterTile;
+    return (uint32_t)(tiles ? tiles : 1);
 }
 
-// From leaf pairs: 8 blocks per thread, in block-ID order (k, wave, lane).
-__global__ __launch_bounds__(256) void k_filter_pairs(const uint2* __restrict__ pairs, uint64_t nb,
-                                                      ActParams act, uint32_t* __restrict__ ids,
-                                                      uint32_t* __restrict__ count,
-                                                      uint32_t* __restrict__ map,
-                                                      uint64_t* status, uint32_t* ticket,
-                                                      uint32_t ntiles)
+// Active flags of this thread's 8 blocks of `tile` as wave ballots, and the
+// wave's counts in s_cnt[k * 4 + wave].
+__device__ __forceinline__ void tile_ballots(const uint2* __restrict__ pairs, uint64_t nb, uint64_t base,
+                                             const ActParams& act, uint64_t (&bal)[kFilterItems],
+                                             uint32_t* s_cnt)
 {
-    __shared__ uint32_t s_tile;
-    __shared__ uint32_t s_cnt[kFilterItems * 4];
-    __shared__ uint64_t s_excl;
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
-    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
-    __syncthreads();
-    const uint32_t tile = s_tile;
-    const uint64_t base = (uint64_t)tile * 256u * kFilterItems;
-    uint64_t bal[kFilterItems];
 #pragma unroll
     for (uint32_t k = 0; k < kFilterItems; ++k) {
-        uint64_t b = base + k * 256u + threadIdx.x;
+        const uint64_t b = base + k * 256u + threadIdx.x;
         bool a = false;
         if (b < nb) {
-            uint2 p = pairs[b];
+            const uint2 p = pairs[b];
             a = block_active(p.x, p.y, act);
         }
         bal[k] = __ballot(a);
         if (lane == 0) s_cnt[k * 4 + w] = (uint32_t)__popcll(bal[k]);
     }
+}
+
+__global__ __launch_bounds__(256) void k_filter_pairs_count(const uint2* __restrict__ pairs, uint64_t nb,
+                                                            ActParams act, uint32_t* __restrict__ tile_count)
+{
+    __shared__ uint32_t s_cnt[kFilterItems * 4];
+    uint64_t bal[kFilterItems];
+    tile_ballots(pairs, nb, (uint64_t)blockIdx.x * kFilterTile, act, bal, s_cnt);
     __syncthreads();
-    if (w == 0) {
-        uint32_t v = lane < kFilterItems * 4 ? s_cnt[lane] : 0u;
-        uint32_t inc = wave_incl_scan<uint32_t>(v);
-        uint32_t agg = __shfl(inc, 63, 64);
-        if (lane < kFilterItems * 4) s_cnt[lane] = inc - v;
-        uint64_t ex = lookback(status, tile, agg);
-        if (lane == 0) {
-            s_excl = ex;
-            if (tile == ntiles - 1) *count = (uint32_t)(ex + agg);
-        }
+    if (threadIdx.x < 64) {
+        const uint32_t v = threadIdx.x < kFilterItems * 4 ? s_cnt[threadIdx.x] : 0u;
+        const uint32_t t = wave_sum<uint32_t>(v);
+        if (threadIdx.x == 0) tile_count[blockIdx.x] = t;
     }
-    __syncthreads();
+}
+
+// Scatter of one tile's IDs (block-ID order: k, wave, lane) at `excl`; writes
+// the dense map of the tile's blocks.  s_cnt: the tile's per-(k, wave) counts.
+__device__ __forceinline__ void tile_scatter(const uint64_t (&bal)[kFilterItems], uint32_t* s_cnt, uint64_t base,
+                                             uint64_t nb, uint64_t excl, uint32_t* __restrict__ ids,
+                                             uint32_t* __restrict__ item_unit, uint32_t unit,
+                                             uint32_t* __restrict__ map)
+{
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
     const uint64_t lt = (1ull << lane) - 1ull;
 #pragma unroll
     for (uint32_t k = 0; k < kFilterItems; ++k) {
         const uint64_t b = base + k * 256u + threadIdx.x;
         uint32_t slot = 0xFFFFFFFFu;
         if ((bal[k] >> lane) & 1ull) {
-            uint64_t pos = s_excl + s_cnt[k * 4 + w] + (uint32_t)__popcll(bal[k] & lt);
+            const uint64_t pos = excl + s_cnt[k * 4 + w] + (uint32_t)__popcll(bal[k] & lt);
             ids[pos] = (uint32_t)b;
+            if (item_unit) item_unit[pos] = unit;
             slot = (uint32_t)pos;
         }
         if (b < nb) map[b] = slot;
     }
 }
 
+// Exclusive scan of the tile's 32 (k, wave) counts in place; returns the tile total.
+__device__ __forceinline__ uint32_t tile_local_scan(uint32_t* s_cnt)
+{
+    uint32_t tot = 0;
+    if (threadIdx.x < 64) {
+        const uint32_t v = threadIdx.x < kFilterItems * 4 ? s_cnt[threadIdx.x] : 0u;
+        const uint32_t inc = wave_incl_scan<uint32_t>(v);
+        tot = __shfl(inc, 63, 64);
+        if (threadIdx.x < kFilterItems * 4) s_cnt[threadIdx.x] = inc - v;
+    }
+    return tot;
+}
+
+__global__ __launch_bounds__(256) void k_filter_pairs_scatter(const uint2* __restrict__ pairs, uint64_t nb,
+                                                              ActParams act, uint32_t* __restrict__ ids,
+                                                              uint32_t* __restrict__ count,
+                                                              uint32_t* __restrict__ map,
+                                                              const uint32_t* __restrict__ tile_count,
+                                                              uint32_t ntiles)
+{
+    __shared__ uint32_t s_cnt[kFilterItems * 4];
+    __shared__ uint64_t s_red[4];
+    const uint32_t tile = blockIdx.x;
+    const uint64_t base = (uint64_t)tile * kFilterTile;
+    const uint64_t excl = tile_prefix<uint32_t>(tile_count, tile, s_red);
+    uint64_t bal[kFilterItems];
+    tile_ballots(pairs, nb, base, act, bal, s_cnt);
+    __syncthreads();
+    const uint32_t tot = tile_local_scan(s_cnt);
+    if (threadIdx.x == 0 && tile == ntiles - 1) *count = (uint32_t)(excl + tot);
+    __syncthreads();
+    tile_scatter(bal, s_cnt, base, nb, excl, ids, nullptr, 0, map);
+}
+
 hipError_t launch_filter_pairs(const uint32_t* pairs, const Grid& g, const ActParams& act,
-                               uint32_t* ids, uint32_t* count, uint32_t* map, ScanState st,
+                               uint32_t* ids, uint32_t* count, uint32_t* map, TileSums ts,
                                hipStream_t s)
 {
     const uint64_t nb = g.nb();
-    uint64_t ntiles = (nb + 256 * kFilterItems - 1) / (256 * kFilterItems);
-    if (ntiles == 0) ntiles = 1;
-    hipLaunchKernelGGL(k_filter_pairs, dim3((uint32_t)ntiles), dim3(256), 0, s,
-                       reinterpret_cast<const uint2*>(pairs), nb, act, ids, count, map, st.status,
-                       st.ticket, (uint32_t)ntiles);
+    const uint32_t ntiles = filter_tiles(nb, false, 0, g);
+    const uint2* pr = reinterpret_cast<const uint2*>(pairs);
+    uint32_t* tc = reinterpret_cast<uint32_t*>(ts.sums);
+    hipLaunchKernelGGL(k_filter_pairs_count, dim3(ntiles), dim3(256), 0, s, pr, nb, act, tc);
+    hipLaunchKernelGGL(k_filter_pairs_scatter, dim3(ntiles), dim3(256), 0, s, pr, nb, act, ids, count, map, tc,
+                       ntiles);
     return hipGetLastError();
 }
 
 // From the fused bit mask: one 32-bit word (32 x-blocks of one row) per thread.
-__global__ __launch_bounds__(256) void k_compact_mask(const uint32_t* __restrict__ mask,
-                                                      uint64_t nwords, uint32_t gx, uint32_t wpr,
-                                                      uint32_t* __restrict__ ids,
-                                                      uint32_t* __restrict__ count,
-                                                      uint32_t* __restrict__ map,
-                                                      uint64_t* status, uint32_t* ticket,
-                                                      uint32_t ntiles)
+__global__ __launch_bounds__(256) void k_mask_count(const uint32_t* __restrict__ mask, uint64_t nwords,
+                                                    uint32_t* __restrict__ tile_count)
 {
-    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_red[4];
+    const uint64_t wi = (uint64_t)blockIdx.x * kMaskTileWords + threadIdx.x;
+    const uint32_t c = wi < nwords ? (uint32_t)__popc(mask[wi]) : 0u;
+    const uint32_t t = wg_sum256<uint32_t>(c, s_red);
+    if (threadIdx.x == 0) tile_count[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(256) void k_mask_scatter(const uint32_t* __restrict__ mask, uint64_t nwords,
+                                                      uint32_t gx, uint32_t wpr, uint32_t* __restrict__ ids,
+                                                      uint32_t* __restrict__ count, uint32_t* __restrict__ map,
+                                                      const uint32_t* __restrict__ tile_count, uint32_t ntiles)
+{
+    __shared__ uint64_t s_red[4];
     __shared__ uint32_t s_w[4];
-    __shared__ uint64_t s_excl;
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
-    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
-    __syncthreads();
-    const uint32_t tile = s_tile;
-    const uint64_t wi = (uint64_t)tile * 256u + threadIdx.x;
+    const uint32_t tile = blockIdx.x;
+    const uint64_t excl = tile_prefix<uint32_t>(tile_count, tile, s_red);
+    const uint64_t wi = (uint64_t)tile * kMaskTileWords + threadIdx.x;
     const uint32_t word = wi < nwords ? mask[wi] : 0u;
     const uint32_t c = (uint32_t)__popc(word);
     const uint32_t inc = wave_incl_scan<uint32_t>(c);
@@ -724,24 +755,17 @@ __global__ __launch_bounds__(256) void k_compact_mask(const uint32_t* __restrict
         woff += j < w ? s_w[j] : 0u;
         agg += s_w[j];
     }
-    if (w == 0) {
-        uint64_t ex = lookback(status, tile, agg);
-        if (lane == 0) {
-            s_excl = ex;
-            if (tile == ntiles - 1) *count = (uint32_t)(ex + agg);
-        }
-    }
-    __syncthreads();
-    uint64_t pos = s_excl + woff + inc - c;
+    if (threadIdx.x == 0 && tile == ntiles - 1) *count = (uint32_t)(excl + agg);
+    uint64_t pos = excl + woff + inc - c;
     if (wi < nwords) {
         const uint64_t row = wi / wpr;
         const uint32_t x0 = (uint32_t)(wi % wpr) * 32u;
-        const uint32_t base = (uint32_t)(row * gx + x0);
+        const uint64_t base = row * gx + x0;
         const uint32_t nbits = min(32u, gx - x0);
         for (uint32_t bit = 0; bit < nbits; ++bit) {
             uint32_t slot = 0xFFFFFFFFu;
             if ((word >> bit) & 1u) {
-                ids[pos] = base + bit;
+                ids[pos] = (uint32_t)(base + bit);
                 slot = (uint32_t)pos++;
             }
             map[base + bit] = slot;
@@ -750,14 +774,99 @@ __global__ __launch_bounds__(256) void k_compact_mask(const uint32_t* __restrict
 }
 
 hipError_t launch_compact_mask(const uint32_t* mask, const Grid& g, uint32_t words_per_row,
-                               uint32_t* ids, uint32_t* count, uint32_t* map, ScanState st,
+                               uint32_t* ids, uint32_t* count, uint32_t* map, TileSums ts,
                                hipStream_t s)
 {
     const uint64_t nwords = (uint64_t)g.gy * g.gz * words_per_row;
-    uint64_t ntiles = (nwords + 255) / 256;
-    if (ntiles == 0) ntiles = 1;
-    hipLaunchKernelGGL(k_compact_mask, dim3((uint32_t)ntiles), dim3(256), 0, s, mask, nwords, g.gx,
-                       words_per_row, ids, count, map, st.status, st.ticket, (uint32_t)ntiles);
+    const uint32_t ntiles = filter_tiles(g.nb(), true, words_per_row, g);
+    uint32_t* tc = reinterpret_cast<uint32_t*>(ts.sums);
+    hipLaunchKernelGGL(k_mask_count, dim3(ntiles), dim3(256), 0, s, mask, nwords, tc);
+    hipLaunchKernelGGL(k_mask_scatter, dim3(ntiles), dim3(256), 0, s, mask, nwords, g.gx, words_per_row, ids,
+                       count, map, tc, ntiles);
+    return hipGetLastError();
+}
+
+// Batched form: one compaction over the tiles of every unit of a class, in
+// unit order, so that the concatenated list is ordered by (unit, block ID).
+// Each unit's blocks start on a tile boundary; a tile finds its unit with one
+// wave ballot per 64 units.  Activity follows the unit's own isovalue and rule.
+__device__ __forceinline__ uint32_t batch_tile_unit(const uint32_t* __restrict__ tile0, uint32_t nunits,
+                                                    uint32_t tile, uint32_t* s_unit)
+{
+    if (threadIdx.x < 64) {
+        uint32_t n = 0;
+        for (uint32_t c = 0; c < nunits; c += 64) {
+            const uint32_t k = c + threadIdx.x;
+            n += (uint32_t)__popcll(__ballot(k < nunits && tile0[k] <= tile));
+        }
+        if (threadIdx.x == 0) *s_unit = n - 1;
+    }
+    __syncthreads();
+    return __builtin_amdgcn_readfirstlane(*s_unit);
+}
+
+__global__ __launch_bounds__(256) void k_filter_batch_count(const BatchUnit* __restrict__ units,
+                                                            const uint32_t* __restrict__ tile0, uint32_t nunits,
+                                                            uint32_t* __restrict__ tile_count)
+{
+    __shared__ uint32_t s_cnt[kFilterItems * 4];
+    __shared__ uint32_t s_unit;
+    const uint32_t tile = blockIdx.x;
+    const uint32_t u = batch_tile_unit(tile0, nunits, tile, &s_unit);
+    const BatchUnit& U = units[u];
+    uint64_t bal[kFilterItems];
+    tile_ballots(reinterpret_cast<const uint2*>(U.pairs), U.nb, (uint64_t)(tile - U.tile0) * kFilterTile, U.act,
+                 bal, s_cnt);
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const uint32_t v = threadIdx.x < kFilterItems * 4 ? s_cnt[threadIdx.x] : 0u;
+        const uint32_t t = wave_sum<uint32_t>(v);
+        if (threadIdx.x == 0) tile_count[tile] = t;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_filter_batch_scatter(const BatchUnit* __restrict__ units,
+                                                              const uint32_t* __restrict__ tile0, uint32_t nunits,
+                                                              uint32_t ntiles, uint32_t* __restrict__ ids,
+                                                              uint32_t* __restrict__ item_unit,
+                                                              uint32_t* __restrict__ unit_first,
+                                                              uint32_t* __restrict__ count,
+                                                              const uint32_t* __restrict__ tile_count)
+{
+    __shared__ uint32_t s_cnt[kFilterItems * 4];
+    __shared__ uint64_t s_red[4];
+    __shared__ uint32_t s_unit;
+    const uint32_t tile = blockIdx.x;
+    const uint32_t u = batch_tile_unit(tile0, nunits, tile, &s_unit);
+    const uint64_t excl = tile_prefix<uint32_t>(tile_count, tile, s_red);
+    const BatchUnit& U = units[u];
+    const uint64_t base = (uint64_t)(tile - U.tile0) * kFilterTile;
+    uint64_t bal[kFilterItems];
+    tile_ballots(reinterpret_cast<const uint2*>(U.pairs), U.nb, base, U.act, bal, s_cnt);
+    __syncthreads();
+    const uint32_t tot = tile_local_scan(s_cnt);
+    if (threadIdx.x == 0) {
+        if (tile == U.tile0) unit_first[u] = (uint32_t)excl;
+        if (tile == ntiles - 1) {
+            *count = (uint32_t)(excl + tot);
+            unit_first[nunits] = (uint32_t)(excl + tot);
+        }
+    }
+    __syncthreads();
+    tile_scatter(bal, s_cnt, base, U.nb, excl, ids, item_unit, u, U.map);
+}
+
+uint32_t batch_filter_tiles(uint32_t nb) { return (nb + kFilterTile - 1) / kFilterTile; }
+
+hipError_t launch_filter_batch(const BatchUnit* units, const uint32_t* tile0, uint32_t nunits,
+                               uint32_t ntiles, uint32_t* ids, uint32_t* item_unit,
+                               uint32_t* unit_first, uint32_t* count, TileSums ts, hipStream_t s)
+{
+    if (nunits == 0 || ntiles == 0) return hipErrorInvalidValue;
+    uint32_t* tc = reinterpret_cast<uint32_t*>(ts.sums);
+    hipLaunchKernelGGL(k_filter_batch_count, dim3(ntiles), dim3(256), 0, s, units, tile0, nunits, tc);
+    hipLaunchKernelGGL(k_filter_batch_scatter, dim3(ntiles), dim3(256), 0, s, units, tile0, nunits, ntiles, ids,
+                       item_unit, unit_first, count, tc);
     return hipGetLastError();
 }
 
@@ -1086,10 +1195,61 @@ __global__ __launch_bounds__(256) void k_extract_count_b8(ExtractArgs a)
     }
 }
 
-// Exclusive scan of per-block {vertices, triangles, meshlets} into bases.
-// Scan state: 3 * tiles status words plus a ticket counter.
+// Exclusive scan of per-block {vertices, triangles, meshlets} into bases, in
+// two launches (reduce-then-scatter, as the compaction): tile sums, then each
+// tile adds up its predecessors' sums and writes its blocks' bases.  Tile
+// sums: 3 words per tile.
 constexpr uint32_t kScanItems = 4;
 uint32_t scan_tile_items() { return 256 * kScanItems; }
+
+__device__ __forceinline__ uint32_t scan_tiles_of(uint32_t na)
+{
+    const uint32_t n = (na + 256 * kScanItems - 1) / (256 * kScanItems);
+    return n ? n : 1;
+}
+
+__device__ __forceinline__ void block_counts(const uint32_t* __restrict__ cnt, uint32_t na, uint64_t i,
+                                             uint64_t& v, uint64_t& t, uint64_t& m)
+{
+    v = i < na ? cnt[2 * i] : 0;
+    t = i < na ? cnt[2 * i + 1] : 0;
+    m = (t + kMeshletPrims - 1) / kMeshletPrims;
+}
+
+__global__ __launch_bounds__(256) void k_extract_tile_sums(const uint32_t* __restrict__ cnt,
+                                                           const uint32_t* __restrict__ count,
+                                                           uint64_t* __restrict__ tile_sums,
+                                                           uint64_t* __restrict__ totals)
+{
+    __shared__ uint64_t s_red[12];
+    const uint32_t na = *count;
+    const uint32_t tile = blockIdx.x;
+    if (tile >= scan_tiles_of(na)) return;
+    if (tile == 0 && threadIdx.x == 0) totals[2] = 0;  // error flags of the emit pass
+    uint64_t sv = 0, st = 0, sm = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kScanItems; ++k) {
+        uint64_t v, t, m;
+        block_counts(cnt, na, (uint64_t)tile * 256 * kScanItems + threadIdx.x * kScanItems + k, v, t, m);
+        sv += v;
+        st += t;
+        sm += m;
+    }
+    sv = wave_sum<uint64_t>(sv);
+    st = wave_sum<uint64_t>(st);
+    sm = wave_sum<uint64_t>(sm);
+    const uint32_t w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        s_red[w] = sv;
+        s_red[4 + w] = st;
+        s_red[8 + w] = sm;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const uint64_t* r = s_red + 4 * threadIdx.x;
+        tile_sums[3 * (uint64_t)tile + threadIdx.x] = r[0] + r[1] + r[2] + r[3];
+    }
+}
 
 __global__ __launch_bounds__(256) void k_extract_scan(const uint32_t* __restrict__ cnt,
                                                       const uint32_t* __restrict__ count,
@@ -1098,41 +1258,51 @@ __global__ __launch_bounds__(256) void k_extract_scan(const uint32_t* __restrict
                                                       uint64_t* __restrict__ mbase,
                                                       uint64_t* __restrict__ totals,
                                                       uint32_t* __restrict__ out_counts,
-                                                      uint64_t* status, uint32_t tcap,
-                                                      uint32_t* ticket)
+                                                      const uint64_t* __restrict__ tile_sums)
 {
-    __shared__ uint32_t s_tile;
     __shared__ uint64_t s_w[12];
-    __shared__ uint64_t s_ex[3];
+    __shared__ uint64_t s_red[12];
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
     const uint32_t na = *count;
-    uint32_t ntiles = (na + 256 * kScanItems - 1) / (256 * kScanItems);
-    if (ntiles == 0) ntiles = 1;
-    if (blockIdx.x >= ntiles) return;  // surplus workgroups leave before taking a ticket
-    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
-    __syncthreads();
-    const uint32_t tile = s_tile;
+    const uint32_t ntiles = scan_tiles_of(na);
+    const uint32_t tile = blockIdx.x;
+    if (tile >= ntiles) return;
+    // exclusive prefix of this tile: its predecessors' sums
+    uint64_t pv = 0, pt = 0, pm = 0;
+    for (uint32_t k = threadIdx.x; k < tile; k += 256) {
+        pv += tile_sums[3 * (uint64_t)k];
+        pt += tile_sums[3 * (uint64_t)k + 1];
+        pm += tile_sums[3 * (uint64_t)k + 2];
+    }
+    pv = wave_sum<uint64_t>(pv);
+    pt = wave_sum<uint64_t>(pt);
+    pm = wave_sum<uint64_t>(pm);
+    if (lane == 0) {
+        s_red[w] = pv;
+        s_red[4 + w] = pt;
+        s_red[8 + w] = pm;
+    }
     uint64_t v[kScanItems], t[kScanItems], m[kScanItems];
     uint64_t sv = 0, st = 0, sm = 0;
 #pragma unroll
     for (uint32_t k = 0; k < kScanItems; ++k) {
-        uint64_t i = (uint64_t)tile * 256 * kScanItems + threadIdx.x * kScanItems + k;
-        v[k] = i < na ? cnt[2 * i] : 0;
-        t[k] = i < na ? cnt[2 * i + 1] : 0;
-        m[k] = (t[k] + kMeshletPrims - 1) / kMeshletPrims;
+        block_counts(cnt, na, (uint64_t)tile * 256 * kScanItems + threadIdx.x * kScanItems + k, v[k], t[k], m[k]);
         sv += v[k];
         st += t[k];
         sm += m[k];
     }
-    uint64_t iv = wave_incl_scan<uint64_t>(sv);
-    uint64_t it = wave_incl_scan<uint64_t>(st);
-    uint64_t im = wave_incl_scan<uint64_t>(sm);
+    const uint64_t iv = wave_incl_scan<uint64_t>(sv);
+    const uint64_t it = wave_incl_scan<uint64_t>(st);
+    const uint64_t im = wave_incl_scan<uint64_t>(sm);
     if (lane == 63) {
         s_w[w] = iv;
         s_w[4 + w] = it;
         s_w[8 + w] = im;
     }
     __syncthreads();
+    const uint64_t exv = s_red[0] + s_red[1] + s_red[2] + s_red[3];
+    const uint64_t ext = s_red[4] + s_red[5] + s_red[6] + s_red[7];
+    const uint64_t exm = s_red[8] + s_red[9] + s_red[10] + s_red[11];
     uint64_t ov = 0, ot = 0, om = 0, av = 0, at = 0, am = 0;
 #pragma unroll
     for (uint32_t j = 0; j < 4; ++j) {
@@ -1143,36 +1313,23 @@ __global__ __launch_bounds__(256) void k_extract_scan(const uint32_t* __restrict
         at += s_w[4 + j];
         am += s_w[8 + j];
     }
-    if (w == 0) {
-        // independent look-backs: vertices in status[0, tcap), triangles in
-        // [tcap, 2 tcap), meshlets in [2 tcap, 3 tcap)
-        uint64_t exv = lookback(status, tile, av);
-        uint64_t ext = lookback(status + tcap, tile, at);
-        uint64_t exm = lookback(status + 2 * (uint64_t)tcap, tile, am);
-        if (lane == 0) {
-            s_ex[0] = exv;
-            s_ex[1] = ext;
-            s_ex[2] = exm;
-            if (tile == ntiles - 1) {
-                totals[0] = exv + av;
-                totals[1] = ext + at;
-                totals[3] = exm + am;
-                if (out_counts) {
-                    uint64_t V = exv + av, I = 3 * (ext + at), M = exm + am;
-                    out_counts[0] = V > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)V;
-                    out_counts[1] = I > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)I;
-                    out_counts[2] = M > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)M;
-                }
-            }
+    if (threadIdx.x == 0 && tile == ntiles - 1) {
+        totals[0] = exv + av;
+        totals[1] = ext + at;
+        totals[3] = exm + am;
+        if (out_counts) {
+            const uint64_t V = exv + av, I = 3 * (ext + at), M = exm + am;
+            out_counts[0] = V > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)V;
+            out_counts[1] = I > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)I;
+            out_counts[2] = M > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)M;
         }
     }
-    __syncthreads();
-    uint64_t rv = s_ex[0] + ov + iv - sv;
-    uint64_t rt = s_ex[1] + ot + it - st;
-    uint64_t rm = s_ex[2] + om + im - sm;
+    uint64_t rv = exv + ov + iv - sv;
+    uint64_t rt = ext + ot + it - st;
+    uint64_t rm = exm + om + im - sm;
 #pragma unroll
     for (uint32_t k = 0; k < kScanItems; ++k) {
-        uint64_t i = (uint64_t)tile * 256 * kScanItems + threadIdx.x * kScanItems + k;
+        const uint64_t i = (uint64_t)tile * 256 * kScanItems + threadIdx.x * kScanItems + k;
         if (i < na) {
             vbase[i] = rv;
             tbase[i] = rt;
@@ -1667,14 +1824,15 @@ hipError_t launch_extract_emit(const ExtractArgs& a, hipStream_t s)
     MTX_DISPATCH_B(launch_emit_b, a, s, nwg)
 }
 
-hipError_t launch_extract_scan(const ExtractArgs& a, ScanState st, hipStream_t s)
+hipError_t launch_extract_scan(const ExtractArgs& a, TileSums ts, hipStream_t s)
 {
     uint64_t ntiles = ((uint64_t)a.max_blocks + 256 * kScanItems - 1) / (256 * kScanItems);
     if (ntiles == 0) ntiles = 1;
     if (ntiles > 0x7FFFFFFFull) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_extract_scan, dim3((uint32_t)ntiles), dim3(256), 0, s, a.blk_counts,
-                       a.count, a.blk_vbase, a.blk_tbase, a.blk_mbase, a.totals, a.out_counts,
-                       st.status, (uint32_t)ntiles, st.ticket);
+    hipLaunchKernelGGL(k_extract_tile_sums, dim3((uint32_t)ntiles), dim3(256), 0, s, a.blk_counts, a.count,
+                       ts.sums, a.totals);
+    hipLaunchKernelGGL(k_extract_scan, dim3((uint32_t)ntiles), dim3(256), 0, s, a.blk_counts, a.count,
+                       a.blk_vbase, a.blk_tbase, a.blk_mbase, a.totals, a.out_counts, ts.sums);
     return hipGetLastError();
 }
 

@@ -12,7 +12,10 @@ __global__ void k_blob_extrema(const mtx_blob* __restrict__ b, int n, uint32_t X
                                uint32_t Z, unsigned long long* ext)
 {
     const uint64_t N = (uint64_t)X * Y * Z;
-    double lo = 1e300, hi = -1e300;
+    // the field is a sum of positive terms, so every value and both start
+    // values are >= 0 and IEEE bit order is numeric order (lanes that see no
+    // voxel must not start the maximum below zero)
+    double lo = 1e300, hi = 0.0;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
          i += (uint64_t)gridDim.x * blockDim.x) {
         uint32_t x = (uint32_t)(i % X), y = (uint32_t)((i / X) % Y), z = (uint32_t)(i / ((uint64_t)X * Y));
@@ -25,7 +28,6 @@ __global__ void k_blob_extrema(const mtx_blob* __restrict__ b, int n, uint32_t X
         lo = a < lo ? a : lo;
         hi = c > hi ? c : hi;
     }
-    // the field is a sum of positive terms: IEEE bit order is numeric order
     if ((threadIdx.x & 63) == 0) {
         atomicMin(&ext[0], (unsigned long long)__double_as_longlong(lo));
         atomicMax(&ext[1], (unsigned long long)__double_as_longlong(hi));
