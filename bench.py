@@ -463,6 +463,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-planes", type=int, default=320,
                     help="z-planes of the volume the CPU baseline processes")
+    ap.add_argument("--skip-plain", action="store_true",
+                    help="profiling aid: leave out the headline workload (value is then null)")
     ap.add_argument("--skip-batched", action="store_true")
     ap.add_argument("--batched-volumes", type=int, default=48)
     ap.add_argument("--batched-steps", type=int, default=10)
@@ -493,6 +495,24 @@ def main():
         return
 
     ctx = Context(0)
+    if args.skip_plain:
+        line = {"metric": METRIC, "value": None, "unit": "Gvoxels/s", "n_gpus": 1, "extra": {}}
+    else:
+        line = headline(args, ctx, stream)
+    if not args.skip_batched:
+        log("batched")
+        line["extra"]["batched"] = bench_batched(args, ctx, stream)
+        log("batched: %s" % json.dumps({k: line["extra"]["batched"][k] for k in
+                                        ("speedup_vs_sequential_preallocated", "speedup_vs_sequential_api")}))
+    if not args.skip_adjoint:
+        log("forward + adjoint")
+        line["extra"]["fwd_adjoint"] = bench_adjoint(args, ctx)
+        log("fwd+adjoint: ratio %.3f" % line["extra"]["fwd_adjoint"]["fwd_plus_adjoint_over_fwd"])
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+def headline(args, ctx, stream):
     log("plain %d^3 f32: %d warmup + %d timed steps" % (args.size, args.warmup, args.steps))
     p = bench_plain(args, ctx, stream)
     ms = p["ms"]
@@ -523,17 +543,7 @@ def main():
         log("cpu baseline: %d z-planes" % args.cpu_planes)
         line["cpu_baseline"] = cpu_baseline(vol.ptr, p["dims"], 0.0, min(args.cpu_planes, args.size))
     vol.free()
-    if not args.skip_batched:
-        log("batched")
-        line["extra"]["batched"] = bench_batched(args, ctx, stream)
-        log("batched: %s" % json.dumps({k: line["extra"]["batched"][k] for k in
-                                        ("speedup_vs_sequential_preallocated", "speedup_vs_sequential_api")}))
-    if not args.skip_adjoint:
-        log("forward + adjoint")
-        line["extra"]["fwd_adjoint"] = bench_adjoint(args, ctx)
-        log("fwd+adjoint: ratio %.3f" % line["extra"]["fwd_adjoint"]["fwd_plus_adjoint_over_fwd"])
-    print(json.dumps(line), flush=True)
-    ctx.close()
+    return line
 
 
 if __name__ == "__main__":

@@ -83,6 +83,8 @@ struct mtx_batch {
     void* out_slab = nullptr;       // vertices, indices, meshlets, edge ids, active ids, ranges
     mtx_batch_output out{};
     hipStream_t cap = nullptr;
+    std::vector<hipStream_t> side;  // capture branches of the classes after the first
+    std::vector<hipEvent_t> events;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -389,6 +391,10 @@ void batch_free(mtx_batch* b)
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->cap) (void)hipStreamDestroy(b->cap);
+    for (hipStream_t st : b->side) (void)hipStreamDestroy(st);
+    for (hipEvent_t ev : b->events) (void)hipEventDestroy(ev);
+    b->side.clear();
+    b->events.clear();
     for (ClassPlan& c : b->classes) c.release();
     if (b->out_slab) (void)hipFree(b->out_slab);
     b->exec = nullptr;
@@ -404,6 +410,40 @@ hipError_t batch_enqueue(mtx_batch* b, hipStream_t s)
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+// Capture: the classes are independent, so each after the first is recorded
+// on a branch forked from the capture stream and joined at the end; the graph
+// then lets them run concurrently.
+hipError_t batch_capture(mtx_batch* b)
+{
+    const size_t nc = b->classes.size();
+    hipError_t e = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking);
+    for (size_t k = 1; e == hipSuccess && k < nc; ++k) {
+        hipStream_t st = nullptr;
+        e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+        if (e == hipSuccess) b->side.push_back(st);
+    }
+    for (size_t k = 0; e == hipSuccess && k < nc; ++k) {
+        hipEvent_t ev = nullptr;
+        e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e == hipSuccess) b->events.push_back(ev);
+    }
+    if (e != hipSuccess) return e;
+    e = hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    hipError_t e2 = hipEventRecord(b->events[0], b->cap);  // fork point
+    for (size_t k = 1; e2 == hipSuccess && k < nc; ++k) {
+        e2 = hipStreamWaitEvent(b->side[k - 1], b->events[0], 0);
+        if (e2 == hipSuccess) e2 = class_enqueue(b->classes[k], b->side[k - 1]);
+        if (e2 == hipSuccess) e2 = hipEventRecord(b->events[k], b->side[k - 1]);
+    }
+    if (e2 == hipSuccess) e2 = class_enqueue(b->classes[0], b->cap);
+    for (size_t k = 1; e2 == hipSuccess && k < nc; ++k) e2 = hipStreamWaitEvent(b->cap, b->events[k], 0);
+    e = hipStreamEndCapture(b->cap, &b->graph);
+    if (e2 != hipSuccess) return e2;
+    if (e != hipSuccess) return e;
+    return hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
 }
 
 }  // namespace
@@ -540,15 +580,7 @@ mtx_status mtx_batch_create(mtx_context* ctx, const mtx_batch_volume* volumes, u
         c.d_ids = b->out.active_block_ids + c.aoff;
     }
     if (o.graph) {
-        if (hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) != hipSuccess)
-            return bail(fail(ctx, MTX_EDEVICE, "capture stream"));
-        hipError_t e = hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal);
-        if (e == hipSuccess) {
-            hipError_t e2 = batch_enqueue(b, b->cap);
-            e = hipStreamEndCapture(b->cap, &b->graph);
-            if (e2 != hipSuccess) e = e2;
-        }
-        if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+        hipError_t e = batch_capture(b);
         if (e != hipSuccess) return bail(fail(ctx, MTX_EDEVICE, "batch graph: %s", hipGetErrorString(e)));
     }
     *out = b;

@@ -1732,50 +1732,51 @@ __global__ __launch_bounds__((64 * TriCfg<B>::WPG)) void k_extract_triangles(Ext
     }
 }
 
+// Persistent grids of the extraction kernels, in workgroups per CU (a
+// multiple of 8 workgroups for the XCD split, never more waves than blocks).
+// Measured on MI355X: the single-volume pass is fastest at 8 per CU (fewer
+// leave CUs idle while long blocks finish, more add per-workgroup set-up); a
+// batch, whose blocks differ far more in work (dense and sparse surfaces of
+// many units), is fastest with finer static shares, 32-128 per CU.
+constexpr uint32_t kWgPerCuSingle = 8;
+constexpr uint32_t kWgPerCuBatch = 64;
+
+template <typename K>
+static hipError_t launch_persistent(K kernel, uint32_t threads, const ExtractArgs& a, hipStream_t s)
+{
+    uint64_t n = (uint64_t)device_cus() * (a.units ? kWgPerCuBatch : kWgPerCuSingle);
+    const uint64_t need = ((uint64_t)a.max_blocks + threads / 64 - 1) / (threads / 64);
+    if (need < n) n = need;
+    n = n < 8 ? 8 : n / 8 * 8;
+    hipLaunchKernelGGL(kernel, dim3((uint32_t)n), dim3(threads), 0, s, a);
+    return hipGetLastError();
+}
+
 template <int B, bool BATCH>
-static hipError_t launch_count_b(const ExtractArgs& a, hipStream_t s, uint32_t nwg)
+static hipError_t launch_count_b(const ExtractArgs& a, hipStream_t s)
 {
     if (B == 8) {
-        if (a.vol.dtype == kF32)
-            hipLaunchKernelGGL((k_extract_count_b8<float, BATCH>), dim3(nwg), dim3(256), 0, s, a);
-        else
-            hipLaunchKernelGGL((k_extract_count_b8<uint8_t, BATCH>), dim3(nwg), dim3(256), 0, s, a);
-        return hipGetLastError();
+        if (a.vol.dtype == kF32) return launch_persistent(k_extract_count_b8<float, BATCH>, 256, a, s);
+        return launch_persistent(k_extract_count_b8<uint8_t, BATCH>, 256, a, s);
     }
-    if (a.vol.dtype == kF32)
-        hipLaunchKernelGGL((k_extract_count<B, float, BATCH>), dim3(nwg), dim3(256), 0, s, a);
-    else
-        hipLaunchKernelGGL((k_extract_count<B, uint8_t, BATCH>), dim3(nwg), dim3(256), 0, s, a);
-    return hipGetLastError();
+    if (a.vol.dtype == kF32) return launch_persistent(k_extract_count<B, float, BATCH>, 256, a, s);
+    return launch_persistent(k_extract_count<B, uint8_t, BATCH>, 256, a, s);
 }
 
 template <int B, bool BATCH>
-static hipError_t launch_emit_b(const ExtractArgs& a, hipStream_t s, uint32_t nwg)
+static hipError_t launch_emit_b(const ExtractArgs& a, hipStream_t s)
 {
     const uint32_t tn = 64 * VertCfg<B, true>::WPG, tf = 64 * VertCfg<B, false>::WPG;
-    const uint32_t gn = nwg * (4 / VertCfg<B, true>::WPG), gf = nwg * (4 / VertCfg<B, false>::WPG);
+    hipError_t e;
     if (a.vol.dtype == kF32) {
-        if (a.normals) hipLaunchKernelGGL((k_extract_vertices<B, float, true, BATCH>), dim3(gn), dim3(tn), 0, s, a);
-        else hipLaunchKernelGGL((k_extract_vertices<B, float, false, BATCH>), dim3(gf), dim3(tf), 0, s, a);
+        e = a.normals ? launch_persistent(k_extract_vertices<B, float, true, BATCH>, tn, a, s)
+                      : launch_persistent(k_extract_vertices<B, float, false, BATCH>, tf, a, s);
     } else {
-        if (a.normals) hipLaunchKernelGGL((k_extract_vertices<B, uint8_t, true, BATCH>), dim3(gn), dim3(tn), 0, s, a);
-        else hipLaunchKernelGGL((k_extract_vertices<B, uint8_t, false, BATCH>), dim3(gf), dim3(tf), 0, s, a);
+        e = a.normals ? launch_persistent(k_extract_vertices<B, uint8_t, true, BATCH>, tn, a, s)
+                      : launch_persistent(k_extract_vertices<B, uint8_t, false, BATCH>, tf, a, s);
     }
-    hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const uint32_t tt = 64 * TriCfg<B>::WPG, gt = nwg * (4 / TriCfg<B>::WPG);
-    hipLaunchKernelGGL((k_extract_triangles<B, BATCH>), dim3(gt), dim3(tt), 0, s, a);
-    return hipGetLastError();
-}
-
-// Persistent grid: enough waves to fill the chip (and a multiple of 8
-// workgroups for the XCD split), never far more than there are blocks.
-static uint32_t persistent_wgs(uint32_t max_blocks)
-{
-    uint32_t n = 2048;
-    const uint32_t need = (max_blocks + 3) / 4;
-    if (need < n) n = need < 8 ? 8 : ((need + 7) / 8) * 8;
-    return n;
+    return launch_persistent(k_extract_triangles<B, BATCH>, 64 * TriCfg<B>::WPG, a, s);
 }
 
 #define MTX_DISPATCH_B(FN, ...)                                   \
@@ -1812,16 +1813,14 @@ bool batch_supported_block(uint32_t B) { return B == 4 || B == 8 || B == 16; }
 
 hipError_t launch_extract_count(const ExtractArgs& a, hipStream_t s)
 {
-    const uint32_t nwg = persistent_wgs(a.max_blocks);
-    if (a.units) { MTX_DISPATCH_BATCH_B(launch_count_b, a, s, nwg) }
-    MTX_DISPATCH_B(launch_count_b, a, s, nwg)
+    if (a.units) { MTX_DISPATCH_BATCH_B(launch_count_b, a, s) }
+    MTX_DISPATCH_B(launch_count_b, a, s)
 }
 
 hipError_t launch_extract_emit(const ExtractArgs& a, hipStream_t s)
 {
-    const uint32_t nwg = persistent_wgs(a.max_blocks);
-    if (a.units) { MTX_DISPATCH_BATCH_B(launch_emit_b, a, s, nwg) }
-    MTX_DISPATCH_B(launch_emit_b, a, s, nwg)
+    if (a.units) { MTX_DISPATCH_BATCH_B(launch_emit_b, a, s) }
+    MTX_DISPATCH_B(launch_emit_b, a, s)
 }
 
 hipError_t launch_extract_scan(const ExtractArgs& a, TileSums ts, hipStream_t s)
