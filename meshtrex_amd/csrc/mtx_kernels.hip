@@ -1358,7 +1358,7 @@ template <int B, bool NORMALS> struct VertCfg {
     static constexpr int A = NORMALS ? 1 : 0;  // apron for central differences
     static constexpr int E = B + 1 + 2 * A;
     static constexpr int NTILE = E * E * E;
-    static constexpr int LIST = 256;            // vertices listed at a time
+    static constexpr int LIST = 512;            // vertices listed at a time
     static constexpr int WAVE_LDS = NTILE * 4 + LIST * 2 + 16;
     static constexpr int WPG = WAVE_LDS * 4 <= 65536 ? 4 : (WAVE_LDS * 2 <= 65536 ? 2 : 1);
 };
@@ -1552,183 +1552,262 @@ __global__ __launch_bounds__((64 * VertCfg<B, NORMALS>::WPG)) void k_extract_ver
 template <int B> struct TriCfg {
     static constexpr int F = B + 1;             // owner corners [0, B]^3
     static constexpr int NF = F * F * F;
-    static constexpr int LIST = 256;            // triangles listed at a time
+    // triangles listed at a time: one pass for all but the densest blocks
+    // (the kernel's registers, not its LDS, bound its occupancy)
+    static constexpr int LIST = B <= 8 ? 1024 : 512;
     static constexpr int WAVE_LDS = NF * 5 + LIST * 4 + 64;
     static constexpr int WPG = WAVE_LDS * 4 + 2048 <= 65536 ? 4 : (WAVE_LDS * 2 + 2048 <= 65536 ? 2 : 1);
+};
+
+// Face corner f of [0, B]^3 that a +x/+y/+z neighbour owns (a coordinate
+// equal to B): x = B (F*F corners), then y = B with x < B (B*F), then z = B
+// with x, y < B (B*B): F^3 - B^3 in all.
+template <int B>
+__device__ __forceinline__ void face_corner(uint32_t f, uint32_t& qx, uint32_t& qy, uint32_t& qz)
+{
+    constexpr uint32_t F = B + 1;
+    if (f < F * F) {
+        qx = B; qy = f % F; qz = f / F;
+    } else if (f < F * F + B * F) {
+        const uint32_t g = f - F * F;
+        qx = g % B; qy = B; qz = g / B;
+    } else {
+        const uint32_t g = f - F * F - B * F;
+        qx = g % B; qy = g / B; qz = B;
+    }
+}
+
+// One block's inputs to the triangle pass.  They are loaded in three stages
+// while the wave works on the previous block: (1) the block's ID, counts,
+// bases and corner entries/cases; (2) the active indices of its +x/+y/+z
+// neighbours (lanes 0..7, neighbour (lane & 1, lane >> 1 & 1, lane >> 2); the
+// (1,1,1) one owns no edge a cell of this block uses); (3) the neighbours'
+// first vertices and the entries of the face corners they own.
+//
+// Bases are kept as their low 32 bits (vertex indices and meshlet offsets are
+// 32-bit; the index base stays 64-bit for the capacity check).  Face entries
+// of 16 bits are packed two to a register; the all-ones entry marks a corner
+// whose neighbour is not active (a real entry is at most (3 B^3) << 3 | 7).
+template <int B>
+struct TriBlock {
+    using Entry = typename EB<B>::Entry;
+    static constexpr bool E16 = sizeof(Entry) == 2;
+    static constexpr int NFACE = (B + 1) * (B + 1) * (B + 1) - B * B * B;
+    static constexpr int NPL = (NFACE + 63) / 64;
+    static constexpr int NFW = E16 ? (NPL + 1) / 2 : NPL;
+    static constexpr uint32_t kNone = E16 ? 0xFFFFu : 0xFFFFFFFFu;
+    uint32_t id, bnv, bnt, unit_v, unit, first;
+    uint32_t vb, mb, uv, ut, nbase;
+    uint64_t tb;
+    int32_t r;
+    LaneCorners<B> lc;
+    uint32_t fentw[NFW];
+    __device__ __forceinline__ void set_face(int k, uint32_t e)
+    {
+        if (E16) fentw[k / 2] = (k & 1) ? (fentw[k / 2] & 0xFFFFu) | (e << 16) : (fentw[k / 2] & 0xFFFF0000u) | e;
+        else fentw[k] = e;
+    }
+    __device__ __forceinline__ uint32_t face(int k) const
+    {
+        return E16 ? (fentw[k / 2] >> (16 * (k & 1))) & 0xFFFFu : fentw[k];
+    }
 };
 
 // Triangle pass: per wave and active block, the global index of the first
 // vertex at every owner corner of [0, B]^3 and its 3 edge flags go to LDS --
 // the block's own corners from its count-pass entries, the corners on the
-// +x/+y/+z faces from the neighbours' entries (one batched gather) -- then the
-// block's triangles are listed (cell, case, k) and resolved three LDS reads
-// each by consecutive lanes.  No voxel is read.
+// +x/+y/+z faces from the neighbours' entries -- then the block's triangles
+// are listed (cell, case, k) and resolved three LDS reads each by consecutive
+// lanes.  No voxel is read.  The next block's inputs are loaded during this
+// one, so no dependent global load waits at a block's start.
 template <int B, bool BATCH>
 __global__ __launch_bounds__((64 * TriCfg<B>::WPG)) void k_extract_triangles(ExtractArgs a)
 {
     using C = TriCfg<B>;
+    using TB = TriBlock<B>;
     constexpr int WPG = C::WPG, F = C::F, NF = C::NF;
     constexpr uint32_t LIST = C::LIST;
     constexpr int NC = EB<B>::NC;
     constexpr int CPL = EB<B>::CPL;
+    constexpr int NFACE = TB::NFACE, NPL = TB::NPL;
     using Entry = typename EB<B>::Entry;
     __shared__ uint32_t s_gidx_all[WPG][NF];
     __shared__ uint8_t s_gfl_all[WPG][NF];
     __shared__ uint32_t s_tl_all[WPG][LIST];
-    __shared__ int32_t s_nbr_all[WPG][8];
-    __shared__ uint32_t s_nbase_all[WPG][8];
     __shared__ uint64_t s_words[256];
+    // face corner f: its index in [0, B]^3 (bits 0..12), the neighbour that
+    // owns it (13..15) and its corner in that neighbour's table (16..27)
+    __shared__ uint32_t s_face[NFACE];
     for (uint32_t c = threadIdx.x; c < 256u; c += blockDim.x) s_words[c] = mtx_tri_words[c];
+    for (uint32_t f = threadIdx.x; f < (uint32_t)NFACE; f += blockDim.x) {
+        uint32_t qx, qy, qz;
+        face_corner<B>(f, qx, qy, qz);
+        const uint32_t sx = qx == (uint32_t)B, sy = qy == (uint32_t)B, sz = qz == (uint32_t)B;
+        s_face[f] = (qx + F * (qy + F * qz)) | ((sx | (sy << 1) | (sz << 2)) << 13) |
+                    (((qx - B * sx) + B * ((qy - B * sy) + B * (qz - B * sz))) << 16);
+    }
     __syncthreads();
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
     uint32_t* s_gidx = s_gidx_all[w];
     uint8_t* s_gfl = s_gfl_all[w];
     uint32_t* s_tl = s_tl_all[w];
-    int32_t* s_nbr = s_nbr_all[w];
-    uint32_t* s_nbase = s_nbase_all[w];
     const uint32_t na = *a.count;
     const Entry* __restrict__ table = static_cast<const Entry*>(a.corner_table);
-    const WaveItems it = wave_items(na, WPG);
-    for (uint32_t i = it.first; i < it.end; i += it.stride) {
-        const uint32_t bnt = a.blk_counts[2 * (uint64_t)i + 1];
-        if (bnt == 0) continue;
-        const uint32_t bnv = a.blk_counts[2 * (uint64_t)i];
-        const uint64_t vb = a.blk_vbase[i], tb = a.blk_tbase[i], mb = a.blk_mbase[i];
-        const uint32_t bnm = (bnt + kMeshletPrims - 1) / kMeshletPrims;
-        if (3 * (tb + bnt) > a.icap || (a.meshlets && mb + bnm > a.mcap)) {
-            if (lane == 0) atomicOr(reinterpret_cast<unsigned long long*>(&a.totals[2]), 1ull);
-            continue;
-        }
-        uint32_t unit;
-        const ItemVol V = item_vol<BATCH>(a, i, unit);
-        // batched: indices and meshlet offsets relative to the unit's first vertex / triangle
-        uint64_t uv = 0, ut = 0;
+
+    // low words of the 64-bit bases
+    const uint32_t* __restrict__ vb32 = reinterpret_cast<const uint32_t*>(a.blk_vbase);
+    const uint32_t* __restrict__ tb32 = reinterpret_cast<const uint32_t*>(a.blk_tbase);
+    const uint32_t* __restrict__ mb32 = reinterpret_cast<const uint32_t*>(a.blk_mbase);
+    auto stage1 = [&](TB& P, uint32_t i) {
+        P.bnv = a.blk_counts[2 * (uint64_t)i];
+        P.bnt = a.blk_counts[2 * (uint64_t)i + 1];
+        P.vb = vb32[2 * (uint64_t)i];
+        P.tb = a.blk_tbase[i];
+        P.mb = mb32[2 * (uint64_t)i];
+        P.id = a.ids[i];
+        if constexpr (BATCH) P.unit_v = a.item_unit[i];
+        P.lc.load(table, a.cases, i, lane, true);
+    };
+    auto stage2 = [&](TB& P) {
+        uint32_t gx, gy, gz;
+        const uint32_t* map;
         if constexpr (BATCH) {
-            const uint32_t f = a.unit_first[unit];
-            uv = a.blk_vbase[f];
-            ut = a.blk_tbase[f];
+            P.unit = __builtin_amdgcn_readfirstlane(P.unit_v);
+            const BatchUnit& U = a.units[P.unit];
+            gx = U.gx; gy = U.gy; gz = U.gz;
+            map = U.map;
+            P.first = a.unit_first[P.unit];
+        } else {
+            gx = a.grid.gx; gy = a.grid.gy; gz = a.grid.gz;
+            map = a.blk_map;
         }
-        const uint32_t id = a.ids[i];
-        const uint32_t bx = id % V.gx, by = (id / V.gx) % V.gy;
-        const uint32_t bz = id / (V.gx * V.gy);
-        LaneCorners<B> lc;
-        lc.load(table, a.cases, i, lane, true);
-        if (lane < 8) {
-            int32_t r = -1;
-            uint32_t base = 0;
-            const uint32_t nx = bx + (lane & 1u), ny = by + ((lane >> 1) & 1u), nz = bz + (lane >> 2);
-            if (lane != 0 && lane != 7 && nx < V.gx && ny < V.gy && nz < V.gz) {
-                r = (int32_t)V.map[nx + V.gx * (ny + (uint64_t)V.gy * nz)];
-                if (r >= 0) base = (uint32_t)(a.blk_vbase[r] - uv);
-            }
-            s_nbr[lane] = r;
-            s_nbase[lane] = base;
+        const uint32_t bx = P.id % gx, by = (P.id / gx) % gy, bz = P.id / (gx * gy);
+        const uint32_t nx = bx + (lane & 1u), ny = by + ((lane >> 1) & 1u), nz = bz + ((lane >> 2) & 1u);
+        P.r = -1;
+        if (lane != 0 && lane < 7 && nx < gx && ny < gy && nz < gz)
+            P.r = (int32_t)map[nx + gx * (ny + (uint64_t)gy * nz)];
+    };
+    auto stage3 = [&](TB& P) {
+        if constexpr (BATCH) {
+            P.uv = vb32[2 * (uint64_t)P.first];
+            P.ut = tb32[2 * (uint64_t)P.first];
+        } else {
+            P.uv = 0;
+            P.ut = 0;
         }
-        // own corners: vertex index = block base + prefix from the count pass
+        P.nbase = (lane < 8 && P.r >= 0) ? vb32[2 * (uint64_t)P.r] : 0u;
+#pragma unroll
+        for (int k = 0; k < TB::NFW; ++k) P.fentw[k] = 0xFFFFFFFFu;
+#pragma unroll
+        for (int k = 0; k < NPL; ++k) {
+            const uint32_t f = lane + 64u * k;
+            const uint32_t fd = s_face[f < (uint32_t)NFACE ? f : 0u];
+            const int32_t jn = __shfl(P.r, (int)((fd >> 13) & 7u), 64);
+            if (f < (uint32_t)NFACE && jn >= 0) P.set_face(k, (uint32_t)table[(uint64_t)jn * NC + (fd >> 16)]);
+        }
+    };
+
+    const WaveItems it = wave_items(na, WPG);
+    TB cur, nxt;
+    uint32_t i = it.first;
+    if (i < it.end) {
+        stage1(cur, i);
+        stage2(cur);
+        stage3(cur);
+    }
+    while (i < it.end) {
+        const uint32_t inx = i + it.stride;
+        const bool more = inx < it.end;
+        if (more) stage1(nxt, inx);
+        const uint32_t bnt = cur.bnt, bnv = cur.bnv;
+        const uint32_t bnm = (bnt + kMeshletPrims - 1) / kMeshletPrims;
+        bool work = bnt != 0;
+        if (work && (3 * (cur.tb + bnt) > a.icap || (a.meshlets && (uint64_t)cur.mb + bnm > a.mcap))) {
+            if (lane == 0) atomicOr(reinterpret_cast<unsigned long long*>(&a.totals[2]), 1ull);
+            work = false;
+        }
         uint32_t nt = 0;
-#pragma unroll
-        for (int j = 0; j < CPL; ++j) {
-            const uint32_t c = lane * CPL + j;
-            if (c < (uint32_t)NC) {
-                const uint32_t q = c % B + F * ((c / B) % B + F * (c / (B * B)));
-                s_gidx[q] = (uint32_t)(vb - uv) + (lc.ent(j) >> 3);
-                s_gfl[q] = (uint8_t)(lc.ent(j) & 7u);
-            }
-            nt += mtx_word_ntri(s_words[lc.cs(j)]);
-        }
-        wave_lds_sync();
-        // neighbour-owned corners (a coordinate equal to B): one batched gather.
-        // Face corner f enumerates x = B (F*F corners), then y = B with x < B
-        // (B*F), then z = B with x, y < B (B*B): F^3 - B^3 in all.
-        {
-            constexpr int NFACE = NF - NC;
-            constexpr int NPL = (NFACE + 63) / 64;
-            Entry fent[NPL];
-            int32_t jn[NPL];
-#pragma unroll
-            for (int k = 0; k < NPL; ++k) {
-                const uint32_t f = lane + 64u * k;
-                uint32_t qx, qy, qz;
-                if (f < (uint32_t)(F * F)) {
-                    qx = B; qy = f % F; qz = f / F;
-                } else if (f < (uint32_t)(F * F + B * F)) {
-                    const uint32_t g = f - F * F;
-                    qx = g % B; qy = B; qz = g / B;
-                } else {
-                    const uint32_t g = f - F * F - B * F;
-                    qx = g % B; qy = g / B; qz = B;
-                }
-                const uint32_t nx = qx == (uint32_t)B, ny = qy == (uint32_t)B, nz = qz == (uint32_t)B;
-                jn[k] = f < (uint32_t)NFACE ? s_nbr[nx | (ny << 1) | (nz << 2)] : -1;
-                fent[k] = 0;
-                if (jn[k] >= 0)
-                    fent[k] = table[(uint64_t)jn[k] * NC + (qx - B * nx) + B * ((qy - B * ny) + B * (qz - B * nz))];
-            }
-#pragma unroll
-            for (int k = 0; k < NPL; ++k) {
-                const uint32_t f = lane + 64u * k;
-                if (f >= (uint32_t)NFACE) continue;
-                uint32_t qx, qy, qz;
-                if (f < (uint32_t)(F * F)) {
-                    qx = B; qy = f % F; qz = f / F;
-                } else if (f < (uint32_t)(F * F + B * F)) {
-                    const uint32_t g = f - F * F;
-                    qx = g % B; qy = B; qz = g / B;
-                } else {
-                    const uint32_t g = f - F * F - B * F;
-                    qx = g % B; qy = g / B; qz = B;
-                }
-                const uint32_t nsel = (qx == (uint32_t)B) | ((qy == (uint32_t)B) << 1) | ((qz == (uint32_t)B) << 2);
-                const uint32_t q = qx + F * (qy + F * qz);
-                s_gidx[q] = jn[k] >= 0 ? s_nbase[nsel] + (uint32_t)(fent[k] >> 3) : 0u;
-                s_gfl[q] = jn[k] >= 0 ? (uint8_t)(fent[k] & 7u) : (uint8_t)0xF0u;
-            }
-        }
-        uint32_t tt;
-        const uint32_t et = wave_excl_scan<uint32_t>(nt, tt);
-        for (uint32_t c0 = 0; c0 < tt; c0 += LIST) {
-            uint32_t run = et;
+        if (work) {
+            // own corners: vertex index = block base + prefix from the count pass
+            const uint32_t vrel = cur.vb - cur.uv;
 #pragma unroll
             for (int j = 0; j < CPL; ++j) {
                 const uint32_t c = lane * CPL + j;
-                const uint32_t cs = lc.cs(j);
-                const uint32_t ntri = mtx_word_ntri(s_words[cs]);
-                for (uint32_t k = 0; k < ntri; ++k, ++run)
-                    if (run - c0 < LIST) s_tl[run - c0] = c | (k << 12) | (cs << 16);
-            }
-            wave_lds_sync();
-            const uint32_t n = min(LIST, tt - c0);
-            for (uint32_t q = lane; q < n; q += 64u) {
-                const uint32_t e = s_tl[q];
-                const uint32_t c = e & 0xFFFu, k = (e >> 12) & 7u;
-                const uint64_t wd = s_words[e >> 16];
-                const uint32_t q0 = c % B + F * ((c / B) % B + F * (c / (B * B)));
-                uint32_t idx3[3];
-#pragma unroll
-                for (uint32_t e3 = 0; e3 < 3; ++e3) {
-                    const uint32_t o = mtx_edge_owner(mtx_word_edge(wd, k, e3));
-                    const uint32_t qq = q0 + (o & 1u) + F * (((o >> 1) & 1u) + F * ((o >> 2) & 1u));
-                    const uint32_t ax = o >> 3;
-                    const uint32_t gf = s_gfl[qq];
-                    if (gf & 0xF0u) {
-                        // an active edge whose owner block is not in the active list:
-                        // the list does not belong to this volume/isovalue
-                        atomicOr(reinterpret_cast<unsigned long long*>(&a.totals[2]), 2ull);
-                    }
-                    idx3[e3] = s_gidx[qq] + __popc(gf & 7u & ((1u << ax) - 1u));
+                if (c < (uint32_t)NC) {
+                    const uint32_t q = c % B + F * ((c / B) % B + F * (c / (B * B)));
+                    s_gidx[q] = vrel + (cur.lc.ent(j) >> 3);
+                    s_gfl[q] = (uint8_t)(cur.lc.ent(j) & 7u);
                 }
-                *reinterpret_cast<uint3*>(a.indices + 3 * (tb + c0 + q)) = make_uint3(idx3[0], idx3[1], idx3[2]);
+                nt += mtx_word_ntri(s_words[cur.lc.cs(j)]);
+            }
+            // neighbour-owned face corners
+#pragma unroll
+            for (int k = 0; k < NPL; ++k) {
+                const uint32_t f = lane + 64u * k;
+                const uint32_t fd = s_face[f < (uint32_t)NFACE ? f : 0u];
+                const uint32_t nb = (uint32_t)__shfl((int)cur.nbase, (int)((fd >> 13) & 7u), 64) - cur.uv;
+                if (f < (uint32_t)NFACE) {
+                    const uint32_t q = fd & 0x1FFFu;
+                    const uint32_t e = cur.face(k);
+                    s_gidx[q] = e != TB::kNone ? nb + (e >> 3) : 0u;
+                    s_gfl[q] = e != TB::kNone ? (uint8_t)(e & 7u) : (uint8_t)0xF0u;
+                }
+            }
+        }
+        if (more) stage2(nxt);
+        if (work) {
+            wave_lds_sync();
+            const uint64_t tb = cur.tb;
+            uint32_t tt;
+            const uint32_t et = wave_excl_scan<uint32_t>(nt, tt);
+            for (uint32_t c0 = 0; c0 < tt; c0 += LIST) {
+                uint32_t run = et;
+#pragma unroll
+                for (int j = 0; j < CPL; ++j) {
+                    const uint32_t c = lane * CPL + j;
+                    const uint32_t cs = cur.lc.cs(j);
+                    const uint32_t ntri = mtx_word_ntri(s_words[cs]);
+                    for (uint32_t k = 0; k < ntri; ++k, ++run)
+                        if (run - c0 < LIST) s_tl[run - c0] = c | (k << 12) | (cs << 16);
+                }
+                wave_lds_sync();
+                const uint32_t n = min(LIST, tt - c0);
+                for (uint32_t q = lane; q < n; q += 64u) {
+                    const uint32_t e = s_tl[q];
+                    const uint32_t c = e & 0xFFFu, k = (e >> 12) & 7u;
+                    const uint64_t wd = s_words[e >> 16];
+                    const uint32_t q0 = c % B + F * ((c / B) % B + F * (c / (B * B)));
+                    uint32_t idx3[3];
+#pragma unroll
+                    for (uint32_t e3 = 0; e3 < 3; ++e3) {
+                        const uint32_t o = mtx_edge_owner(mtx_word_edge(wd, k, e3));
+                        const uint32_t qq = q0 + (o & 1u) + F * (((o >> 1) & 1u) + F * ((o >> 2) & 1u));
+                        const uint32_t ax = o >> 3;
+                        const uint32_t gf = s_gfl[qq];
+                        if (gf & 0xF0u) {
+                            // an active edge whose owner block is not in the active list:
+                            // the list does not belong to this volume/isovalue
+                            atomicOr(reinterpret_cast<unsigned long long*>(&a.totals[2]), 2ull);
+                        }
+                        idx3[e3] = s_gidx[qq] + __popc(gf & 7u & ((1u << ax) - 1u));
+                    }
+                    *reinterpret_cast<uint3*>(a.indices + 3 * (tb + c0 + q)) = make_uint3(idx3[0], idx3[1], idx3[2]);
+                }
+                wave_lds_sync();
+            }
+            if (a.meshlets) {
+                for (uint32_t k = lane; k < bnm; k += 64u) {
+                    const uint32_t prims = min(kMeshletPrims, bnt - k * kMeshletPrims);
+                    a.meshlets[(uint64_t)cur.mb + k] = make_uint4(cur.vb - cur.uv,
+                                                                  3u * ((uint32_t)tb - cur.ut + k * kMeshletPrims),
+                                                                  bnv, prims);
+                }
             }
             wave_lds_sync();
         }
-        if (a.meshlets) {
-            for (uint32_t k = lane; k < bnm; k += 64u) {
-                const uint32_t prims = min(kMeshletPrims, bnt - k * kMeshletPrims);
-                a.meshlets[mb + k] = make_uint4((uint32_t)(vb - uv),
-                                                (uint32_t)(3 * (tb - ut + (uint64_t)k * kMeshletPrims)), bnv, prims);
-            }
-        }
-        wave_lds_sync();
+        if (more) stage3(nxt);
+        cur = nxt;
+        i = inx;
     }
 }
 
