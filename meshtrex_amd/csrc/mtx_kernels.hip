@@ -1341,25 +1341,29 @@ __global__ __launch_bounds__(256) void k_extract_scan(const uint32_t* __restrict
     }
 }
 
-__device__ __forceinline__ void grad_normal(const float* s, uint32_t t, uint32_t E, float n[3])
+template <typename LT>
+__device__ __forceinline__ void grad_normal(const LT* s, uint32_t t, uint32_t E, float n[3])
 {
     // normalize(s(p-x)-s(p+x), s(p-y)-s(p+y), s(p-z)-s(p+z)); (0,1,0) where the
     // gradient vanishes (|g| < 1e-5)
-    const float gx = s[t - 1] - s[t + 1];
-    const float gy = s[t - E] - s[t + E];
-    const float gz = s[t - E * E] - s[t + E * E];
+    const float gx = vfloat(s[t - 1]) - vfloat(s[t + 1]);
+    const float gy = vfloat(s[t - E]) - vfloat(s[t + E]);
+    const float gz = vfloat(s[t - E * E]) - vfloat(s[t + E * E]);
     const float l2 = gx * gx + gy * gy + gz * gz;
     if (l2 < 1e-10f) { n[0] = 0.f; n[1] = 1.f; n[2] = 0.f; return; }
     const float r = rsqrtf(l2);
     n[0] = gx * r; n[1] = gy * r; n[2] = gz * r;
 }
 
-template <int B, bool NORMALS> struct VertCfg {
+// The tile is kept in LDS in the volume's own type (u8 voxels as bytes: a
+// quarter of the LDS of f32, so LDS does not bound the kernel's occupancy).
+template <int B, bool NORMALS, typename T> struct VertCfg {
     static constexpr int A = NORMALS ? 1 : 0;  // apron for central differences
     static constexpr int E = B + 1 + 2 * A;
     static constexpr int NTILE = E * E * E;
+    static constexpr int NTILE_PAD = (NTILE + 3) / 4 * 4;
     static constexpr int LIST = 512;            // vertices listed at a time
-    static constexpr int WAVE_LDS = NTILE * 4 + LIST * 2 + 16;
+    static constexpr int WAVE_LDS = NTILE_PAD * (int)sizeof(T) + LIST * 2 + 16;
     static constexpr int WPG = WAVE_LDS * 4 <= 65536 ? 4 : (WAVE_LDS * 2 <= 65536 ? 2 : 1);
 };
 
@@ -1413,17 +1417,17 @@ struct LaneCorners {
 // the block's vertices (corner, axis) in prefix order and let the 64 lanes
 // take consecutive entries: balanced work and coalesced writes.
 template <int B, typename T, bool NORMALS, bool BATCH>
-__global__ __launch_bounds__((64 * VertCfg<B, NORMALS>::WPG)) void k_extract_vertices(ExtractArgs a)
+__global__ __launch_bounds__((64 * VertCfg<B, NORMALS, T>::WPG)) void k_extract_vertices(ExtractArgs a)
 {
-    using C = VertCfg<B, NORMALS>;
+    using C = VertCfg<B, NORMALS, T>;
     constexpr int A = C::A, E = C::E, NTILE = C::NTILE, WPG = C::WPG;
     constexpr uint32_t LIST = C::LIST;
     constexpr int CPL = EB<B>::CPL;
     using Entry = typename EB<B>::Entry;
-    __shared__ float s_val_all[WPG][NTILE];
+    __shared__ T s_val_all[WPG][C::NTILE_PAD];
     __shared__ uint16_t s_vl_all[WPG][LIST];
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
-    float* s_val = s_val_all[w];
+    T* s_val = s_val_all[w];
     uint16_t* s_vl = s_vl_all[w];
     const uint32_t na = *a.count;
     const Entry* __restrict__ table = static_cast<const Entry*>(a.corner_table);
@@ -1466,7 +1470,7 @@ __global__ __launch_bounds__((64 * VertCfg<B, NORMALS>::WPG)) void k_extract_ver
 #pragma unroll
                 for (int k = 0; k < NLD; ++k) {
                     const uint32_t j = lane + 64u * (k0 + k);
-                    if (j < (uint32_t)NTILE) s_val[j] = vfloat(v[k]);
+                    if (j < (uint32_t)NTILE) s_val[j] = v[k];
                 }
             }
         } else {
@@ -1486,7 +1490,7 @@ __global__ __launch_bounds__((64 * VertCfg<B, NORMALS>::WPG)) void k_extract_ver
 #pragma unroll
                 for (int k = 0; k < NLD; ++k) {
                     const uint32_t j = lane + 64u * (k0 + k);
-                    if (j < (uint32_t)NTILE) s_val[j] = vfloat(v[k]);
+                    if (j < (uint32_t)NTILE) s_val[j] = v[k];
                 }
             }
         }
@@ -1512,7 +1516,7 @@ __global__ __launch_bounds__((64 * VertCfg<B, NORMALS>::WPG)) void k_extract_ver
                 const uint32_t x = ox + lx, y = oy + ly, z = oz + lz;
                 const uint32_t t0 = (lx + A) + E * ((ly + A) + E * (lz + A));
                 const uint32_t st = ax == 0 ? 1u : (ax == 1 ? (uint32_t)E : (uint32_t)(E * E));
-                const float v0 = s_val[t0], v1 = s_val[t0 + st];
+                const float v0 = vfloat(s_val[t0]), v1 = vfloat(s_val[t0 + st]);
                 float t = (iso - v0) / (v1 - v0);
                 t = fminf(fmaxf(t, 0.0f), 1.0f);
                 const float px = (float)x + (ax == 0 ? t : 0.0f);
@@ -1845,14 +1849,13 @@ static hipError_t launch_count_b(const ExtractArgs& a, hipStream_t s)
 template <int B, bool BATCH>
 static hipError_t launch_emit_b(const ExtractArgs& a, hipStream_t s)
 {
-    const uint32_t tn = 64 * VertCfg<B, true>::WPG, tf = 64 * VertCfg<B, false>::WPG;
     hipError_t e;
     if (a.vol.dtype == kF32) {
-        e = a.normals ? launch_persistent(k_extract_vertices<B, float, true, BATCH>, tn, a, s)
-                      : launch_persistent(k_extract_vertices<B, float, false, BATCH>, tf, a, s);
+        e = a.normals ? launch_persistent(k_extract_vertices<B, float, true, BATCH>, 64 * VertCfg<B, true, float>::WPG, a, s)
+                      : launch_persistent(k_extract_vertices<B, float, false, BATCH>, 64 * VertCfg<B, false, float>::WPG, a, s);
     } else {
-        e = a.normals ? launch_persistent(k_extract_vertices<B, uint8_t, true, BATCH>, tn, a, s)
-                      : launch_persistent(k_extract_vertices<B, uint8_t, false, BATCH>, tf, a, s);
+        e = a.normals ? launch_persistent(k_extract_vertices<B, uint8_t, true, BATCH>, 64 * VertCfg<B, true, uint8_t>::WPG, a, s)
+                      : launch_persistent(k_extract_vertices<B, uint8_t, false, BATCH>, 64 * VertCfg<B, false, uint8_t>::WPG, a, s);
     }
     if (e != hipSuccess) return e;
     return launch_persistent(k_extract_triangles<B, BATCH>, 64 * TriCfg<B>::WPG, a, s);
