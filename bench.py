@@ -190,6 +190,54 @@ def bench_plain(args, ctx, stream):
     return res
 
 
+def bench_ingest(args, ctx, vol_ptr, dims, minmax_ms):
+    """Host volume -> min/max: slabs uploaded on a copy stream, each slab's
+    min/max run as soon as it is resident; pageable and pinned host memory,
+    beside a bare pinned copy and the device-resident min/max."""
+    import numpy as np
+    from meshtrex_amd import _lib as L
+    from meshtrex_amd.managers import (MinMaxOutput, Volume, _check, makeOptions, makePushConstants,
+                                       pinned_array, read_device)
+    nvox = dims[0] * dims[1] * dims[2]
+    nbytes = 4 * nvox
+    host = read_device(vol_ptr, np.float32, nvox)
+    pin = pinned_array(nvox, np.float32)
+    pin[:] = host
+    lib, h = L.lib(), ctx.handle
+    pc = makePushConstants(dims, 8, 0.0)
+    o = makeOptions(build_pyramid=0, reuse=1)
+    res = {"volume_bytes": nbytes}
+    for name, arr in (("pageable", host), ("pinned", pin)):
+        v = Volume(dims, "float32", arr)._c()
+        mm = MinMaxOutput(ctx)
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            _check(ctx, lib.mtx_compute_min_max(h, ctypes.byref(v), ctypes.byref(pc), ctypes.byref(o), None,
+                                                ctypes.byref(mm.c)))
+            times.append(time.perf_counter() - t0)
+        mm.cleanup()
+        best = min(times[1:])   # the first call also allocates the device copy
+        res[name] = {"ms": round(best * 1e3, 2), "GBps": round(nbytes / best / 1e9, 2)}
+    from meshtrex_amd.managers import DeviceBuffer
+    d = DeviceBuffer(nbytes)
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        assert lib.mtx_memcpy_h2d(d.ptr, pin.ctypes.data, nbytes) == 0
+        times.append(time.perf_counter() - t0)
+    d.free()
+    copy_s = min(times)
+    res["pinned_copy_only"] = {"ms": round(copy_s * 1e3, 2), "GBps": round(nbytes / copy_s / 1e9, 2)}
+    res["device_resident_minmax_ms"] = round(minmax_ms, 3)
+    serial = copy_s * 1e3 + minmax_ms
+    res["pinned_overlap"] = {"serial_copy_then_minmax_ms": round(serial, 2),
+                             "slabbed_ms": res["pinned"]["ms"],
+                             "hidden_fraction_of_minmax": round(max(0.0, min(1.0, (serial - res["pinned"]["ms"])
+                                                                             / minmax_ms)), 3)}
+    return res
+
+
 def bench_batched(args, ctx, stream):
     import numpy as np
     from meshtrex_amd import _lib as L
@@ -471,6 +519,7 @@ def main():
     ap.add_argument("--skip-plain", action="store_true",
                     help="profiling aid: leave out the headline workload (value is then null)")
     ap.add_argument("--skip-batched", action="store_true")
+    ap.add_argument("--skip-ingest", action="store_true")
     ap.add_argument("--batched-volumes", type=int, default=48)
     ap.add_argument("--batched-steps", type=int, default=10)
     ap.add_argument("--batch-block", type=int, default=0, help="0: autotuned")
@@ -547,6 +596,9 @@ def headline(args, ctx, stream):
     if not args.no_cpu_baseline:
         log("cpu baseline: %d z-planes" % args.cpu_planes)
         line["cpu_baseline"] = cpu_baseline(vol.ptr, p["dims"], 0.0, min(args.cpu_planes, args.size))
+    if not args.skip_ingest:
+        log("ingest from host memory")
+        line["extra"]["ingest"] = bench_ingest(args, ctx, vol.ptr, p["dims"], p["extra"]["stage_ms"]["minmax_leaf"])
     vol.free()
     return line
 

@@ -363,6 +363,13 @@ mtx_status mtx_write_obj(mtx_context* ctx, mtx_extraction_output* ext, const cha
 
 mtx_status mtx_device_malloc(void** ptr, uint64_t bytes);
 mtx_status mtx_device_free(void* ptr);
+/* Pinned (page-locked) host memory.  A volume read into it uploads by
+ * asynchronous DMA: mtx_compute_min_max copies a host volume in slabs of whole
+ * block planes on a copy stream and runs the min/max of each slab as soon as
+ * it is resident (volume ingest, src/volume.h:14-42 and
+ * src/minMaxManager.cpp:18-58,115-125). */
+mtx_status mtx_host_malloc(void** ptr, uint64_t bytes);
+mtx_status mtx_host_free(void* ptr);
 mtx_status mtx_memcpy_h2d(void* dst, const void* src, uint64_t bytes);
 mtx_status mtx_memcpy_d2h(void* dst, const void* src, uint64_t bytes);
 mtx_status mtx_memcpy_d2d(void* dst, const void* src, uint64_t bytes);

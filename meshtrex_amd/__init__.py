@@ -10,5 +10,6 @@ from .managers import (  # noqa: F401
     Context, DeviceBuffer, Volume, MinMaxOutput, FilteringOutput, ExtractionOutput, MtxError,
     computeMinMaxMip, filterActiveBlocks, readActiveBlockCount, extractMeshletDescriptors,
     loadVolume, makePushConstants, makeOptions, read_device, MtxProfiler, writeGPUExtractionToOBJ,
+    PinnedBuffer, pinned_array,
 )
 from .batch import Batch  # noqa: F401,E402

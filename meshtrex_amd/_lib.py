@@ -117,6 +117,8 @@ _SIGS = {
     "mtx_extraction_release": (None, [vp, ctypes.POINTER(CExtractionOutput)]),
     "mtx_device_malloc": (ctypes.c_int, [ctypes.POINTER(vp), u64]),
     "mtx_device_free": (ctypes.c_int, [vp]),
+    "mtx_host_malloc": (ctypes.c_int, [ctypes.POINTER(vp), u64]),
+    "mtx_host_free": (ctypes.c_int, [vp]),
     "mtx_memcpy_h2d": (ctypes.c_int, [vp, vp, u64]),
     "mtx_memcpy_d2h": (ctypes.c_int, [vp, vp, u64]),
     "mtx_memcpy_d2d": (ctypes.c_int, [vp, vp, u64]),

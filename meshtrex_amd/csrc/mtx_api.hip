@@ -24,6 +24,53 @@ mtx_options defaults()
     return o;
 }
 
+// Bytes per uploaded slab of a host volume: large enough for full-rate DMA,
+// small enough that min/max of the first slabs runs while the rest uploads.
+constexpr uint64_t kSlabBytes = 256ull << 20;
+
+// Host volume -> device in slabs of whole block planes on the context's copy
+// stream; after each slab (and the one plane it shares with the next) is
+// resident, the compute stream runs the B = 8 min/max over its block planes.
+// Pinned host memory makes the copies asynchronous DMA; pageable memory is
+// staged by the runtime.
+hipError_t upload_and_minmax(mtx_context* ctx, hipStream_t s, const void* host, void* dev, const VolDesc& v,
+                             const Grid& g, uint64_t esz, uint32_t* pairs, uint32_t* mask, const ActParams* act,
+                             uint32_t rows)
+{
+    const uint64_t plane = (uint64_t)v.X * v.Y * esz;
+    uint64_t slab_planes = kSlabBytes / (plane ? plane : 1);
+    uint32_t slab_blocks = (uint32_t)(slab_planes / 8 ? slab_planes / 8 : 1);
+    if (slab_blocks > g.gz) slab_blocks = g.gz;
+    const uint32_t nslab = (g.gz + slab_blocks - 1) / slab_blocks;
+    hipError_t e = hipSuccess;
+    if (!ctx->copy) e = hipStreamCreateWithFlags(&ctx->copy, hipStreamNonBlocking);
+    while (e == hipSuccess && ctx->slab_events.size() < nslab) {
+        hipEvent_t ev = nullptr;
+        e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e == hipSuccess) ctx->slab_events.push_back(ev);
+    }
+    if (e != hipSuccess) return e;
+    // the copy stream starts after earlier work on `s` (the buffer may be in use)
+    e = hipEventRecord(ctx->slab_events[0], s);
+    if (e == hipSuccess) e = hipStreamWaitEvent(ctx->copy, ctx->slab_events[0], 0);
+    uint64_t copied = 0;  // planes resident
+    for (uint32_t k = 0; e == hipSuccess && k < nslab; ++k) {
+        const uint32_t bzb = k * slab_blocks;
+        const uint32_t bze = bzb + slab_blocks < g.gz ? bzb + slab_blocks : g.gz;
+        uint64_t need = (uint64_t)bze * 8 + 1;  // planes [0, need) for these blocks
+        if (need > v.Z) need = v.Z;
+        if (need > copied) {
+            e = hipMemcpyAsync(static_cast<char*>(dev) + copied * plane, static_cast<const char*>(host) + copied * plane,
+                               (need - copied) * plane, hipMemcpyHostToDevice, ctx->copy);
+            copied = need;
+        }
+        if (e == hipSuccess) e = hipEventRecord(ctx->slab_events[k], ctx->copy);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s, ctx->slab_events[k], 0);
+        if (e == hipSuccess) e = launch_minmax_b8(v, g, bzb, bze, pairs, mask, act, rows, s);
+    }
+    return e;
+}
+
 // Pyramid layout, src/minMaxManager.cpp:85-92,147-151.
 uint32_t pyramid_layout(const uint32_t G[3], uint32_t (*ext)[3], uint64_t* off)
 {
@@ -121,6 +168,11 @@ void mtx_context_destroy(mtx_context* ctx)
         hipStreamSynchronize(ctx->own);
         hipStreamDestroy(ctx->own);
     }
+    if (ctx->copy) {
+        hipStreamSynchronize(ctx->copy);
+        hipStreamDestroy(ctx->copy);
+    }
+    for (hipEvent_t e : ctx->slab_events) hipEventDestroy(e);
     delete ctx;
 }
 
@@ -210,20 +262,21 @@ mtx_status mtx_compute_min_max(mtx_context* ctx, const mtx_volume* volume,
     out->overlap = o.overlap;
     out->mask_valid = 0;
     out->levels_built = 1;
-    if (volume->on_device) {
-        out->volume = volume->data;
-    } else {
-        out->volume = out->owned_volume;
-        hipError_t e = hipMemcpyAsync(out->owned_volume, volume->data, vbytes, hipMemcpyHostToDevice, bd.s);
-        if (e != hipSuccess) return fail(ctx, MTX_EDEVICE, "volume upload: %s", hipGetErrorString(e));
-    }
+    out->volume = volume->on_device ? volume->data : out->owned_volume;
     VolDesc v{ out->volume, X, Y, Z, (int)volume->dtype };
     ActParams act = make_act((int)volume->dtype, pc->isovalue, (int)o.rule);
+    uint32_t* mask = fuse ? out->active_mask : nullptr;
     hipError_t e;
-    if (fast)
-        e = launch_minmax_b8(v, g, out->minmax, fuse ? out->active_mask : nullptr, &act, o.minmax_rows, bd.s);
-    else
-        e = launch_minmax_generic(v, g, (int)o.overlap, out->minmax, bd.s);
+    if (volume->on_device) {
+        e = fast ? launch_minmax_b8(v, g, 0, g.gz, out->minmax, mask, &act, o.minmax_rows, bd.s)
+                 : launch_minmax_generic(v, g, (int)o.overlap, out->minmax, bd.s);
+    } else if (fast) {
+        e = upload_and_minmax(ctx, bd.s, volume->data, out->owned_volume, v, g, esz, out->minmax, mask, &act,
+                              o.minmax_rows);
+    } else {
+        e = hipMemcpyAsync(out->owned_volume, volume->data, vbytes, hipMemcpyHostToDevice, bd.s);
+        if (e == hipSuccess) e = launch_minmax_generic(v, g, (int)o.overlap, out->minmax, bd.s);
+    }
     if (e == hipSuccess && o.build_pyramid && out->num_levels > 1) {
         e = launch_pyramid(out->minmax, out->num_levels, out->level_extent, out->level_offset, bd.s);
         out->levels_built = out->num_levels;
@@ -559,6 +612,18 @@ mtx_status mtx_device_malloc(void** ptr, uint64_t bytes)
     *ptr = nullptr;
     hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);
     return e == hipSuccess ? MTX_OK : (e == hipErrorOutOfMemory ? MTX_ENOMEM : MTX_EDEVICE);
+}
+
+mtx_status mtx_host_malloc(void** ptr, uint64_t bytes)
+{
+    if (!ptr) return MTX_EINVAL;
+    *ptr = nullptr;
+    return hipHostMalloc(ptr, bytes ? bytes : 1, hipHostMallocDefault) == hipSuccess ? MTX_OK : MTX_ENOMEM;
+}
+
+mtx_status mtx_host_free(void* ptr)
+{
+    return hipHostFree(ptr) == hipSuccess ? MTX_OK : MTX_EINVAL;
 }
 
 mtx_status mtx_device_free(void* ptr)

@@ -10,10 +10,13 @@
 #include <cstdio>
 #include <mutex>
 #include <string>
+#include <vector>
 
 struct mtx_context {
     int device = 0;
     hipStream_t own = nullptr;
+    hipStream_t copy = nullptr;           // host-to-device uploads (created on first use)
+    std::vector<hipEvent_t> slab_events;  // one per uploaded slab in flight
     std::string last_error;
     std::mutex mu;
 };

@@ -42,8 +42,10 @@ struct TileSums {
 // Block min/max, B=8, overlap 1, coalesced wave64 kernel.  Optionally writes
 // the activity bit mask (one 32-bit word per 32 x-blocks) for `act`.
 // rows: y-blocks per wavefront (1, 2, 4) or 0 to choose from the shape.
-hipError_t launch_minmax_b8(const VolDesc& v, const Grid& g, uint32_t* pairs, uint32_t* mask,
-                            const ActParams* act, uint32_t rows, hipStream_t s);
+// Covers z-blocks [bzb, bze); their voxel planes [8 bzb, min(8 bze, Z - 1)]
+// must be resident.
+hipError_t launch_minmax_b8(const VolDesc& v, const Grid& g, uint32_t bzb, uint32_t bze, uint32_t* pairs,
+                            uint32_t* mask, const ActParams* act, uint32_t rows, hipStream_t s);
 
 // Block min/max for any B, overlap 0 or 1 (one wave per block).
 hipError_t launch_minmax_generic(const VolDesc& v, const Grid& g, int overlap, uint32_t* pairs,

@@ -125,9 +125,10 @@ __device__ __forceinline__ void mm_plane(MMAcc (&out)[M], bool merge, const T* _
 template <typename T, bool ALIGNED, bool MASK, int M>
 __global__ __launch_bounds__(256) void k_minmax_b8(
     const T* __restrict__ vol, uint32_t X, uint32_t Y, uint32_t Z, uint32_t gx, uint32_t gy,
-    uint32_t gz, uint32_t ntx, uint32_t wx, uint32_t ntxg, uint32_t nygrp, uint32_t zc,
+    uint32_t bzb, uint32_t bze, uint32_t ntx, uint32_t wx, uint32_t ntxg, uint32_t nygrp, uint32_t zc,
     uint2* __restrict__ pairs, uint32_t* __restrict__ mask, uint32_t wpr, ActParams act)
 {
+    // z-blocks [bzb, bze) of the volume (all of it, or one uploaded slab)
     // tile order: x groups fastest, then y groups, then z chunks; XCD-contiguous
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const uint32_t txg = tile % ntxg;
@@ -140,9 +141,9 @@ __global__ __launch_bounds__(256) void k_minmax_b8(
     const uint32_t tx = txg * wx + w % wx;
     const uint32_t by0 = (ygrp * wy + w / wx) * (uint32_t)M;
     if (tx >= ntx || by0 >= gy) return;
-    const uint32_t bz0 = cz * zc;
-    if (bz0 >= gz) return;
-    const uint32_t bz1 = min(bz0 + zc, gz);
+    const uint32_t bz0 = bzb + cz * zc;
+    if (bz0 >= bze) return;
+    const uint32_t bz1 = min(bz0 + zc, bze);
 
     const uint32_t x0 = tx * 256u + lane * 4u;
     const bool lane_ok = x0 < X;
@@ -246,18 +247,18 @@ static MMPlan plan_minmax(const Grid& g, uint32_t ntx, const int (&resident)[3],
 }
 
 template <typename T, bool ALIGNED, bool MASK, int M>
-static hipError_t launch_mm_m(const VolDesc& v, const Grid& g, const MMPlan& p, uint32_t ntx,
-                              uint32_t* pairs, uint32_t* mask, const ActParams& a, hipStream_t s)
+static hipError_t launch_mm_m(const VolDesc& v, const Grid& g, uint32_t bzb, uint32_t bze, const MMPlan& p,
+                              uint32_t ntx, uint32_t* pairs, uint32_t* mask, const ActParams& a, hipStream_t s)
 {
     hipLaunchKernelGGL((k_minmax_b8<T, ALIGNED, MASK, M>), dim3((uint32_t)p.nwg), dim3(256), 0, s,
-                       static_cast<const T*>(v.data), v.X, v.Y, v.Z, g.gx, g.gy, g.gz, ntx, p.wx,
+                       static_cast<const T*>(v.data), v.X, v.Y, v.Z, g.gx, g.gy, bzb, bze, ntx, p.wx,
                        p.ntxg, p.nygrp, p.zc, reinterpret_cast<uint2*>(pairs), mask, (g.gx + 31u) / 32u, a);
     return hipGetLastError();
 }
 
 template <typename T, bool MASK>
-static hipError_t launch_mm_b8_t(const VolDesc& v, const Grid& g, uint32_t* pairs, uint32_t* mask,
-                                 const ActParams* act, uint32_t rows, hipStream_t s)
+static hipError_t launch_mm_b8_t(const VolDesc& v, const Grid& g, uint32_t bzb, uint32_t bze, uint32_t* pairs,
+                                 uint32_t* mask, const ActParams* act, uint32_t rows, hipStream_t s)
 {
     const uint32_t ntx = (g.gx + 31u) / 32u;
     static int resident[3] = { 0, 0, 0 };
@@ -268,15 +269,17 @@ static hipError_t launch_mm_b8_t(const VolDesc& v, const Grid& g, uint32_t* pair
         for (int& r : resident)
             if (r < 1) r = 1;
     }
-    const MMPlan p = plan_minmax(g, ntx, resident, rows);
+    Grid gr = g;
+    gr.gz = bze - bzb;  // the planned range
+    const MMPlan p = plan_minmax(gr, ntx, resident, rows);
     if (p.nwg == 0 || p.nwg > 0x7FFFFFFFull) return hipErrorInvalidValue;
     const bool aligned = (v.X % 4u) == 0 && (((uintptr_t)v.data) % (4 * sizeof(T))) == 0;
     ActParams a{};
     if (act) a = *act;
 #define MTX_MM_CASE(MM)                                                                            \
     if (p.M == MM)                                                                                 \
-        return aligned ? launch_mm_m<T, true, MASK, MM>(v, g, p, ntx, pairs, mask, a, s)           \
-                       : launch_mm_m<T, false, MASK, MM>(v, g, p, ntx, pairs, mask, a, s);
+        return aligned ? launch_mm_m<T, true, MASK, MM>(v, g, bzb, bze, p, ntx, pairs, mask, a, s) \
+                       : launch_mm_m<T, false, MASK, MM>(v, g, bzb, bze, p, ntx, pairs, mask, a, s);
     MTX_MM_CASE(1)
     MTX_MM_CASE(2)
     MTX_MM_CASE(4)
@@ -284,16 +287,17 @@ static hipError_t launch_mm_b8_t(const VolDesc& v, const Grid& g, uint32_t* pair
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_minmax_b8(const VolDesc& v, const Grid& g, uint32_t* pairs, uint32_t* mask,
-                            const ActParams* act, uint32_t rows, hipStream_t s)
+hipError_t launch_minmax_b8(const VolDesc& v, const Grid& g, uint32_t bzb, uint32_t bze, uint32_t* pairs,
+                            uint32_t* mask, const ActParams* act, uint32_t rows, hipStream_t s)
 {
     if (rows != 0 && rows != 1 && rows != 2 && rows != 4) return hipErrorInvalidValue;
+    if (bzb >= bze || bze > g.gz) return hipErrorInvalidValue;
     const bool m = mask != nullptr;
     if (v.dtype == kF32)
-        return m ? launch_mm_b8_t<float, true>(v, g, pairs, mask, act, rows, s)
-                 : launch_mm_b8_t<float, false>(v, g, pairs, mask, act, rows, s);
-    return m ? launch_mm_b8_t<uint8_t, true>(v, g, pairs, mask, act, rows, s)
-             : launch_mm_b8_t<uint8_t, false>(v, g, pairs, mask, act, rows, s);
+        return m ? launch_mm_b8_t<float, true>(v, g, bzb, bze, pairs, mask, act, rows, s)
+                 : launch_mm_b8_t<float, false>(v, g, bzb, bze, pairs, mask, act, rows, s);
+    return m ? launch_mm_b8_t<uint8_t, true>(v, g, bzb, bze, pairs, mask, act, rows, s)
+             : launch_mm_b8_t<uint8_t, false>(v, g, bzb, bze, pairs, mask, act, rows, s);
 }
 
 // ---------------------------------------------------------------------------

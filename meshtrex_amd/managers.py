@@ -147,12 +147,57 @@ class Volume:
 _NAME_RE = re.compile(r"(\w+)_(\d+)x(\d+)x(\d+)_(.+)\.raw")
 
 
-def loadVolume(path):
+class PinnedBuffer:
+    """Page-locked host memory (mtx_host_malloc) with a numpy view: a volume in
+    it uploads by asynchronous DMA, overlapped with the min/max of the slabs
+    already resident."""
+
+    def __init__(self, nbytes):
+        p = ctypes.c_void_p()
+        st = L.lib().mtx_host_malloc(ctypes.byref(p), nbytes)
+        if st != L.MTX_OK:
+            raise MtxError(st, "pinned host allocation of %d bytes" % nbytes)
+        self.ptr = p.value
+        self.nbytes = nbytes
+
+    def view(self, dtype, count):
+        buf = (ctypes.c_char * self.nbytes).from_address(self.ptr)
+        return np.frombuffer(buf, dtype=dtype, count=count)
+
+    def free(self):
+        if self.ptr:
+            L.lib().mtx_host_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def pinned_array(shape_or_n, dtype):
+    """A numpy array in pinned host memory (the buffer lives as long as the array)."""
+    n = int(np.prod(shape_or_n))
+    dtype = np.dtype(dtype)
+    buf = PinnedBuffer(n * dtype.itemsize)
+    a = buf.view(dtype, n)
+    # keep the allocation alive with the array
+    holder = np.ndarray.__new__(_PinnedArray, a.shape, dtype, buffer=a)
+    holder._pinned = buf
+    return holder
+
+
+class _PinnedArray(np.ndarray):
+    pass
+
+
+def loadVolume(path, pinned=False):
     """Raw-file loader with the reference's naming contract (src/volume.h:14-42):
     '<name>_<X>x<Y>x<Z>_<type>.raw'.  Types uint8 and float32 are read; the
     reference reads 2 bytes per voxel for any non-uint8 type (src/volume.h:30),
     which has no consumer on its path (its upload is R8 only), so other types
-    are rejected here."""
+    are rejected here.  pinned=True reads the file into pinned host memory."""
     m = _NAME_RE.search(str(path))
     if not m:
         raise RuntimeError("Invalaid raw file naming scheme: expected "
@@ -162,9 +207,16 @@ def loadVolume(path):
     if vtype not in _DT:
         raise RuntimeError("unsupported raw volume type %r" % vtype)
     n = dims[0] * dims[1] * dims[2]
-    data = np.fromfile(path, dtype=np.dtype(vtype), count=n)
-    if data.size != n:
-        raise RuntimeError("Failed to read volume data: %d of %d voxels" % (data.size, n))
+    if pinned:
+        data = pinned_array(n, vtype)
+        with open(path, "rb") as f:
+            got = f.readinto(memoryview(data).cast("B"))
+        if got != data.nbytes:
+            raise RuntimeError("Failed to read volume data: %d of %d bytes" % (got, data.nbytes))
+    else:
+        data = np.fromfile(path, dtype=np.dtype(vtype), count=n)
+        if data.size != n:
+            raise RuntimeError("Failed to read volume data: %d of %d voxels" % (data.size, n))
     return Volume(dims, vtype, data)
 
 
