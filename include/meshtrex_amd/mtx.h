@@ -101,7 +101,11 @@ typedef struct mtx_options {
                                 mtx_read_extraction_counts (default 0) */
     uint32_t minmax_rows;    /* B = 8 min-max: y-blocks per wavefront (1, 2 or 4); 0 = chosen
                                 from the volume's shape (default) */
-    uint32_t _reserved[6];
+    uint32_t coarse_reject;  /* filter from min/max pairs: skip a block whose pyramid-level-2
+                                ancestor (4^3 blocks) is inactive, without reading its own pair
+                                (src/shaders/occupiedBlockPrefixSum.comp.glsl:47-63); needs the
+                                pyramid; same result (default 0) */
+    uint32_t _reserved[5];
 } mtx_options;
 
 /* Min/max of every block (level 0) and the min/max pyramid, as uint32 key

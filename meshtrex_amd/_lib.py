@@ -34,7 +34,8 @@ class CVolume(ctypes.Structure):
 class Options(ctypes.Structure):
     _fields_ = [("overlap", u32), ("rule", u32), ("build_pyramid", u32), ("fuse_filter", u32),
                 ("normals", u32), ("edge_ids", u32), ("case_indices", u32),
-                ("voxel_positions", u32), ("reuse", u32), ("minmax_rows", u32), ("_reserved", u32 * 6)]
+                ("voxel_positions", u32), ("reuse", u32), ("minmax_rows", u32), ("coarse_reject", u32),
+                ("_reserved", u32 * 5)]
 
 
 class CMinMaxOutput(ctypes.Structure):

@@ -367,10 +367,14 @@ mtx_status mtx_filter_active_blocks(mtx_context* ctx, const mtx_minmax_output* m
     if (use_mask)
         e = launch_compact_mask(mm->active_mask, g, mm->mask_words_per_row, out->compacted_block_ids,
                                 out->active_block_count, out->block_to_active, ts, bd.s);
-    else
+    else {
+        // coarse reject from pyramid level 2 (an ancestor's range contains its blocks')
+        const bool coarse = o.coarse_reject && mm->levels_built > 2;
         e = launch_filter_pairs(mm->minmax, g, make_act((int)mm->dtype, pc->isovalue, (int)o.rule),
                                 out->compacted_block_ids, out->active_block_count, out->block_to_active, ts,
-                                bd.s);
+                                coarse ? mm->minmax + 2 * mm->level_offset[2] : nullptr,
+                                coarse ? &mm->level_extent[2] : nullptr, bd.s);
+    }
     if (e != hipSuccess) return fail(ctx, MTX_EDEVICE, "filter launch: %s", hipGetErrorString(e));
     return finish(bd);
 }

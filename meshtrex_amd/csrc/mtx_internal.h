@@ -65,9 +65,11 @@ hipError_t launch_compact_mask(const uint32_t* mask, const Grid& g, uint32_t wor
                                hipStream_t s);
 
 // Ordered compaction of active block IDs from leaf min/max pairs (+ dense map).
+// coarse: pyramid level 2 (4^3-block ancestors) of the same pairs, with its
+// extent, to skip blocks under inactive ancestors; or nullptr.
 hipError_t launch_filter_pairs(const uint32_t* pairs, const Grid& g, const ActParams& act,
                                uint32_t* ids, uint32_t* count, uint32_t* map, TileSums ts,
-                               hipStream_t s);
+                               const uint32_t* coarse, const uint32_t (*coarse_ext)[3], hipStream_t s);
 
 
 // ---- batches -----------------------------------------------------------------

@@ -621,11 +621,22 @@ uint32_t filter_tiles(uint64_t nb, bool from_mask, uint32_t words_per_row, const
     return (uint32_t)(tiles ? tiles : 1);
 }
 
+// Coarse reject of the pairs path: the pyramid level holding 4^3-block
+// ancestors (level 2), or none.
+struct Coarse {
+    const uint2* pairs;   // level-2 pairs, or nullptr
+    uint32_t gx, gy;      // level-0 grid (x, y)
+    uint32_t cx, cy;      // level-2 extent (x, y)
+};
+
 // Active flags of this thread's 8 blocks of `tile` as wave ballots, and the
-// wave's counts in s_cnt[k * 4 + wave].
+// wave's counts in s_cnt[k * 4 + wave].  With a coarse level, a block under an
+// inactive ancestor is inactive without reading its own pair: the ancestor's
+// range contains the block's, so under either rule it is active whenever the
+// block is (src/shaders/occupiedBlockPrefixSum.comp.glsl:47-63).
 __device__ __forceinline__ void tile_ballots(const uint2* __restrict__ pairs, uint64_t nb, uint64_t base,
                                              const ActParams& act, uint64_t (&bal)[kFilterItems],
-                                             uint32_t* s_cnt)
+                                             uint32_t* s_cnt, const Coarse& cr = Coarse{})
 {
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
 #pragma unroll
@@ -633,8 +644,17 @@ __device__ __forceinline__ void tile_ballots(const uint2* __restrict__ pairs, ui
         const uint64_t b = base + k * 256u + threadIdx.x;
         bool a = false;
         if (b < nb) {
-            const uint2 p = pairs[b];
-            a = block_active(p.x, p.y, act);
+            bool live = true;
+            if (cr.pairs) {
+                const uint32_t bx = (uint32_t)(b % cr.gx), by = (uint32_t)((b / cr.gx) % cr.gy);
+                const uint32_t bz = (uint32_t)(b / ((uint64_t)cr.gx * cr.gy));
+                const uint2 c = cr.pairs[(bx >> 2) + cr.cx * ((by >> 2) + (uint64_t)cr.cy * (bz >> 2))];
+                live = block_active(c.x, c.y, act);
+            }
+            if (live) {
+                const uint2 p = pairs[b];
+                a = block_active(p.x, p.y, act);
+            }
         }
         bal[k] = __ballot(a);
         if (lane == 0) s_cnt[k * 4 + w] = (uint32_t)__popcll(bal[k]);
@@ -642,11 +662,12 @@ __device__ __forceinline__ void tile_ballots(const uint2* __restrict__ pairs, ui
 }
 
 __global__ __launch_bounds__(256) void k_filter_pairs_count(const uint2* __restrict__ pairs, uint64_t nb,
-                                                            ActParams act, uint32_t* __restrict__ tile_count)
+                                                            ActParams act, uint32_t* __restrict__ tile_count,
+                                                            Coarse cr)
 {
     __shared__ uint32_t s_cnt[kFilterItems * 4];
     uint64_t bal[kFilterItems];
-    tile_ballots(pairs, nb, (uint64_t)blockIdx.x * kFilterTile, act, bal, s_cnt);
+    tile_ballots(pairs, nb, (uint64_t)blockIdx.x * kFilterTile, act, bal, s_cnt, cr);
     __syncthreads();
     if (threadIdx.x < 64) {
         const uint32_t v = threadIdx.x < kFilterItems * 4 ? s_cnt[threadIdx.x] : 0u;
@@ -696,7 +717,7 @@ __global__ __launch_bounds__(256) void k_filter_pairs_scatter(const uint2* __res
                                                               uint32_t* __restrict__ count,
                                                               uint32_t* __restrict__ map,
                                                               const uint32_t* __restrict__ tile_count,
-                                                              uint32_t ntiles)
+                                                              uint32_t ntiles, Coarse cr)
 {
     __shared__ uint32_t s_cnt[kFilterItems * 4];
     __shared__ uint64_t s_red[4];
@@ -704,7 +725,7 @@ __global__ __launch_bounds__(256) void k_filter_pairs_scatter(const uint2* __res
     const uint64_t base = (uint64_t)tile * kFilterTile;
     const uint64_t excl = tile_prefix<uint32_t>(tile_count, tile, s_red);
     uint64_t bal[kFilterItems];
-    tile_ballots(pairs, nb, base, act, bal, s_cnt);
+    tile_ballots(pairs, nb, base, act, bal, s_cnt, cr);
     __syncthreads();
     const uint32_t tot = tile_local_scan(s_cnt);
     if (threadIdx.x == 0 && tile == ntiles - 1) *count = (uint32_t)(excl + tot);
@@ -714,15 +735,21 @@ __global__ __launch_bounds__(256) void k_filter_pairs_scatter(const uint2* __res
 
 hipError_t launch_filter_pairs(const uint32_t* pairs, const Grid& g, const ActParams& act,
                                uint32_t* ids, uint32_t* count, uint32_t* map, TileSums ts,
-                               hipStream_t s)
+                               const uint32_t* coarse, const uint32_t (*coarse_ext)[3], hipStream_t s)
 {
     const uint64_t nb = g.nb();
     const uint32_t ntiles = filter_tiles(nb, false, 0, g);
     const uint2* pr = reinterpret_cast<const uint2*>(pairs);
     uint32_t* tc = reinterpret_cast<uint32_t*>(ts.sums);
-    hipLaunchKernelGGL(k_filter_pairs_count, dim3(ntiles), dim3(256), 0, s, pr, nb, act, tc);
+    Coarse cr{};
+    if (coarse) {
+        cr.pairs = reinterpret_cast<const uint2*>(coarse);
+        cr.gx = g.gx; cr.gy = g.gy;
+        cr.cx = (*coarse_ext)[0]; cr.cy = (*coarse_ext)[1];
+    }
+    hipLaunchKernelGGL(k_filter_pairs_count, dim3(ntiles), dim3(256), 0, s, pr, nb, act, tc, cr);
     hipLaunchKernelGGL(k_filter_pairs_scatter, dim3(ntiles), dim3(256), 0, s, pr, nb, act, ids, count, map, tc,
-                       ntiles);
+                       ntiles, cr);
     return hipGetLastError();
 }
 

@@ -311,3 +311,29 @@ def test_block_map_and_meshlets(ctx):
     got = read_device(ex.c.meshlets, np.uint32, 4 * nm).reshape(-1, 4)
     assert nm == len(exp)
     np.testing.assert_array_equal(got, np.array(exp, np.uint32).reshape(-1, 4))
+
+
+@pytest.mark.parametrize("B,rule", [(8, 0), (8, 1), (4, 0), (5, 1)])
+def test_coarse_reject_gives_the_same_list(ctx, B, rule):
+    # sparse vessels: most level-2 ancestors inactive
+    dims = (150, 131, 97)
+    data = oracle.gen_vessels_u8(dims, oracle.SEED_BASE + 1, 4, 8)
+    iso = 80.0
+    pc = makePushConstants(dims, B, iso)
+    vol = vol_of(data, dims)
+    # fused mask made for another isovalue: the filter reads the pairs
+    mm = computeMinMaxMip(ctx, vol, makePushConstants(dims, B, 7.0), options=makeOptions(rule=rule))
+    assert mm.c.levels_built > 2
+    from meshtrex_amd.managers import read_device
+    out = []
+    for coarse in (0, 1):
+        f = filterActiveBlocks(ctx, mm, pc, options=makeOptions(rule=rule, coarse_reject=coarse))
+        readActiveBlockCount(ctx, f)
+        nb = int(np.prod(mm.grid))
+        out.append((f.read_ids(), read_device(f.c.block_to_active, np.uint32, nb)))
+        f.cleanup()
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+    ref = oracle.filter_ids(oracle.minmax(data, dims, B, 1), oracle.U8, iso, rule)
+    np.testing.assert_array_equal(out[1][0], ref)
+    assert 0 < ref.size < nb // 2
