@@ -156,6 +156,7 @@ SYNTH_SIGS = {
     "mtx_synth_sdf_f32": (ctypes.c_int, [vp, ctypes.POINTER(u32), u64, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_double, ctypes.c_double, vp]),
     "mtx_bench_copy": (ctypes.c_int, [vp, vp, u64, vp]),
+    "mtx_bench_read": (ctypes.c_int, [vp, u64, vp, vp]),
 }
 
 _lib = None

@@ -173,7 +173,40 @@ __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, fl
     for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256u)
         dst[i] = src[i];
 }
+
+// Read-only stream: every thread sums its float4s (8 loads in flight per
+// iteration) and one value per workgroup is written, so the rate is the
+// achievable HBM read rate the min/max kernel is compared with.
+__global__ __launch_bounds__(256) void k_read(const float4* __restrict__ src, float* __restrict__ out, uint64_t n)
+{
+    float acc = 0.0f;
+    const uint64_t stride = (uint64_t)gridDim.x * 256u;
+    uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    for (; i + 7 * stride < n; i += 8 * stride) {
+        float4 v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = src[i + k * stride];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc += v[k].x + v[k].y + v[k].z + v[k].w;
+    }
+    for (; i < n; i += stride) {
+        const float4 v = src[i];
+        acc += v.x + v.y + v.z + v.w;
+    }
+    for (int d = 32; d > 0; d >>= 1) acc += __shfl_xor(acc, d, 64);
+    if ((threadIdx.x & 63) == 0) out[blockIdx.x * 4 + (threadIdx.x >> 6)] = acc;
+}
 }  // namespace
+
+// Reads `bytes` (a multiple of 16) once; `sink` holds 4 floats per workgroup
+// (8192 workgroups: 128 KiB).
+extern "C" int mtx_bench_read(const void* src, uint64_t bytes, void* sink, void* stream)
+{
+    if (!src || !sink || bytes % 16) return 1;
+    hipLaunchKernelGGL(k_read, dim3(8192), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       static_cast<const float4*>(src), static_cast<float*>(sink), bytes / 16);
+    return hipGetLastError() == hipSuccess ? 0 : 4;
+}
 
 extern "C" int mtx_bench_copy(void* dst, const void* src, uint64_t bytes, void* stream)
 {
