@@ -28,6 +28,12 @@
 #else
 #define MTX_HD
 #endif
+/* constexpr where the language has it (the oracle includes this from C) */
+#ifdef __cplusplus
+#define MTX_CE constexpr
+#else
+#define MTX_CE
+#endif
 
 static const uint64_t mtx_tri_words[256] = {
     0x0000000000000000ull, 0x1000000000000380ull, 0x1000000000000910ull, 0x2000000000189381ull,
@@ -97,23 +103,23 @@ static const uint64_t mtx_tri_words[256] = {
 };
 
 /* Number of triangles case c produces (0..5). */
-static inline MTX_HD unsigned mtx_word_ntri(uint64_t w) { return (unsigned)(w >> 60); }
+static inline MTX_CE MTX_HD unsigned mtx_word_ntri(uint64_t w) { return (unsigned)(w >> 60); }
 /* Edge index (0..11) of vertex j of triangle k. */
-static inline MTX_HD unsigned mtx_word_edge(uint64_t w, unsigned k, unsigned j)
+static inline MTX_CE MTX_HD unsigned mtx_word_edge(uint64_t w, unsigned k, unsigned j)
 {
     return (unsigned)((w >> (4u * (3u * k + j))) & 0xFu);
 }
 
 /* Corner offsets, bit-packed: corner i -> (x,y,z). */
-static inline MTX_HD unsigned mtx_corner_x(unsigned i) { return ((i + 1u) >> 1) & 1u; }
-static inline MTX_HD unsigned mtx_corner_y(unsigned i) { return (i >> 1) & 1u; }
-static inline MTX_HD unsigned mtx_corner_z(unsigned i) { return i >> 2; }
+static inline MTX_CE MTX_HD unsigned mtx_corner_x(unsigned i) { return ((i + 1u) >> 1) & 1u; }
+static inline MTX_CE MTX_HD unsigned mtx_corner_y(unsigned i) { return (i >> 1) & 1u; }
+static inline MTX_CE MTX_HD unsigned mtx_corner_z(unsigned i) { return i >> 2; }
 
 /* Cube edges: edges 0..3 run around the z=0 face (0-1, 1-2, 2-3, 3-0),
  * edges 4..7 around the z=1 face (4-5, 5-6, 6-7, 7-4), edges 8..11 are the
  * vertical edges i -> i+4. */
-static inline MTX_HD unsigned mtx_edge_c0(unsigned e) { return e < 8u ? e : e - 8u; }
-static inline MTX_HD unsigned mtx_edge_c1(unsigned e)
+static inline MTX_CE MTX_HD unsigned mtx_edge_c0(unsigned e) { return e < 8u ? e : e - 8u; }
+static inline MTX_CE MTX_HD unsigned mtx_edge_c1(unsigned e)
 {
     return e < 8u ? ((e & 4u) | ((e + 1u) & 3u)) : e - 4u;
 }
@@ -123,7 +129,7 @@ static inline MTX_HD unsigned mtx_edge_c1(unsigned e)
  * Packed as dx | dy<<1 | dz<<2 | axis<<3.  Derived from the corner geometry;
  * equals the reference's edgeOwner table
  * (src/shaders/marching_cubes_pmb.mesh.glsl:39-42). */
-static inline MTX_HD unsigned mtx_edge_owner(unsigned e)
+static inline MTX_CE MTX_HD unsigned mtx_edge_owner(unsigned e)
 {
     unsigned a = mtx_edge_c0(e), b = mtx_edge_c1(e);
     unsigned ax = mtx_corner_x(a), ay = mtx_corner_y(a), az = mtx_corner_z(a);

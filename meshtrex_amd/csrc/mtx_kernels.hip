@@ -1594,6 +1594,15 @@ template <int B> struct TriCfg {
     static constexpr int WPG = WAVE_LDS * 4 + 2048 <= 65536 ? 4 : (WAVE_LDS * 2 + 2048 <= 65536 ? 2 : 1);
 };
 
+// Owners of the 12 cube edges, 5 bits each (dx | dy << 1 | dz << 2 | axis << 3).
+constexpr uint64_t edge_owner_table()
+{
+    uint64_t t = 0;
+    for (unsigned e = 0; e < 12u; ++e) t |= (uint64_t)mtx_edge_owner(e) << (5u * e);
+    return t;
+}
+constexpr uint64_t kEdgeOwners = edge_owner_table();
+
 // Face corner f of [0, B]^3 that a +x/+y/+z neighbour owns (a coordinate
 // equal to B): x = B (F*F corners), then y = B with x < B (B*F), then z = B
 // with x, y < B (B*B): F^3 - B^3 in all.
@@ -1799,23 +1808,24 @@ __global__ __launch_bounds__((64 * TriCfg<B>::WPG)) void k_extract_triangles(Ext
                 uint32_t run = et;
 #pragma unroll
                 for (int j = 0; j < CPL; ++j) {
+                    // entry: cell (12 bits) | the triangle's three edges (4 bits each)
                     const uint32_t c = lane * CPL + j;
-                    const uint32_t cs = cur.lc.cs(j);
-                    const uint32_t ntri = mtx_word_ntri(s_words[cs]);
+                    const uint64_t wd = s_words[cur.lc.cs(j)];
+                    const uint32_t ntri = mtx_word_ntri(wd);
                     for (uint32_t k = 0; k < ntri; ++k, ++run)
-                        if (run - c0 < LIST) s_tl[run - c0] = c | (k << 12) | (cs << 16);
+                        if (run - c0 < LIST) s_tl[run - c0] = c | ((uint32_t)(wd >> (12u * k)) & 0xFFFu) << 12;
                 }
                 wave_lds_sync();
                 const uint32_t n = min(LIST, tt - c0);
                 for (uint32_t q = lane; q < n; q += 64u) {
                     const uint32_t e = s_tl[q];
-                    const uint32_t c = e & 0xFFFu, k = (e >> 12) & 7u;
-                    const uint64_t wd = s_words[e >> 16];
+                    const uint32_t c = e & 0xFFFu;
                     const uint32_t q0 = c % B + F * ((c / B) % B + F * (c / (B * B)));
                     uint32_t idx3[3];
 #pragma unroll
                     for (uint32_t e3 = 0; e3 < 3; ++e3) {
-                        const uint32_t o = mtx_edge_owner(mtx_word_edge(wd, k, e3));
+                        const uint32_t edge = (e >> (12u + 4u * e3)) & 0xFu;
+                        const uint32_t o = (uint32_t)(kEdgeOwners >> (5u * edge)) & 0x1Fu;
                         const uint32_t qq = q0 + (o & 1u) + F * (((o >> 1) & 1u) + F * ((o >> 2) & 1u));
                         const uint32_t ax = o >> 3;
                         const uint32_t gf = s_gfl[qq];
