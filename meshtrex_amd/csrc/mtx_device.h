@@ -25,6 +25,18 @@ __device__ __forceinline__ float unkey(uint32_t k)
     return __uint_as_float(b);
 }
 
+// A pointer read from a descriptor in memory (a batch unit's volume, pairs or
+// map) is generic to the compiler, and loads through it become flat loads,
+// which also count against the 4-bit LDS/scalar counter and so cap the loads in
+// flight.  Every such pointer is device global memory: rebuilding it as an
+// address-space-1 pointer lets the loads derived from it be global ones.
+template <typename T>
+__device__ __forceinline__ T* as_global(T* p)
+{
+    using G = __attribute__((address_space(1))) T;
+    return (T*)(G*)(uintptr_t)p;
+}
+
 __device__ __forceinline__ float vfloat(float f) { return f; }
 __device__ __forceinline__ float vfloat(uint8_t v) { return (float)v; }
 

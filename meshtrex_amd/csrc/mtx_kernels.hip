@@ -315,7 +315,7 @@ __device__ __forceinline__ void mm_batch_body(const BatchVolume& V, uint32_t til
 {
     constexpr uint32_t L = B / 4;       // lanes per x-block
     constexpr uint32_t BPW = 64 / L;    // x-blocks per wave segment
-    const T* __restrict__ vol = static_cast<const T*>(V.data);
+    const T* __restrict__ vol = as_global(static_cast<const T*>(V.data));
     const uint32_t X = V.X, Y = V.Y, Z = V.Z;
     const uint32_t txg = tile % V.ntxg;
     const uint32_t rest = tile / V.ntxg;
@@ -337,7 +337,7 @@ __device__ __forceinline__ void mm_batch_body(const BatchVolume& V, uint32_t til
     const bool xe_ok = xe < X;
     const uint32_t y0 = by0 * (uint32_t)B;
     const uint64_t XY = (uint64_t)X * Y;
-    uint2* __restrict__ pairs = reinterpret_cast<uint2*>(V.pairs);
+    uint2* __restrict__ pairs = as_global(reinterpret_cast<uint2*>(V.pairs));
 
     MMAcc first[M];
     mm_plane<T, ALIGNED, M, B>(first, false, vol, (uint64_t)(bz0 * B) * XY, X, Y, y0, x0, lane_ok, xe, xe_ok);
@@ -846,8 +846,8 @@ __global__ __launch_bounds__(256) void k_filter_batch_count(const BatchUnit* __r
     const uint32_t u = batch_tile_unit(tile0, nunits, tile, &s_unit);
     const BatchUnit& U = units[u];
     uint64_t bal[kFilterItems];
-    tile_ballots(reinterpret_cast<const uint2*>(U.pairs), U.nb, (uint64_t)(tile - U.tile0) * kFilterTile, U.act,
-                 bal, s_cnt);
+    tile_ballots(as_global(reinterpret_cast<const uint2*>(U.pairs)), U.nb, (uint64_t)(tile - U.tile0) * kFilterTile,
+                 U.act, bal, s_cnt);
     __syncthreads();
     if (threadIdx.x < 64) {
         const uint32_t v = threadIdx.x < kFilterItems * 4 ? s_cnt[threadIdx.x] : 0u;
@@ -873,7 +873,7 @@ __global__ __launch_bounds__(256) void k_filter_batch_scatter(const BatchUnit* _
     const BatchUnit& U = units[u];
     const uint64_t base = (uint64_t)(tile - U.tile0) * kFilterTile;
     uint64_t bal[kFilterItems];
-    tile_ballots(reinterpret_cast<const uint2*>(U.pairs), U.nb, base, U.act, bal, s_cnt);
+    tile_ballots(as_global(reinterpret_cast<const uint2*>(U.pairs)), U.nb, base, U.act, bal, s_cnt);
     __syncthreads();
     const uint32_t tot = tile_local_scan(s_cnt);
     if (threadIdx.x == 0) {
@@ -884,7 +884,7 @@ __global__ __launch_bounds__(256) void k_filter_batch_scatter(const BatchUnit* _
         }
     }
     __syncthreads();
-    tile_scatter(bal, s_cnt, base, U.nb, excl, ids, item_unit, u, U.map);
+    tile_scatter(bal, s_cnt, base, U.nb, excl, ids, item_unit, u, as_global(U.map));
 }
 
 uint32_t batch_filter_tiles(uint32_t nb) { return (nb + kFilterTile - 1) / kFilterTile; }
@@ -952,7 +952,8 @@ struct WaveItems {
 
 __device__ __forceinline__ WaveItems wave_items(uint32_t na, uint32_t wpg)
 {
-    const uint32_t w = threadIdx.x >> 6;
+    // wave-uniform: item indices, block IDs and tile bases stay in scalar registers
+    const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     WaveItems r;
     if (gridDim.x % 8u) {
         r.first = blockIdx.x * wpg + w;
@@ -992,11 +993,11 @@ __device__ __forceinline__ ItemVol item_vol(const ExtractArgs& a, uint32_t i, ui
     if constexpr (BATCH) {
         unit = __builtin_amdgcn_readfirstlane(a.item_unit[i]);
         const BatchUnit& U = a.units[unit];
-        r.data = U.data;
+        r.data = as_global(U.data);
         r.X = U.X; r.Y = U.Y; r.Z = U.Z;
         r.gx = U.gx; r.gy = U.gy; r.gz = U.gz;
         r.iso = U.act.iso;
-        r.map = U.map;
+        r.map = as_global(U.map);
     } else {
         unit = 0;
         r.data = a.vol.data;
@@ -1115,8 +1116,9 @@ __global__ __launch_bounds__(256) void k_extract_count(ExtractArgs a)
 // Count pass specialised for B = 8: the below-isovalue bits of the 9^3 tile are
 // gathered with wave ballots (12 x 64 bits, in LDS), each lane owns one x-row
 // of 8 corners (ly = lane & 7, lz = lane >> 3) and classifies it with bit
-// operations on its four 9-bit rows (y, y+1) x (z, z+1).  Interior blocks load
-// their tile through per-lane offsets computed once.
+// operations on its four 9-bit rows (y, y+1) x (z, z+1).  A lane's 12 tile
+// coordinates are packed once; the loads take 32-bit offsets from the tile
+// origin.
 template <typename T, bool BATCH>
 __global__ __launch_bounds__(256) void k_extract_count_b8(ExtractArgs a)
 {
@@ -1127,15 +1129,14 @@ __global__ __launch_bounds__(256) void k_extract_count_b8(ExtractArgs a)
     __syncthreads();
     const uint32_t lane = lane_id();
     uint64_t* s_bits = s_bits_all[threadIdx.x >> 6];
-    uint32_t toff[LPL];
-    auto tile_offsets = [&](uint32_t X, uint32_t Y, bool off32) {
+    // tile coordinates of the voxels this lane loads, j = lane + 64 k:
+    // (j % 9) | (j / 9 % 9) << 4 | (j / 81) << 8, and bit 12 past the tile
+    uint32_t tcoord[LPL];
 #pragma unroll
-        for (int k = 0; k < LPL; ++k) {
-            const uint32_t j = min(lane + 64u * k, (uint32_t)NTILE - 1u);
-            toff[k] = off32 ? j % E + X * ((j / E) % E + Y * (j / (E * E))) : 0u;
-        }
-    };
-    if constexpr (!BATCH) tile_offsets(a.vol.X, a.vol.Y, (uint64_t)a.vol.X * a.vol.Y * E < 0x80000000ull);
+    for (int k = 0; k < LPL; ++k) {
+        const uint32_t j = lane + 64u * k;
+        tcoord[k] = j < (uint32_t)NTILE ? (j % E) | (((j / E) % E) << 4) | ((j / (E * E)) << 8) : 0x1000u;
+    }
     const uint32_t na = *a.count;
     uint16_t* __restrict__ table = static_cast<uint16_t*>(a.corner_table);
     const uint32_t ly = lane & 7u, lz = lane >> 3;
@@ -1147,28 +1148,32 @@ __global__ __launch_bounds__(256) void k_extract_count_b8(ExtractArgs a)
         const uint32_t X = V.X, Y = V.Y, Z = V.Z;
         const bool degen = X < 2 || Y < 2 || Z < 2;
         const bool off32 = (uint64_t)X * Y * E < 0x80000000ull;
-        if constexpr (BATCH) tile_offsets(X, Y, off32);
         const uint32_t id = a.ids[i];
         const uint32_t bx = id % V.gx, by = (id / V.gx) % V.gy;
         const uint32_t bz = id / (V.gx * V.gy);
         const uint32_t ox = bx * B, oy = by * B, oz = bz * B;
         const uint64_t base = ox + (uint64_t)X * (oy + (uint64_t)Y * oz);
+        const T* __restrict__ tile = vol + base;
         const bool interior = off32 && ox + B < X && oy + B < Y && oz + B < Z;
+        // the coordinates are re-read per block (opaque to the compiler), so
+        // the 36 values derived from them are not kept live across blocks
+        uint32_t tc[LPL];
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) {
+            tc[k] = tcoord[k];
+            asm volatile("" : "+v"(tc[k]));
+        }
         T v[LPL];
         bool in[LPL];
-        if (interior) {
 #pragma unroll
-            for (int k = 0; k < LPL; ++k) {
-                v[k] = vol[base + toff[k]];
-                in[k] = lane + 64u * k < (uint32_t)NTILE;
-            }
-        } else {
-#pragma unroll
-            for (int k = 0; k < LPL; ++k) {
-                const uint32_t j = lane + 64u * k;
-                const uint32_t x = ox + j % E, y = oy + (j / E) % E, z = oz + j / (E * E);
-                in[k] = j < (uint32_t)NTILE && x < X && y < Y && z < Z;
-                v[k] = vol[in[k] ? x + (uint64_t)X * (y + (uint64_t)Y * z) : 0];
+        for (int k = 0; k < LPL; ++k) {
+            const uint32_t tx = tc[k] & 15u, ty = (tc[k] >> 4) & 15u, tz = (tc[k] >> 8) & 15u;
+            in[k] = !(tc[k] >> 12) && (interior || (ox + tx < X && oy + ty < Y && oz + tz < Z));
+            if (off32) {
+                // tile-relative offsets fit 32 bits; voxels outside read the tile origin
+                v[k] = tile[in[k] ? tx + X * (ty + Y * tz) : 0u];
+            } else {
+                v[k] = tile[in[k] ? tx + (uint64_t)X * (ty + (uint64_t)Y * tz) : 0ull];
             }
         }
 #pragma unroll
@@ -1719,7 +1724,7 @@ __global__ __launch_bounds__((64 * TriCfg<B>::WPG)) void k_extract_triangles(Ext
             P.unit = __builtin_amdgcn_readfirstlane(P.unit_v);
             const BatchUnit& U = a.units[P.unit];
             gx = U.gx; gy = U.gy; gz = U.gz;
-            map = U.map;
+            map = as_global(U.map);
             P.first = a.unit_first[P.unit];
         } else {
             gx = a.grid.gx; gy = a.grid.gy; gz = a.grid.gz;
