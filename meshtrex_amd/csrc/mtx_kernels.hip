@@ -1595,8 +1595,16 @@ template <int B> struct TriCfg {
     // triangles listed at a time: one pass for all but the densest blocks
     // (the kernel's registers, not its LDS, bound its occupancy)
     static constexpr int LIST = B <= 8 ? 1024 : 512;
+    // list entry: owner-corner index of the cell's (0,0,0) corner (QB bits)
+    // | decoded-triangle index (case * 5 + k) << QB
+    static constexpr int QB = NF <= 1024 ? 10 : 13;
+    // decoded triangle: per vertex, DB bits = offset of its edge's owner
+    // corner from the cell's (DB - 2 bits) | rank mask (1 << axis) - 1
+    static constexpr int DB = F * F + F < 256 ? 10 : 20;
+    using TDec = typename std::conditional<DB == 10, uint32_t, uint64_t>::type;
     static constexpr int WAVE_LDS = NF * 5 + LIST * 4 + 64;
-    static constexpr int WPG = WAVE_LDS * 4 + 2048 <= 65536 ? 4 : (WAVE_LDS * 2 + 2048 <= 65536 ? 2 : 1);
+    static constexpr int WG_LDS = 256 + 1280 * (int)sizeof(TDec) + 4 * (NF - B * B * B);
+    static constexpr int WPG = WAVE_LDS * 4 + WG_LDS <= 65536 ? 4 : (WAVE_LDS * 2 + WG_LDS <= 65536 ? 2 : 1);
 };
 
 // Owners of the 12 cube edges, 5 bits each (dx | dy << 1 | dz << 2 | axis << 3).
@@ -1683,11 +1691,28 @@ __global__ __launch_bounds__((64 * TriCfg<B>::WPG)) void k_extract_triangles(Ext
     __shared__ uint32_t s_gidx_all[WPG][NF];
     __shared__ uint8_t s_gfl_all[WPG][NF];
     __shared__ uint32_t s_tl_all[WPG][LIST];
-    __shared__ uint64_t s_words[256];
+    using TDec = typename C::TDec;
+    constexpr int QB = C::QB, DB = C::DB;
+    __shared__ uint8_t s_ntri[256];
+    __shared__ TDec s_tdec[256 * 5];
     // face corner f: its index in [0, B]^3 (bits 0..12), the neighbour that
     // owns it (13..15) and its corner in that neighbour's table (16..27)
     __shared__ uint32_t s_face[NFACE];
-    for (uint32_t c = threadIdx.x; c < 256u; c += blockDim.x) s_words[c] = mtx_tri_words[c];
+    for (uint32_t c = threadIdx.x; c < 256u; c += blockDim.x) s_ntri[c] = (uint8_t)mtx_word_ntri(mtx_tri_words[c]);
+    for (uint32_t x = threadIdx.x; x < 256u * 5u; x += blockDim.x) {
+        const uint64_t wd = mtx_tri_words[x / 5u];
+        const uint32_t k = x % 5u;
+        TDec d = 0;
+        if (k < mtx_word_ntri(wd)) {
+            for (uint32_t j = 0; j < 3u; ++j) {
+                const uint32_t o = (uint32_t)(kEdgeOwners >> (5u * mtx_word_edge(wd, k, j))) & 0x1Fu;
+                const uint32_t off = (o & 1u) + F * (((o >> 1) & 1u) + F * ((o >> 2) & 1u));
+                const uint32_t m = (1u << (o >> 3)) - 1u;
+                d |= (TDec)(off | (m << (DB - 2))) << (DB * j);
+            }
+        }
+        s_tdec[x] = d;
+    }
     for (uint32_t f = threadIdx.x; f < (uint32_t)NFACE; f += blockDim.x) {
         uint32_t qx, qy, qz;
         face_corner<B>(f, qx, qy, qz);
@@ -1776,6 +1801,8 @@ __global__ __launch_bounds__((64 * TriCfg<B>::WPG)) void k_extract_triangles(Ext
             work = false;
         }
         uint32_t nt = 0;
+        constexpr bool PACKN = CPL * 3 <= 32;   // per-cell triangle counts kept in one register
+        uint32_t npk = 0;
         if (work) {
             // own corners: vertex index = block base + prefix from the count pass
             const uint32_t vrel = cur.vb - cur.uv;
@@ -1787,7 +1814,9 @@ __global__ __launch_bounds__((64 * TriCfg<B>::WPG)) void k_extract_triangles(Ext
                     s_gidx[q] = vrel + (cur.lc.ent(j) >> 3);
                     s_gfl[q] = (uint8_t)(cur.lc.ent(j) & 7u);
                 }
-                nt += mtx_word_ntri(s_words[cur.lc.cs(j)]);
+                const uint32_t n = s_ntri[cur.lc.cs(j)];
+                nt += n;
+                if (PACKN) npk |= n << (3 * j);
             }
             // neighbour-owned face corners
 #pragma unroll
@@ -1809,42 +1838,51 @@ __global__ __launch_bounds__((64 * TriCfg<B>::WPG)) void k_extract_triangles(Ext
             const uint64_t tb = cur.tb;
             uint32_t tt;
             const uint32_t et = wave_excl_scan<uint32_t>(nt, tt);
+            uint32_t bad = 0;
             for (uint32_t c0 = 0; c0 < tt; c0 += LIST) {
-                uint32_t run = et;
+                // list entries in output order: cell by cell, the case's triangles k = 0..4
+                uint32_t run = et - c0;
+                // re-read per pass (opaque to the compiler): nothing derived
+                // from them stays live across passes, which are rare
+                LaneCorners<B> lc = cur.lc;
+#pragma unroll
+                for (int k = 0; k < LaneCorners<B>::NCW; ++k) asm volatile("" : "+v"(lc.csw[k]));
+                asm volatile("" : "+v"(npk));
 #pragma unroll
                 for (int j = 0; j < CPL; ++j) {
-                    // entry: cell (12 bits) | the triangle's three edges (4 bits each)
                     const uint32_t c = lane * CPL + j;
-                    const uint64_t wd = s_words[cur.lc.cs(j)];
-                    const uint32_t ntri = mtx_word_ntri(wd);
-                    for (uint32_t k = 0; k < ntri; ++k, ++run)
-                        if (run - c0 < LIST) s_tl[run - c0] = c | ((uint32_t)(wd >> (12u * k)) & 0xFFFu) << 12;
+                    if (c < (uint32_t)NC) {
+                        const uint32_t cs = lc.cs(j);
+                        const uint32_t n = PACKN ? (npk >> (3 * j)) & 7u : (uint32_t)s_ntri[cs];
+                        const uint32_t ent = (c % B + F * ((c / B) % B + F * (c / (B * B)))) | ((cs * 5u) << QB);
+#pragma unroll
+                        for (uint32_t k = 0; k < 5u; ++k)
+                            if (k < n && run + k < LIST) s_tl[run + k] = ent + (k << QB);
+                        run += n;
+                    }
                 }
                 wave_lds_sync();
                 const uint32_t n = min(LIST, tt - c0);
                 for (uint32_t q = lane; q < n; q += 64u) {
                     const uint32_t e = s_tl[q];
-                    const uint32_t c = e & 0xFFFu;
-                    const uint32_t q0 = c % B + F * ((c / B) % B + F * (c / (B * B)));
+                    const uint32_t q0 = e & ((1u << QB) - 1u);
+                    const TDec d = s_tdec[e >> QB];
                     uint32_t idx3[3];
 #pragma unroll
                     for (uint32_t e3 = 0; e3 < 3; ++e3) {
-                        const uint32_t edge = (e >> (12u + 4u * e3)) & 0xFu;
-                        const uint32_t o = (uint32_t)(kEdgeOwners >> (5u * edge)) & 0x1Fu;
-                        const uint32_t qq = q0 + (o & 1u) + F * (((o >> 1) & 1u) + F * ((o >> 2) & 1u));
-                        const uint32_t ax = o >> 3;
+                        const uint32_t f = (uint32_t)(d >> (DB * e3));
+                        const uint32_t qq = q0 + (f & ((1u << (DB - 2)) - 1u));
                         const uint32_t gf = s_gfl[qq];
-                        if (gf & 0xF0u) {
-                            // an active edge whose owner block is not in the active list:
-                            // the list does not belong to this volume/isovalue
-                            atomicOr(reinterpret_cast<unsigned long long*>(&a.totals[2]), 2ull);
-                        }
-                        idx3[e3] = s_gidx[qq] + __popc(gf & 7u & ((1u << ax) - 1u));
+                        bad |= gf;
+                        idx3[e3] = __popc(gf & (f >> (DB - 2)) & 3u) + s_gidx[qq];
                     }
                     *reinterpret_cast<uint3*>(a.indices + 3 * (tb + c0 + q)) = make_uint3(idx3[0], idx3[1], idx3[2]);
                 }
                 wave_lds_sync();
             }
+            // an active edge whose owner block is not in the active list: the
+            // list does not belong to this volume/isovalue
+            if (bad & 0xF0u) atomicOr(reinterpret_cast<unsigned long long*>(&a.totals[2]), 2ull);
             if (a.meshlets) {
                 for (uint32_t k = lane; k < bnm; k += 64u) {
                     const uint32_t prims = min(kMeshletPrims, bnt - k * kMeshletPrims);
