@@ -1901,17 +1901,23 @@ __global__ __launch_bounds__((64 * TriCfg<B>::WPG)) void k_extract_triangles(Ext
 
 // Persistent grids of the extraction kernels, in workgroups per CU (a
 // multiple of 8 workgroups for the XCD split, never more waves than blocks).
-// Measured on MI355X: the single-volume pass is fastest at 8 per CU (fewer
-// leave CUs idle while long blocks finish, more add per-workgroup set-up); a
-// batch, whose blocks differ far more in work (dense and sparse surfaces of
-// many units), is fastest with finer static shares, 32-128 per CU.
+// Measured on MI355X: a single volume is fastest with whole rounds of the
+// resident workgroups -- 8 per CU for the triangle pass (4 resident), 10 for
+// the vertex pass (5 resident), 24 for the count pass (8 resident, the extra
+// rounds even out the blocks' load times); fewer leave CUs idle while long
+// blocks finish, more add per-workgroup set-up.  A batch, whose blocks differ
+// far more in work (dense and sparse surfaces of many units), is fastest with
+// finer static shares, 32-128 per CU.
 constexpr uint32_t kWgPerCuSingle = 8;
+constexpr uint32_t kWgPerCuSingleVertices = 10;
+constexpr uint32_t kWgPerCuSingleCount = 24;
 constexpr uint32_t kWgPerCuBatch = 64;
 
 template <typename K>
-static hipError_t launch_persistent(K kernel, uint32_t threads, const ExtractArgs& a, hipStream_t s)
+static hipError_t launch_persistent(K kernel, uint32_t threads, const ExtractArgs& a, hipStream_t s,
+                                    uint32_t per_cu_single = kWgPerCuSingle)
 {
-    uint64_t n = (uint64_t)device_cus() * (a.units ? kWgPerCuBatch : kWgPerCuSingle);
+    uint64_t n = (uint64_t)device_cus() * (a.units ? kWgPerCuBatch : per_cu_single);
     const uint64_t need = ((uint64_t)a.max_blocks + threads / 64 - 1) / (threads / 64);
     if (need < n) n = need;
     n = n < 8 ? 8 : n / 8 * 8;
@@ -1923,8 +1929,9 @@ template <int B, bool BATCH>
 static hipError_t launch_count_b(const ExtractArgs& a, hipStream_t s)
 {
     if (B == 8) {
-        if (a.vol.dtype == kF32) return launch_persistent(k_extract_count_b8<float, BATCH>, 256, a, s);
-        return launch_persistent(k_extract_count_b8<uint8_t, BATCH>, 256, a, s);
+        if (a.vol.dtype == kF32)
+            return launch_persistent(k_extract_count_b8<float, BATCH>, 256, a, s, kWgPerCuSingleCount);
+        return launch_persistent(k_extract_count_b8<uint8_t, BATCH>, 256, a, s, kWgPerCuSingleCount);
     }
     if (a.vol.dtype == kF32) return launch_persistent(k_extract_count<B, float, BATCH>, 256, a, s);
     return launch_persistent(k_extract_count<B, uint8_t, BATCH>, 256, a, s);
@@ -1935,11 +1942,15 @@ static hipError_t launch_emit_b(const ExtractArgs& a, hipStream_t s)
 {
     hipError_t e;
     if (a.vol.dtype == kF32) {
-        e = a.normals ? launch_persistent(k_extract_vertices<B, float, true, BATCH>, 64 * VertCfg<B, true, float>::WPG, a, s)
-                      : launch_persistent(k_extract_vertices<B, float, false, BATCH>, 64 * VertCfg<B, false, float>::WPG, a, s);
+        e = a.normals ? launch_persistent(k_extract_vertices<B, float, true, BATCH>, 64 * VertCfg<B, true, float>::WPG,
+                                          a, s, kWgPerCuSingleVertices)
+                      : launch_persistent(k_extract_vertices<B, float, false, BATCH>, 64 * VertCfg<B, false, float>::WPG,
+                                          a, s, kWgPerCuSingleVertices);
     } else {
-        e = a.normals ? launch_persistent(k_extract_vertices<B, uint8_t, true, BATCH>, 64 * VertCfg<B, true, uint8_t>::WPG, a, s)
-                      : launch_persistent(k_extract_vertices<B, uint8_t, false, BATCH>, 64 * VertCfg<B, false, uint8_t>::WPG, a, s);
+        e = a.normals ? launch_persistent(k_extract_vertices<B, uint8_t, true, BATCH>,
+                                          64 * VertCfg<B, true, uint8_t>::WPG, a, s, kWgPerCuSingleVertices)
+                      : launch_persistent(k_extract_vertices<B, uint8_t, false, BATCH>,
+                                          64 * VertCfg<B, false, uint8_t>::WPG, a, s, kWgPerCuSingleVertices);
     }
     if (e != hipSuccess) return e;
     return launch_persistent(k_extract_triangles<B, BATCH>, 64 * TriCfg<B>::WPG, a, s);
