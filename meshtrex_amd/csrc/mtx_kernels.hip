@@ -1378,28 +1378,51 @@ __global__ __launch_bounds__(256) void k_extract_scan(const uint32_t* __restrict
 }
 
 template <typename LT>
-__device__ __forceinline__ void grad_normal(const LT* s, uint32_t t, uint32_t E, float n[3])
+__device__ __forceinline__ void grad_normal(const LT* s, uint32_t t, uint32_t sy, uint32_t sz, float n[3])
 {
     // normalize(s(p-x)-s(p+x), s(p-y)-s(p+y), s(p-z)-s(p+z)); (0,1,0) where the
-    // gradient vanishes (|g| < 1e-5)
+    // gradient vanishes (|g| < 1e-5); sy, sz: the tile's y and z strides
     const float gx = vfloat(s[t - 1]) - vfloat(s[t + 1]);
-    const float gy = vfloat(s[t - E]) - vfloat(s[t + E]);
-    const float gz = vfloat(s[t - E * E]) - vfloat(s[t + E * E]);
+    const float gy = vfloat(s[t - sy]) - vfloat(s[t + sy]);
+    const float gz = vfloat(s[t - sz]) - vfloat(s[t + sz]);
     const float l2 = gx * gx + gy * gy + gz * gz;
     if (l2 < 1e-10f) { n[0] = 0.f; n[1] = 1.f; n[2] = 0.f; return; }
     const float r = rsqrtf(l2);
     n[0] = gx * r; n[1] = gy * r; n[2] = gz * r;
 }
 
+// Direct global -> LDS loads of 4 or 16 bytes per lane: lane l's bytes land at
+// dst + l * size; dst is wave-uniform.  Completion is tracked by the
+// vector-memory counter.
+__device__ __forceinline__ void lds_direct_load4(const void* src, void* dst)
+{
+    __builtin_amdgcn_global_load_lds((__attribute__((address_space(1))) void*)(uintptr_t)src,
+                                     (__attribute__((address_space(3))) void*)dst, 4, 0, 0);
+}
+__device__ __forceinline__ void lds_direct_load16(const void* src, void* dst)
+{
+    __builtin_amdgcn_global_load_lds((__attribute__((address_space(1))) void*)(uintptr_t)src,
+                                     (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
+}
+
 // The tile is kept in LDS in the volume's own type (u8 voxels as bytes: a
 // quarter of the LDS of f32, so LDS does not bound the kernel's occupancy).
+// Its rows are padded to whole 4-voxel segments (pitch RP): interior tiles
+// are loaded one segment per lane straight from global memory into LDS
+// (16-byte direct loads for f32, 4-byte for u8), ND loads per lane and no
+// registers, the last load's spare lanes writing past the tile.
 template <int B, bool NORMALS, typename T> struct VertCfg {
     static constexpr int A = NORMALS ? 1 : 0;  // apron for central differences
     static constexpr int E = B + 1 + 2 * A;
     static constexpr int NTILE = E * E * E;
-    static constexpr int NTILE_PAD = (NTILE + 3) / 4 * 4;
+    static constexpr int RP = (E + 3) / 4 * 4;  // LDS row pitch, voxels
+    static constexpr int NS = RP / 4;           // 4-voxel segments per row
+    static constexpr int NSEG = E * E * NS;
+    static constexpr int ND = (NSEG + 63) / 64;
+    static constexpr int TILE = ND * 64 * 4;    // LDS voxels incl. the spare lanes' segments
+    static constexpr unsigned SEGB = 4 * sizeof(T);   // bytes per direct load and lane
     static constexpr int LIST = 512;            // vertices listed at a time
-    static constexpr int WAVE_LDS = NTILE_PAD * (int)sizeof(T) + LIST * 2 + 16;
+    static constexpr int WAVE_LDS = TILE * (int)sizeof(T) + LIST * 2 + 16;
     static constexpr int WPG = WAVE_LDS * 4 <= 65536 ? 4 : (WAVE_LDS * 2 <= 65536 ? 2 : 1);
 };
 
@@ -1457,12 +1480,13 @@ __global__ __launch_bounds__((64 * VertCfg<B, NORMALS, T>::WPG)) void k_extract_
 {
     using C = VertCfg<B, NORMALS, T>;
     constexpr int A = C::A, E = C::E, NTILE = C::NTILE, WPG = C::WPG;
+    constexpr uint32_t RP = C::RP, NS = C::NS, NSEG = C::NSEG, ND = C::ND;
     constexpr uint32_t LIST = C::LIST;
     constexpr int CPL = EB<B>::CPL;
     using Entry = typename EB<B>::Entry;
-    __shared__ T s_val_all[WPG][C::NTILE_PAD];
+    __shared__ __align__(16) T s_val_all[WPG][C::TILE];
     __shared__ uint16_t s_vl_all[WPG][LIST];
-    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    const uint32_t lane = lane_id(), w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     T* s_val = s_val_all[w];
     uint16_t* s_vl = s_vl_all[w];
     const uint32_t na = *a.count;
@@ -1490,26 +1514,27 @@ __global__ __launch_bounds__((64 * VertCfg<B, NORMALS, T>::WPG)) void k_extract_
         const uint32_t ox = bx * B, oy = by * B, oz = bz * B;
         LaneCorners<B> lc;
         lc.load(table, nullptr, i, lane, false);
-        const bool interior = off32 && ox >= (uint32_t)A && oy >= (uint32_t)A && oz >= (uint32_t)A &&
-                              ox + B + A < X && oy + B + A < Y && oz + B + A < Z;
-        constexpr int NLD = LPL < 11 ? LPL : 11;
-        if (interior) {
+        // LDS voxel (tx, ty, tz) of the tile from (ox - A, oy - A, oz - A) sits at tx + RP * (ty + E * tz)
+        const bool direct = off32 && ox >= (uint32_t)A && oy >= (uint32_t)A && oz >= (uint32_t)A &&
+                            ox - A + RP <= X && oy + B + A < Y && oz + B + A < Z;
+        if (direct) {
+            // every segment lies inside its row of the volume
             const T* __restrict__ t0p = vol + ((ox - A) + (uint64_t)X * ((oy - A) + (uint64_t)Y * (oz - A)));
-#pragma unroll 1
-            for (int k0 = 0; k0 < LPL; k0 += NLD) {
-                T v[NLD];
+            // the lane index re-read per block (opaque to the compiler): the
+            // segment offsets are not kept live across blocks
+            uint32_t ln = lane;
+            asm volatile("" : "+v"(ln));
 #pragma unroll
-                for (int k = 0; k < NLD; ++k) {
-                    const uint32_t j = min(lane + 64u * (k0 + k), (uint32_t)NTILE - 1u);
-                    v[k] = t0p[j % E + X * ((j / E) % E + Y * (j / (E * E)))];
-                }
-#pragma unroll
-                for (int k = 0; k < NLD; ++k) {
-                    const uint32_t j = lane + 64u * (k0 + k);
-                    if (j < (uint32_t)NTILE) s_val[j] = v[k];
-                }
+            for (uint32_t k = 0; k < ND; ++k) {
+                const uint32_t m = min(ln + 64u * k, NSEG - 1u);
+                const uint32_t r = m / NS;
+                const T* src = t0p + (4u * (m % NS) + X * (r % (uint32_t)E + Y * (r / (uint32_t)E)));
+                if constexpr (C::SEGB == 16) lds_direct_load16(src, s_val + 256u * k);
+                else lds_direct_load4(src, s_val + 256u * k);
             }
+            __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the segments are in LDS
         } else {
+            constexpr int NLD = LPL < 11 ? LPL : 11;
 #pragma unroll 1
             for (int k0 = 0; k0 < LPL; k0 += NLD) {
                 T v[NLD];
@@ -1526,7 +1551,7 @@ __global__ __launch_bounds__((64 * VertCfg<B, NORMALS, T>::WPG)) void k_extract_
 #pragma unroll
                 for (int k = 0; k < NLD; ++k) {
                     const uint32_t j = lane + 64u * (k0 + k);
-                    if (j < (uint32_t)NTILE) s_val[j] = v[k];
+                    if (j < (uint32_t)NTILE) s_val[j % E + RP * (j / E)] = v[k];
                 }
             }
         }
@@ -1550,8 +1575,8 @@ __global__ __launch_bounds__((64 * VertCfg<B, NORMALS, T>::WPG)) void k_extract_
                 const uint32_t c = e & 0xFFFu, ax = e >> 12;
                 const uint32_t lx = c % B, ly = (c / B) % B, lz = c / (B * B);
                 const uint32_t x = ox + lx, y = oy + ly, z = oz + lz;
-                const uint32_t t0 = (lx + A) + E * ((ly + A) + E * (lz + A));
-                const uint32_t st = ax == 0 ? 1u : (ax == 1 ? (uint32_t)E : (uint32_t)(E * E));
+                const uint32_t t0 = (lx + A) + RP * ((ly + A) + E * (lz + A));
+                const uint32_t st = ax == 0 ? 1u : (ax == 1 ? RP : RP * E);
                 const float v0 = vfloat(s_val[t0]), v1 = vfloat(s_val[t0 + st]);
                 float t = (iso - v0) / (v1 - v0);
                 t = fminf(fmaxf(t, 0.0f), 1.0f);
@@ -1562,8 +1587,8 @@ __global__ __launch_bounds__((64 * VertCfg<B, NORMALS, T>::WPG)) void k_extract_
                 float4 nrm = make_float4(0.f, 0.f, 0.f, 0.f);
                 if (NORMALS) {
                     float n0[3], n1[3];
-                    grad_normal(s_val, t0, E, n0);
-                    grad_normal(s_val, t0 + st, E, n1);
+                    grad_normal(s_val, t0, RP, RP * E, n0);
+                    grad_normal(s_val, t0 + st, RP, RP * E, n1);
                     float m0 = n0[0] * (1.0f - t) + n1[0] * t;
                     float m1 = n0[1] * (1.0f - t) + n1[1] * t;
                     float m2 = n0[2] * (1.0f - t) + n1[2] * t;
