@@ -1619,7 +1619,7 @@ template <int B> struct TriCfg {
     static constexpr int NF = F * F * F;
     // triangles listed at a time: one pass for all but the densest blocks
     // (the kernel's registers, not its LDS, bound its occupancy)
-    static constexpr int LIST = B <= 8 ? 1024 : 512;
+    static constexpr int LIST = 512;
     // list entry: owner-corner index of the cell's (0,0,0) corner (QB bits)
     // | decoded-triangle index (case * 5 + k) << QB
     static constexpr int QB = NF <= 1024 ? 10 : 13;
@@ -1936,6 +1936,7 @@ __global__ __launch_bounds__((64 * TriCfg<B>::WPG)) void k_extract_triangles(Ext
 constexpr uint32_t kWgPerCuSingle = 8;
 constexpr uint32_t kWgPerCuSingleVertices = 10;
 constexpr uint32_t kWgPerCuSingleCount = 24;
+constexpr uint32_t kWgPerCuSingleTriangles = 10;
 constexpr uint32_t kWgPerCuBatch = 64;
 
 template <typename K>
@@ -1978,7 +1979,7 @@ static hipError_t launch_emit_b(const ExtractArgs& a, hipStream_t s)
                                           64 * VertCfg<B, false, uint8_t>::WPG, a, s, kWgPerCuSingleVertices);
     }
     if (e != hipSuccess) return e;
-    return launch_persistent(k_extract_triangles<B, BATCH>, 64 * TriCfg<B>::WPG, a, s);
+    return launch_persistent(k_extract_triangles<B, BATCH>, 64 * TriCfg<B>::WPG, a, s, kWgPerCuSingleTriangles);
 }
 
 #define MTX_DISPATCH_B(FN, ...)                                   \
