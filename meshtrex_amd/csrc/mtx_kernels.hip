@@ -1524,13 +1524,51 @@ __global__ __launch_bounds__((64 * VertCfg<B, NORMALS, T>::WPG)) void k_extract_
             // segment offsets are not kept live across blocks
             uint32_t ln = lane;
             asm volatile("" : "+v"(ln));
+            // tile rows (ty, tz) -- bit ty + E tz -- that the block's vertices
+            // read: a vertex at core corner (Y0, Z0) = (ly + 1, lz + 1) reads the
+            // rows (Y0, Z0) + {0, +-e_y, +-e_z} around each endpoint, its other
+            // endpoint being +e_x, +e_y or +e_z away: offsets (dy, dz) in
+            // {(0,0), (+-1,0), (2,0), (0,+-1), (1,+-1), (-1,1), (0,2)}.  With B = 8
+            // lane ly + 8 lz holds the corner row (ly, lz), so one ballot gives
+            // the rows with vertices; rows outside the mask are not loaded.
+            uint64_t need_lo = ~0ull, need_hi = ~0ull;
+            if constexpr (B == 8 && NORMALS) {
+                uint32_t f = 0;
+#pragma unroll
+                for (int j = 0; j < CPL; ++j) f |= lc.ent(j) & 7u;
+                const uint64_t rowsv = __ballot(f != 0u);
+                uint32_t mrow[E];
+#pragma unroll
+                for (int tz = 0; tz < E; ++tz) mrow[tz] = 0;
+#pragma unroll
+                for (int lz = 0; lz < B; ++lz) {
+                    const uint32_t a1 = (uint32_t)((rowsv >> (8 * lz)) & 0xFFu) << 1;   // ty = ly + 1
+                    mrow[lz] |= a1 | (a1 << 1);
+                    mrow[lz + 1] |= a1 | (a1 << 1) | (a1 >> 1) | (a1 << 2);
+                    mrow[lz + 2] |= a1 | (a1 << 1) | (a1 >> 1);
+                    mrow[lz + 3] |= a1;
+                }
+                need_lo = need_hi = 0;
+#pragma unroll
+                for (int tz = 0; tz < E; ++tz) {
+                    const int sh = E * tz;
+                    if (sh < 64) {
+                        need_lo |= (uint64_t)mrow[tz] << sh;
+                        if (sh + E > 64) need_hi |= (uint64_t)mrow[tz] >> (64 - sh);
+                    } else {
+                        need_hi |= (uint64_t)mrow[tz] << (sh - 64);
+                    }
+                }
+            }
 #pragma unroll
             for (uint32_t k = 0; k < ND; ++k) {
                 const uint32_t m = min(ln + 64u * k, NSEG - 1u);
                 const uint32_t r = m / NS;
                 const T* src = t0p + (4u * (m % NS) + X * (r % (uint32_t)E + Y * (r / (uint32_t)E)));
-                if constexpr (C::SEGB == 16) lds_direct_load16(src, s_val + 256u * k);
-                else lds_direct_load4(src, s_val + 256u * k);
+                if ((((r < 64u) ? need_lo >> r : need_hi >> (r - 64u)) & 1u) != 0) {
+                    if constexpr (C::SEGB == 16) lds_direct_load16(src, s_val + 256u * k);
+                    else lds_direct_load4(src, s_val + 256u * k);
+                }
             }
             __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the segments are in LDS
         } else {
