@@ -42,6 +42,29 @@ __device__ __forceinline__ void mm_merge(MMAcc& a, const MMAcc& b)
     a.xmin = min(a.xmin, b.xmin); a.xmax = max(a.xmax, b.xmax);
 }
 
+// Direct global -> LDS loads of 4 or 16 bytes per lane: lane l's bytes land at
+// dst + l * size; dst is wave-uniform.  Completion is tracked by the
+// vector-memory counter.
+__device__ __forceinline__ void lds_direct_load4(const void* src, void* dst)
+{
+    __builtin_amdgcn_global_load_lds((__attribute__((address_space(1))) void*)(uintptr_t)src,
+                                     (__attribute__((address_space(3))) void*)dst, 4, 0, 0);
+}
+__device__ __forceinline__ void lds_direct_load16(const void* src, void* dst)
+{
+    __builtin_amdgcn_global_load_lds((__attribute__((address_space(1))) void*)(uintptr_t)src,
+                                     (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
+}
+
+// 4 voxels of T per lane (16 bytes for f32, 4 for u8); the source needs no
+// alignment beyond the voxel's.
+template <typename T>
+__device__ __forceinline__ void lds_direct_load_vox4(const T* src, T* dst)
+{
+    if constexpr (sizeof(T) == 4) lds_direct_load16(src, dst);
+    else lds_direct_load4(src, dst);
+}
+
 template <typename T> struct Vec4;
 template <> struct Vec4<float> { using type = float4; };
 template <> struct Vec4<uint8_t> { using type = uchar4; };
@@ -92,7 +115,8 @@ __device__ __forceinline__ MMAcc mm_row(const uint32_t (&k)[4], uint32_t kx, uin
 template <typename T, bool ALIGNED, int M, int NB = 8>
 __device__ __forceinline__ void mm_plane(MMAcc (&out)[M], bool merge, const T* __restrict__ vol,
                                          uint64_t zoff, uint32_t X, uint32_t Y, uint32_t y0,
-                                         uint32_t x0, bool lane_ok, uint32_t xe, bool xe_ok)
+                                         uint32_t x0, bool lane_ok, uint32_t xe, bool xe_ok,
+                                         T* stage = nullptr, bool zlast = false)
 {
     uint32_t k0[4], kx0;
     mm_load<T, ALIGNED>(vol + zoff + (uint64_t)min(y0, Y - 1u) * X, X, x0, lane_ok, xe, xe_ok, k0, kx0);
@@ -102,10 +126,48 @@ __device__ __forceinline__ void mm_plane(MMAcc (&out)[M], bool merge, const T* _
         const uint32_t yb = y0 + (uint32_t)NB * b;
         const uint32_t ylast = min(yb + (uint32_t)NB, Y - 1u);
         uint32_t k[NB][4], kx[NB];
+        if (ALIGNED || X < 4u) {
 #pragma unroll
-        for (int r = 0; r < NB; ++r) {
-            const uint32_t y = min(yb + 1u + r, ylast);
-            mm_load<T, ALIGNED>(vol + zoff + (uint64_t)y * X, X, x0, lane_ok, xe, xe_ok, k[r], kx[r]);
+            for (int r = 0; r < NB; ++r) {
+                const uint32_t y = min(yb + 1u + r, ylast);
+                mm_load<T, ALIGNED>(vol + zoff + (uint64_t)y * X, X, x0, lane_ok, xe, xe_ok, k[r], kx[r]);
+            }
+        } else {
+            // Unaligned rows through LDS: one 4-voxel direct load per lane and
+            // row (the source needs only voxel alignment), read back aligned.
+            // Lanes past the row's end read its start (their voxels are
+            // neutral); the lane at the end reads on into the next row, except
+            // on the volume's last row, where it reads the row's last 4 voxels
+            // and shifts them into place.
+            const bool tail = lane_ok && x0 + 4u > X;
+#pragma unroll
+            for (int r = 0; r < NB; ++r) {
+                const uint32_t y = min(yb + 1u + r, ylast);
+                const T* row = vol + zoff + (uint64_t)y * X;
+                const bool vend = zlast && y == Y - 1u;
+                const uint32_t xs = !lane_ok ? 0u : ((tail && vend) ? X - 4u : x0);
+                lds_direct_load_vox4<T>(row + xs, stage + 256u * r);
+                kx[r] = vkey(row[xe_ok ? xe : 0u]);
+            }
+            __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the rows are in LDS
+            using V4 = typename Vec4<T>::type;
+#pragma unroll
+            for (int r = 0; r < NB; ++r) {
+                const V4 v = reinterpret_cast<const V4*>(stage + 256u * r)[lane_id()];
+                uint32_t l0 = vkey(v.x), l1 = vkey(v.y), l2 = vkey(v.z), l3 = vkey(v.w);
+                const uint32_t y = min(yb + 1u + r, ylast);
+                if (zlast && y == Y - 1u && tail) {
+                    // voxels X-4 .. X-1 were read; this lane's start at x0 - (X - 4)
+                    const uint32_t sh = x0 - (X - 4u);
+                    l0 = sh == 1u ? l1 : (sh == 2u ? l2 : l3);
+                    l1 = sh == 1u ? l2 : l3;
+                    l2 = l3;
+                }
+                k[r][0] = x0 < X ? l0 : 0xFFFFFFFFu;
+                k[r][1] = x0 + 1u < X ? l1 : 0xFFFFFFFFu;
+                k[r][2] = x0 + 2u < X ? l2 : 0xFFFFFFFFu;
+                k[r][3] = x0 + 3u < X ? l3 : 0xFFFFFFFFu;
+            }
         }
         MMAcc a = carry;
 #pragma unroll
@@ -136,7 +198,13 @@ __global__ __launch_bounds__(256) void k_minmax_b8(
     const uint32_t ygrp = rest % nygrp;
     const uint32_t cz = rest / nygrp;
     const uint32_t lane = lane_id();
-    const uint32_t w = threadIdx.x >> 6;
+    // the wave index is made uniform only for the unaligned kernel: the
+    // aligned one is measured fastest with the register allocation (2 waves
+    // per SIMD) that the plain index gives
+    const uint32_t w = ALIGNED ? threadIdx.x >> 6 : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // unaligned rows are staged through LDS, 8 rows of 256 voxels per wave
+    __shared__ __align__(16) T s_stage[ALIGNED ? 1 : 4][ALIGNED ? 1 : 8 * 256];
+    T* stage = s_stage[ALIGNED ? 0u : w];
     const uint32_t wy = 4u / wx;
     const uint32_t tx = txg * wx + w % wx;
     const uint32_t by0 = (ygrp * wy + w / wx) * (uint32_t)M;
@@ -153,7 +221,8 @@ __global__ __launch_bounds__(256) void k_minmax_b8(
     const uint64_t XY = (uint64_t)X * Y;
 
     MMAcc first[M];
-    mm_plane<T, ALIGNED, M>(first, false, vol, (uint64_t)(bz0 * 8u) * XY, X, Y, y0, x0, lane_ok, xe, xe_ok);
+    mm_plane<T, ALIGNED, M>(first, false, vol, (uint64_t)(bz0 * 8u) * XY, X, Y, y0, x0, lane_ok, xe, xe_ok,
+                            stage, bz0 * 8u == Z - 1u);
 
     const uint32_t bx = tx * 32u + (lane >> 1);
     for (uint32_t bz = bz0; bz < bz1; ++bz) {
@@ -162,10 +231,13 @@ __global__ __launch_bounds__(256) void k_minmax_b8(
         for (int b = 0; b < M; ++b) acc[b] = first[b];
         const uint32_t zl = min(bz * 8u + 8u, Z - 1u);
 #pragma unroll 1
-        for (uint32_t dz = 1; dz < 8u; ++dz)
-            mm_plane<T, ALIGNED, M>(acc, true, vol, (uint64_t)min(bz * 8u + dz, zl) * XY, X, Y, y0, x0,
-                                    lane_ok, xe, xe_ok);
-        mm_plane<T, ALIGNED, M>(first, false, vol, (uint64_t)zl * XY, X, Y, y0, x0, lane_ok, xe, xe_ok);
+        for (uint32_t dz = 1; dz < 8u; ++dz) {
+            const uint32_t z = min(bz * 8u + dz, zl);
+            mm_plane<T, ALIGNED, M>(acc, true, vol, (uint64_t)z * XY, X, Y, y0, x0, lane_ok, xe, xe_ok, stage,
+                                    z == Z - 1u);
+        }
+        mm_plane<T, ALIGNED, M>(first, false, vol, (uint64_t)zl * XY, X, Y, y0, x0, lane_ok, xe, xe_ok, stage,
+                                zl == Z - 1u);
 #pragma unroll
         for (int b = 0; b < M; ++b) {
             mm_merge(acc[b], first[b]);
@@ -261,19 +333,24 @@ static hipError_t launch_mm_b8_t(const VolDesc& v, const Grid& g, uint32_t bzb, 
                                  uint32_t* mask, const ActParams* act, uint32_t rows, hipStream_t s)
 {
     const uint32_t ntx = (g.gx + 31u) / 32u;
-    static int resident[3] = { 0, 0, 0 };
-    if (!resident[0]) {
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident[0], k_minmax_b8<T, true, MASK, 1>, 256, 0);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident[1], k_minmax_b8<T, true, MASK, 2>, 256, 0);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident[2], k_minmax_b8<T, true, MASK, 4>, 256, 0);
-        for (int& r : resident)
-            if (r < 1) r = 1;
+    const bool aligned = (v.X % 4u) == 0 && (((uintptr_t)v.data) % (4 * sizeof(T))) == 0;
+    // workgroups per CU of the aligned and the unaligned (LDS-staged) kernels
+    static int resident[2][3] = { { 0, 0, 0 }, { 0, 0, 0 } };
+    if (!resident[0][0]) {
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident[0][0], k_minmax_b8<T, true, MASK, 1>, 256, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident[0][1], k_minmax_b8<T, true, MASK, 2>, 256, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident[0][2], k_minmax_b8<T, true, MASK, 4>, 256, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident[1][0], k_minmax_b8<T, false, MASK, 1>, 256, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident[1][1], k_minmax_b8<T, false, MASK, 2>, 256, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident[1][2], k_minmax_b8<T, false, MASK, 4>, 256, 0);
+        for (auto& rr : resident)
+            for (int& r : rr)
+                if (r < 1) r = 1;
     }
     Grid gr = g;
     gr.gz = bze - bzb;  // the planned range
-    const MMPlan p = plan_minmax(gr, ntx, resident, rows);
+    const MMPlan p = plan_minmax(gr, ntx, resident[aligned ? 0 : 1], rows);
     if (p.nwg == 0 || p.nwg > 0x7FFFFFFFull) return hipErrorInvalidValue;
-    const bool aligned = (v.X % 4u) == 0 && (((uintptr_t)v.data) % (4 * sizeof(T))) == 0;
     ActParams a{};
     if (act) a = *act;
 #define MTX_MM_CASE(MM)                                                                            \
@@ -311,7 +388,7 @@ hipError_t launch_minmax_b8(const VolDesc& v, const Grid& g, uint32_t bzb, uint3
 // ---------------------------------------------------------------------------
 
 template <typename T, bool ALIGNED, int B, int M>
-__device__ __forceinline__ void mm_batch_body(const BatchVolume& V, uint32_t tile)
+__device__ __forceinline__ void mm_batch_body(const BatchVolume& V, uint32_t tile, T* stage)
 {
     constexpr uint32_t L = B / 4;       // lanes per x-block
     constexpr uint32_t BPW = 64 / L;    // x-blocks per wave segment
@@ -322,7 +399,7 @@ __device__ __forceinline__ void mm_batch_body(const BatchVolume& V, uint32_t til
     const uint32_t ygrp = rest % V.nygrp;
     const uint32_t cz = rest / V.nygrp;
     const uint32_t lane = lane_id();
-    const uint32_t w = threadIdx.x >> 6;
+    const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t wy = 4u / V.wx;
     const uint32_t tx = txg * V.wx + w % V.wx;
     const uint32_t by0 = (ygrp * wy + w / V.wx) * (uint32_t)M;
@@ -340,7 +417,8 @@ __device__ __forceinline__ void mm_batch_body(const BatchVolume& V, uint32_t til
     uint2* __restrict__ pairs = as_global(reinterpret_cast<uint2*>(V.pairs));
 
     MMAcc first[M];
-    mm_plane<T, ALIGNED, M, B>(first, false, vol, (uint64_t)(bz0 * B) * XY, X, Y, y0, x0, lane_ok, xe, xe_ok);
+    mm_plane<T, ALIGNED, M, B>(first, false, vol, (uint64_t)(bz0 * B) * XY, X, Y, y0, x0, lane_ok, xe, xe_ok,
+                               stage, bz0 * B == Z - 1u);
     const uint32_t bx = tx * BPW + lane / L;
     for (uint32_t bz = bz0; bz < bz1; ++bz) {
         MMAcc acc[M];
@@ -348,10 +426,13 @@ __device__ __forceinline__ void mm_batch_body(const BatchVolume& V, uint32_t til
         for (int b = 0; b < M; ++b) acc[b] = first[b];
         const uint32_t zl = min(bz * B + B, Z - 1u);
 #pragma unroll 1
-        for (uint32_t dz = 1; dz < (uint32_t)B; ++dz)
-            mm_plane<T, ALIGNED, M, B>(acc, true, vol, (uint64_t)min(bz * B + dz, zl) * XY, X, Y, y0, x0,
-                                       lane_ok, xe, xe_ok);
-        mm_plane<T, ALIGNED, M, B>(first, false, vol, (uint64_t)zl * XY, X, Y, y0, x0, lane_ok, xe, xe_ok);
+        for (uint32_t dz = 1; dz < (uint32_t)B; ++dz) {
+            const uint32_t z = min(bz * B + dz, zl);
+            mm_plane<T, ALIGNED, M, B>(acc, true, vol, (uint64_t)z * XY, X, Y, y0, x0, lane_ok, xe, xe_ok, stage,
+                                       z == Z - 1u);
+        }
+        mm_plane<T, ALIGNED, M, B>(first, false, vol, (uint64_t)zl * XY, X, Y, y0, x0, lane_ok, xe, xe_ok, stage,
+                                   zl == Z - 1u);
 #pragma unroll
         for (int b = 0; b < M; ++b) {
             mm_merge(acc[b], first[b]);
@@ -380,6 +461,8 @@ __global__ __launch_bounds__(256) void k_minmax_batch(const BatchVolume* __restr
                                                       const uint32_t* __restrict__ wg0, uint32_t nvol)
 {
     __shared__ uint32_t s_vol;
+    // unaligned rows are staged through LDS, B rows of 256 voxels per wave
+    __shared__ __align__(16) T s_stage[4][B * 256];
     const uint32_t wg = xcd_remap(blockIdx.x, gridDim.x);
     if (threadIdx.x < 64) {
         // the last volume whose first workgroup is <= wg (wg0 ascending, wg0[0] = 0)
@@ -394,8 +477,9 @@ __global__ __launch_bounds__(256) void k_minmax_batch(const BatchVolume* __restr
     const uint32_t v = __builtin_amdgcn_readfirstlane(s_vol);
     const BatchVolume& V = vols[v];
     const uint32_t tile = wg - V.wg0;
-    if (V.aligned) mm_batch_body<T, true, B, M>(V, tile);
-    else mm_batch_body<T, false, B, M>(V, tile);
+    T* stage = s_stage[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)];
+    if (V.aligned) mm_batch_body<T, true, B, M>(V, tile, stage);
+    else mm_batch_body<T, false, B, M>(V, tile, stage);
 }
 
 template <typename T, int B, int M>
@@ -1389,20 +1473,6 @@ __device__ __forceinline__ void grad_normal(const LT* s, uint32_t t, uint32_t sy
     if (l2 < 1e-10f) { n[0] = 0.f; n[1] = 1.f; n[2] = 0.f; return; }
     const float r = rsqrtf(l2);
     n[0] = gx * r; n[1] = gy * r; n[2] = gz * r;
-}
-
-// Direct global -> LDS loads of 4 or 16 bytes per lane: lane l's bytes land at
-// dst + l * size; dst is wave-uniform.  Completion is tracked by the
-// vector-memory counter.
-__device__ __forceinline__ void lds_direct_load4(const void* src, void* dst)
-{
-    __builtin_amdgcn_global_load_lds((__attribute__((address_space(1))) void*)(uintptr_t)src,
-                                     (__attribute__((address_space(3))) void*)dst, 4, 0, 0);
-}
-__device__ __forceinline__ void lds_direct_load16(const void* src, void* dst)
-{
-    __builtin_amdgcn_global_load_lds((__attribute__((address_space(1))) void*)(uintptr_t)src,
-                                     (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
 }
 
 // The tile is kept in LDS in the volume's own type (u8 voxels as bytes: a
