@@ -156,7 +156,7 @@ def bench_plain(args, ctx, stream):
 
     # achievable HBM rates on this GPU: a read-only stream over the 4 GiB volume
     # and a plain 1 GiB device copy (read + write), same stream
-    sink = DeviceBuffer(8192 * 4 * 4)
+    sink = DeviceBuffer(16384 * 4 * 4)
     for _ in range(2):
         assert L.synth().mtx_bench_read(vol.ptr, 4 * nvox, sink.ptr, stream) == 0
     c0, c1 = hiprt.Event(), hiprt.Event()

@@ -44,8 +44,10 @@ def run_minmax(ctx, data, dims, B, iso=0.5, **opt):
 
 # ---- block min/max -------------------------------------------------------------
 
+# x extents of every residue mod 4: unaligned rows are staged through LDS and
+# the lane at the volume's last row end takes a shift of 1, 2 or 3 voxels
 FAST_DIMS = [(64, 48, 40), (61, 33, 17), (300, 20, 26), (256, 9, 8), (8, 8, 8), (1, 1, 1),
-             (1030, 3, 5), (2, 2, 2)]
+             (1030, 3, 5), (2, 2, 2), (63, 17, 9), (7, 5, 3)]
 
 
 @pytest.mark.parametrize("dims", FAST_DIMS)
