@@ -1719,8 +1719,9 @@ __global__ __launch_bounds__((64 * VertCfg<B, NORMALS, T>::WPG)) void k_extract_
                 }
                 const uint64_t gv = vb + c0 + v;
                 float4* dst = reinterpret_cast<float4*>(a.vertices) + 2 * gv;
-                dst[0] = pos;
-                dst[1] = nrm;
+                // nontemporal: written once, read by the caller later
+                __builtin_nontemporal_store(mtx_f4v{pos.x, pos.y, pos.z, pos.w}, reinterpret_cast<mtx_f4v*>(dst));
+                __builtin_nontemporal_store(mtx_f4v{nrm.x, nrm.y, nrm.z, nrm.w}, reinterpret_cast<mtx_f4v*>(dst + 1));
                 if (a.edge_ids) a.edge_ids[gv] = 3ull * (x + (uint64_t)X * (y + (uint64_t)Y * z)) + ax;
                 if (a.edge_t) a.edge_t[gv] = t;
                 if (a.vox_pos) {
@@ -2021,7 +2022,10 @@ __global__ __launch_bounds__((64 * TriCfg<B>::WPG)) void k_extract_triangles(Ext
                         bad |= gf;
                         idx3[e3] = __popc(gf & (f >> (DB - 2)) & 3u) + s_gidx[qq];
                     }
-                    *reinterpret_cast<uint3*>(a.indices + 3 * (tb + c0 + q)) = make_uint3(idx3[0], idx3[1], idx3[2]);
+                    uint32_t* di = a.indices + 3 * (tb + c0 + q);
+                    __builtin_nontemporal_store(idx3[0], di);
+                    __builtin_nontemporal_store(idx3[1], di + 1);
+                    __builtin_nontemporal_store(idx3[2], di + 2);
                 }
                 wave_lds_sync();
             }
