@@ -16,7 +16,8 @@ namespace mtx {
 //
 // Lane l of a wave reads 4 consecutive voxels (16 B for f32, 4 B for u8) of a
 // row, so one wave load is a fully coalesced 1 KiB (f32) row segment covering
-// 32 x-blocks.  Lanes 2k, 2k+1 cover x-block k; its overlap column x = 8k+8 is
+// 32 x-blocks: a nontemporal load for aligned rows, a direct global->LDS load
+// (read back aligned) for rows that are not 16-byte aligned.  Lanes 2k, 2k+1 cover x-block k; its overlap column x = 8k+8 is
 // lane 2k+2's first voxel (the last block of the segment takes it from one
 // wave-uniform load).  A wave walks M consecutive y-blocks within each plane,
 // carrying the shared row 8(by+1) from block by into by+1, and sweeps a chunk
@@ -1698,7 +1699,9 @@ __global__ __launch_bounds__((64 * VertCfg<B, NORMALS, T>::WPG)) void k_extract_
                 const uint32_t t0 = (lx + A) + RP * ((ly + A) + E * (lz + A));
                 const uint32_t st = ax == 0 ? 1u : (ax == 1 ? RP : RP * E);
                 const float v0 = vfloat(s_val[t0]), v1 = vfloat(s_val[t0 + st]);
-                float t = (iso - v0) / (v1 - v0);
+                // reciprocal (1 ulp) instead of an IEEE division: t stays within
+                // ~2 ulp of the quotient; v1 != v0 on an active edge
+                float t = (iso - v0) * __builtin_amdgcn_rcpf(v1 - v0);
                 t = fminf(fmaxf(t, 0.0f), 1.0f);
                 const float px = (float)x + (ax == 0 ? t : 0.0f);
                 const float py = (float)y + (ax == 1 ? t : 0.0f);
