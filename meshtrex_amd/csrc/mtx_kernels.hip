@@ -1609,27 +1609,33 @@ __global__ __launch_bounds__((64 * VertCfg<B, NORMALS, T>::WPG)) void k_extract_
             asm volatile("" : "+v"(ln));
             // tile rows (ty, tz) -- bit ty + E tz -- that the block's vertices
             // read: a vertex at core corner (Y0, Z0) = (ly + 1, lz + 1) reads the
-            // rows (Y0, Z0) + {0, +-e_y, +-e_z} around each endpoint, its other
-            // endpoint being +e_x, +e_y or +e_z away: offsets (dy, dz) in
-            // {(0,0), (+-1,0), (2,0), (0,+-1), (1,+-1), (-1,1), (0,2)}.  With B = 8
-            // lane ly + 8 lz holds the corner row (ly, lz), so one ballot gives
-            // the rows with vertices; rows outside the mask are not loaded.
+            // rows plus(Y0, Z0) = (Y0, Z0) + {0, +-e_y, +-e_z} around its first
+            // endpoint, and plus(Y0 + 1, Z0) or plus(Y0, Z0 + 1) around the other
+            // for a y or z edge (an x edge stays in the row).  With B = 8 lane
+            // ly + 8 lz holds the corner row (ly, lz), so three ballots (any
+            // vertex, a y-edge vertex, a z-edge vertex) give the rows that are
+            // read; rows outside the mask are not loaded.
             uint64_t need_lo = ~0ull, need_hi = ~0ull;
             if constexpr (B == 8 && NORMALS) {
                 uint32_t f = 0;
 #pragma unroll
                 for (int j = 0; j < CPL; ++j) f |= lc.ent(j) & 7u;
-                const uint64_t rowsv = __ballot(f != 0u);
+                const uint64_t rows_a = __ballot(f != 0u);
+                const uint64_t rows_y = __ballot((f & 2u) != 0u);
+                const uint64_t rows_z = __ballot((f & 4u) != 0u);
                 uint32_t mrow[E];
 #pragma unroll
                 for (int tz = 0; tz < E; ++tz) mrow[tz] = 0;
 #pragma unroll
                 for (int lz = 0; lz < B; ++lz) {
-                    const uint32_t a1 = (uint32_t)((rowsv >> (8 * lz)) & 0xFFu) << 1;   // ty = ly + 1
-                    mrow[lz] |= a1 | (a1 << 1);
-                    mrow[lz + 1] |= a1 | (a1 << 1) | (a1 >> 1) | (a1 << 2);
-                    mrow[lz + 2] |= a1 | (a1 << 1) | (a1 >> 1);
-                    mrow[lz + 3] |= a1;
+                    // bit ty = ly + 1 of the corner rows in plane lz
+                    const uint32_t a1 = (uint32_t)((rows_a >> (8 * lz)) & 0xFFu) << 1;
+                    const uint32_t y1 = (uint32_t)((rows_y >> (8 * lz)) & 0xFFu) << 1;
+                    const uint32_t z1 = (uint32_t)((rows_z >> (8 * lz)) & 0xFFu) << 1;
+                    mrow[lz] |= a1 | (y1 << 1);
+                    mrow[lz + 1] |= a1 | (a1 << 1) | (a1 >> 1) | y1 | (y1 << 1) | (y1 << 2) | z1;
+                    mrow[lz + 2] |= a1 | (y1 << 1) | z1 | (z1 << 1) | (z1 >> 1);
+                    mrow[lz + 3] |= z1;
                 }
                 need_lo = need_hi = 0;
 #pragma unroll
