@@ -1747,8 +1747,9 @@ __global__ __launch_bounds__((64 * VertCfg<B, NORMALS, T>::WPG)) void k_extract_
 template <int B> struct TriCfg {
     static constexpr int F = B + 1;             // owner corners [0, B]^3
     static constexpr int NF = F * F * F;
-    // triangles listed at a time: one pass for all but the densest blocks
-    // (the kernel's registers, not its LDS, bound its occupancy)
+    // triangles listed at a time: 512 keeps the B = 8 workgroup at 29 KB of
+    // LDS (5 waves per SIMD, as its registers allow); blocks with more
+    // triangles (dense noisy surfaces) are listed in several passes
     static constexpr int LIST = 512;
     // list entry: owner-corner index of the cell's (0,0,0) corner (QB bits)
     // | decoded-triangle index (case * 5 + k) << QB
@@ -2060,12 +2061,12 @@ __global__ __launch_bounds__((64 * TriCfg<B>::WPG)) void k_extract_triangles(Ext
 // Persistent grids of the extraction kernels, in workgroups per CU (a
 // multiple of 8 workgroups for the XCD split, never more waves than blocks).
 // Measured on MI355X: a single volume is fastest with whole rounds of the
-// resident workgroups -- 8 per CU for the triangle pass (4 resident), 10 for
-// the vertex pass (5 resident), 24 for the count pass (8 resident, the extra
-// rounds even out the blocks' load times); fewer leave CUs idle while long
-// blocks finish, more add per-workgroup set-up.  A batch, whose blocks differ
-// far more in work (dense and sparse surfaces of many units), is fastest with
-// finer static shares, 32-128 per CU.
+// resident workgroups -- 10 per CU for the vertex and triangle passes (5
+// resident each), 24 for the count pass (8 resident, the extra rounds even out
+// the blocks' load times), 8 for the other block sizes' count pass; fewer
+// leave CUs idle while long blocks finish, more add per-workgroup set-up.  A
+// batch, whose blocks differ far more in work (dense and sparse surfaces of
+// many units), is fastest with finer static shares, 32-128 per CU.
 constexpr uint32_t kWgPerCuSingle = 8;
 constexpr uint32_t kWgPerCuSingleVertices = 10;
 constexpr uint32_t kWgPerCuSingleCount = 24;
