@@ -868,6 +868,7 @@ __global__ __launch_bounds__(256) void k_mask_scatter(const uint32_t* __restrict
 {
     __shared__ uint64_t s_red[4];
     __shared__ uint32_t s_w[4];
+    __shared__ uint32_t s_word[kMaskTileWords], s_pref[kMaskTileWords];
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
     const uint32_t tile = blockIdx.x;
     const uint64_t excl = tile_prefix<uint32_t>(tile_count, tile, s_red);
@@ -884,20 +885,26 @@ __global__ __launch_bounds__(256) void k_mask_scatter(const uint32_t* __restrict
         agg += s_w[j];
     }
     if (threadIdx.x == 0 && tile == ntiles - 1) *count = (uint32_t)(excl + agg);
-    uint64_t pos = excl + woff + inc - c;
-    if (wi < nwords) {
-        const uint64_t row = wi / wpr;
-        const uint32_t x0 = (uint32_t)(wi % wpr) * 32u;
-        const uint64_t base = row * gx + x0;
-        const uint32_t nbits = min(32u, gx - x0);
-        for (uint32_t bit = 0; bit < nbits; ++bit) {
-            uint32_t slot = 0xFFFFFFFFu;
-            if ((word >> bit) & 1u) {
-                ids[pos] = (uint32_t)(base + bit);
-                slot = (uint32_t)pos++;
-            }
-            map[base + bit] = slot;
+    s_word[threadIdx.x] = word;
+    s_pref[threadIdx.x] = woff + inc - c;   // tile-relative index of the word's first active block
+    __syncthreads();
+    // blocks of the tile in order, 256 at a time: consecutive lanes write
+    // consecutive map entries (and the active ones consecutive IDs)
+    for (uint32_t t = threadIdx.x; t < kMaskTileWords * 32u; t += 256u) {
+        const uint32_t wl = t >> 5, bit = t & 31u;
+        const uint64_t wg = (uint64_t)tile * kMaskTileWords + wl;
+        if (wg >= nwords) break;
+        const uint32_t x = (uint32_t)(wg % wpr) * 32u + bit;
+        if (x >= gx) continue;   // padding bits past the row's end
+        const uint32_t wd = s_word[wl];
+        const uint64_t blk = (wg / wpr) * gx + x;
+        uint32_t slot = 0xFFFFFFFFu;
+        if ((wd >> bit) & 1u) {
+            const uint64_t pos = excl + s_pref[wl] + (uint32_t)__popc(wd & ((1u << bit) - 1u));
+            ids[pos] = (uint32_t)blk;
+            slot = (uint32_t)pos;
         }
+        map[blk] = slot;
     }
 }
 
