@@ -708,7 +708,7 @@ hipError_t launch_pyramid(uint32_t* pairs, uint32_t levels, const uint32_t (*ext
 
 constexpr uint32_t kFilterItems = 8;                     // blocks per thread (pairs path)
 constexpr uint32_t kFilterTile = 256 * kFilterItems;     // blocks per tile (pairs path)
-constexpr uint32_t kMaskTileWords = 256;                 // mask words per tile (one per thread)
+constexpr uint32_t kMaskTileWords = 64;                  // mask words per tile (one per lane of wave 0)
 
 uint32_t filter_tiles(uint64_t nb, bool from_mask, uint32_t words_per_row, const Grid& g)
 {
@@ -854,10 +854,10 @@ hipError_t launch_filter_pairs(const uint32_t* pairs, const Grid& g, const ActPa
 __global__ __launch_bounds__(256) void k_mask_count(const uint32_t* __restrict__ mask, uint64_t nwords,
                                                     uint32_t* __restrict__ tile_count)
 {
-    __shared__ uint32_t s_red[4];
+    if (threadIdx.x >= 64) return;
     const uint64_t wi = (uint64_t)blockIdx.x * kMaskTileWords + threadIdx.x;
     const uint32_t c = wi < nwords ? (uint32_t)__popc(mask[wi]) : 0u;
-    const uint32_t t = wg_sum256<uint32_t>(c, s_red);
+    const uint32_t t = wave_sum<uint32_t>(c);
     if (threadIdx.x == 0) tile_count[blockIdx.x] = t;
 }
 
@@ -867,26 +867,20 @@ __global__ __launch_bounds__(256) void k_mask_scatter(const uint32_t* __restrict
                                                       const uint32_t* __restrict__ tile_count, uint32_t ntiles)
 {
     __shared__ uint64_t s_red[4];
-    __shared__ uint32_t s_w[4];
     __shared__ uint32_t s_word[kMaskTileWords], s_pref[kMaskTileWords];
-    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
     const uint32_t tile = blockIdx.x;
     const uint64_t excl = tile_prefix<uint32_t>(tile_count, tile, s_red);
-    const uint64_t wi = (uint64_t)tile * kMaskTileWords + threadIdx.x;
-    const uint32_t word = wi < nwords ? mask[wi] : 0u;
-    const uint32_t c = (uint32_t)__popc(word);
-    const uint32_t inc = wave_incl_scan<uint32_t>(c);
-    if (lane == 63) s_w[w] = inc;
-    __syncthreads();
-    uint32_t woff = 0, agg = 0;
-#pragma unroll
-    for (uint32_t j = 0; j < 4; ++j) {
-        woff += j < w ? s_w[j] : 0u;
-        agg += s_w[j];
+    if (threadIdx.x < 64) {
+        // wave 0: the tile's words and the tile-relative index of each word's
+        // first active block
+        const uint64_t wi = (uint64_t)tile * kMaskTileWords + threadIdx.x;
+        const uint32_t word = wi < nwords ? mask[wi] : 0u;
+        const uint32_t c = (uint32_t)__popc(word);
+        const uint32_t inc = wave_incl_scan<uint32_t>(c);
+        s_word[threadIdx.x] = word;
+        s_pref[threadIdx.x] = inc - c;
+        if (threadIdx.x == 63 && tile == ntiles - 1) *count = (uint32_t)(excl + inc);
     }
-    if (threadIdx.x == 0 && tile == ntiles - 1) *count = (uint32_t)(excl + agg);
-    s_word[threadIdx.x] = word;
-    s_pref[threadIdx.x] = woff + inc - c;   // tile-relative index of the word's first active block
     __syncthreads();
     // blocks of the tile in order, 256 at a time: consecutive lanes write
     // consecutive map entries (and the active ones consecutive IDs)
