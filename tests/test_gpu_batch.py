@@ -20,7 +20,8 @@ from meshtrex_amd.batch import Batch
 
 pytestmark = pytest.mark.gpu
 
-# uneven dimensions, none a multiple of 4; u8 blobs and f32 SDFs
+# u8 blobs and f32 SDFs; x extents of every residue mod 4 (rows staged through
+# LDS), one multiple of 4 (aligned rows) and one narrower than 4 voxels
 SPECS = [
     ((37, 45, 29), "u8", (90.5, 128.0, 200.0)),
     ((50, 33, 41), "f32", (0.0, 3.5)),
@@ -28,6 +29,8 @@ SPECS = [
     ((23, 61, 35), "f32", (-1.0, 0.0, 2.0)),
     ((5, 7, 3), "u8", (100.0,)),
     ((71, 39, 17), "f32", (1.5,)),
+    ((64, 21, 18), "f32", (0.0,)),
+    ((3, 11, 9), "u8", (128.0,)),
 ]
 
 
