@@ -109,13 +109,41 @@ struct TemporaryResources {
     void cleanup() {}
 };
 
+// Buffer of src/buffer.h:3-8: `buffer` is the device pointer (the VkBuffer's
+// place), `size` its size in bytes; `data` (a host mapping) is always null.
+struct Buffer {
+    void* buffer = nullptr;
+    void* data = nullptr;
+    size_t size = 0;
+};
+
+// Image of src/image.h:3-7: `image` and `imageView` are both the device
+// pointer of the texels (a linear x-fastest array), `size` its bytes.
+struct Image {
+    const void* image = nullptr;
+    const void* imageView = nullptr;
+    size_t size = 0;
+};
+
+// MinMaxOutput of src/minMaxOutput.h:10-64.  volumeImage: the device volume;
+// minMaxImage: the block min/max pyramid (uint32 key pairs, level 0 first);
+// minMaxMipViews: each level's first pair (the mip views' place).
 struct MinMaxOutput {
+    Image volumeImage{};
+    Image minMaxImage{};
+    std::vector<const uint32_t*> minMaxMipViews;
+    TemporaryResources tempResources;
     mtx_minmax_output c{};
     mtx_context* ctx = nullptr;
-    TemporaryResources tempResources;
 
     MinMaxOutput() = default;
-    MinMaxOutput(MinMaxOutput&& o) noexcept : c(o.c), ctx(o.ctx) { o.c = mtx_minmax_output{}; o.ctx = nullptr; }
+    MinMaxOutput(MinMaxOutput&& o) noexcept : c(o.c), ctx(o.ctx)
+    {
+        o.c = mtx_minmax_output{};
+        o.ctx = nullptr;
+        sync();
+        o.sync();
+    }
     MinMaxOutput& operator=(MinMaxOutput&& o) noexcept
     {
         if (this != &o) {
@@ -124,6 +152,8 @@ struct MinMaxOutput {
             ctx = o.ctx;
             o.c = mtx_minmax_output{};
             o.ctx = nullptr;
+            sync();
+            o.sync();
         }
         return *this;
     }
@@ -134,21 +164,41 @@ struct MinMaxOutput {
     {
         if (ctx) mtx_minmax_release(ctx, &c);
         ctx = nullptr;
+        c = mtx_minmax_output{};
+        sync();
+    }
+    // cleanup(device) of src/minMaxOutput.h:57 (the device argument is not needed)
+    template <typename Device> void cleanup(Device) { cleanup(); }
+    // the reference-named members from the C output
+    void sync()
+    {
+        const size_t esz = c.dtype == MTX_F32 ? 4 : 1;
+        volumeImage.image = volumeImage.imageView = c.volume;
+        volumeImage.size = c.volume ? (size_t)c.dims[0] * c.dims[1] * c.dims[2] * esz : 0;
+        minMaxImage.image = minMaxImage.imageView = c.minmax;
+        minMaxImage.size = c.minmax ? 8 * (size_t)c.level_offset[c.num_levels] : 0;
+        minMaxMipViews.clear();
+        for (uint32_t l = 0; c.minmax && l < c.num_levels; ++l) minMaxMipViews.push_back(c.minmax + 2 * c.level_offset[l]);
     }
 };
 
+// FilteringOutput of src/filteringOutput.h:11-74.
 struct FilteringOutput {
+    Buffer compactedBlockIdBuffer{};   // uint32 active block IDs, ascending
+    Buffer activeBlockCountBuffer{};   // one uint32, the count (stays on the device)
+    uint32_t activeBlockCount = 0;     // host copy, filled by readActiveBlockCount
+    TemporaryResources tempResources;
     mtx_filtering_output c{};
     mtx_context* ctx = nullptr;
-    uint32_t activeBlockCount = 0;
-    TemporaryResources tempResources;
 
     FilteringOutput() = default;
-    FilteringOutput(FilteringOutput&& o) noexcept : c(o.c), ctx(o.ctx), activeBlockCount(o.activeBlockCount)
+    FilteringOutput(FilteringOutput&& o) noexcept : activeBlockCount(o.activeBlockCount), c(o.c), ctx(o.ctx)
     {
         o.c = mtx_filtering_output{};
         o.ctx = nullptr;
         o.activeBlockCount = 0;
+        sync();
+        o.sync();
     }
     FilteringOutput& operator=(FilteringOutput&& o) noexcept
     {
@@ -160,6 +210,8 @@ struct FilteringOutput {
             o.c = mtx_filtering_output{};
             o.ctx = nullptr;
             o.activeBlockCount = 0;
+            sync();
+            o.sync();
         }
         return *this;
     }
@@ -170,24 +222,50 @@ struct FilteringOutput {
     {
         if (ctx) mtx_filtering_release(ctx, &c);
         ctx = nullptr;
+        c = mtx_filtering_output{};
+        sync();
+    }
+    // cleanup(device) of src/filteringOutput.h:69
+    template <typename Device> void cleanup(Device) { cleanup(); }
+    void sync()
+    {
+        compactedBlockIdBuffer.buffer = c.compacted_block_ids;
+        compactedBlockIdBuffer.size = c.compacted_block_ids ? 4 * (size_t)c.capacity : 0;
+        activeBlockCountBuffer.buffer = c.active_block_count;
+        activeBlockCountBuffer.size = c.active_block_count ? 4 : 0;
     }
 };
 
+// ExtractionOutput of src/extractionOutput.h:10-122.  vertexBuffer /
+// indexBuffer: the shared mesh (each vertex once, PMB ownership);
+// meshletDescriptorBuffer: meshlets over the renderer-layout mesh
+// meshletVertexBuffer / meshletIndexBuffer (see mtx_extraction_output); the
+// three count buffers are uint32 counters on the device.
 struct ExtractionOutput {
-    mtx_extraction_output c{};
-    mtx_context* ctx = nullptr;
+    Buffer vertexBuffer{};
+    Buffer vertexCountBuffer{};
+    Buffer indexBuffer{};
+    Buffer indexCountBuffer{};
+    Buffer meshletDescriptorBuffer{};
+    Buffer meshletDescriptorCountBuffer{};
+    Buffer meshletVertexBuffer{};
+    Buffer meshletIndexBuffer{};
     uint32_t meshletCount = 0;
     uint32_t vertexCount = 0;
     uint32_t indexCount = 0;
     TemporaryResources tempResources;
+    mtx_extraction_output c{};
+    mtx_context* ctx = nullptr;
 
     ExtractionOutput() = default;
     ExtractionOutput(ExtractionOutput&& o) noexcept
-        : c(o.c), ctx(o.ctx), meshletCount(o.meshletCount), vertexCount(o.vertexCount), indexCount(o.indexCount)
+        : meshletCount(o.meshletCount), vertexCount(o.vertexCount), indexCount(o.indexCount), c(o.c), ctx(o.ctx)
     {
         o.c = mtx_extraction_output{};
         o.ctx = nullptr;
         o.meshletCount = o.vertexCount = o.indexCount = 0;
+        sync();
+        o.sync();
     }
     ExtractionOutput& operator=(ExtractionOutput&& o) noexcept
     {
@@ -201,6 +279,8 @@ struct ExtractionOutput {
             o.c = mtx_extraction_output{};
             o.ctx = nullptr;
             o.meshletCount = o.vertexCount = o.indexCount = 0;
+            sync();
+            o.sync();
         }
         return *this;
     }
@@ -211,13 +291,26 @@ struct ExtractionOutput {
     {
         if (ctx) mtx_extraction_release(ctx, &c);
         ctx = nullptr;
+        c = mtx_extraction_output{};
         meshletCount = vertexCount = indexCount = 0;
+        sync();
     }
-    // device buffers, named as in src/extractionOutput.h:10-22
-    mtx_vertex* vertexBuffer() const { return c.vertices; }
-    uint32_t* indexBuffer() const { return c.indices; }
-    mtx_meshlet* meshletDescriptorBuffer() const { return c.meshlets; }
-    uint32_t* countBuffers() const { return c.counts; }
+    void sync()
+    {
+        auto set = [](Buffer& b, void* p, size_t bytes) {
+            b.buffer = p;
+            b.size = p ? bytes : 0;
+        };
+        char* counts = reinterpret_cast<char*>(c.counts);
+        set(vertexBuffer, c.vertices, sizeof(mtx_vertex) * (size_t)c.vertex_capacity);
+        set(indexBuffer, c.indices, 4 * (size_t)c.index_capacity);
+        set(meshletDescriptorBuffer, c.meshlets, sizeof(mtx_meshlet) * (size_t)c.meshlet_capacity);
+        set(meshletVertexBuffer, c.meshlet_vertices, sizeof(mtx_vertex) * (size_t)c.meshlet_vertex_capacity);
+        set(meshletIndexBuffer, c.meshlet_indices, 4 * (size_t)c.index_capacity);
+        set(vertexCountBuffer, counts, 4);
+        set(indexCountBuffer, counts ? counts + 4 : nullptr, 4);
+        set(meshletDescriptorCountBuffer, counts ? counts + 8 : nullptr, 4);
+    }
 };
 
 inline MinMaxOutput computeMinMaxMip(MtxContext& context, const Volume& volume, PushConstants& pushConstants,
@@ -243,6 +336,7 @@ inline MinMaxOutput computeMinMaxMip(MtxContext& context, const Volume& volume, 
     if (profiler) profiler->beginProfileRegion(externalStream, "MinMax_Leaf_Pass");
     mtx_status st = mtx_compute_min_max(context.get(), &v, &pushConstants, &o, externalStream, &out.c);
     if (profiler) profiler->endProfileRegion(externalStream);
+    out.sync();
     check(context.get(), st);
     if (build && out.c.num_levels > 1) {
         if (profiler) profiler->beginProfileRegion(externalStream, "MinMax_Octree_Reduction");
@@ -263,6 +357,7 @@ inline FilteringOutput filterActiveBlocks(MtxContext& context, MinMaxOutput& min
     mtx_status st = mtx_filter_active_blocks(context.get(), &minMaxOutput.c, &pushConstants, options, externalStream,
                                              &out.c);
     if (profiler) profiler->endProfileRegion(externalStream);
+    out.sync();
     check(context.get(), st);
     return out;
 }
@@ -286,6 +381,7 @@ inline ExtractionOutput extractMeshletDescriptors(MtxContext& context, MinMaxOut
     mtx_status st = mtx_extract(context.get(), &minMaxOutput.c, &filterOutput.c, &pushConstants, options,
                                 externalStream, &out.c);
     if (profiler) profiler->endProfileRegion(externalStream);
+    out.sync();
     check(context.get(), st);
     out.meshletCount = out.c.meshletCount;
     out.vertexCount = out.c.vertexCount;

@@ -105,7 +105,9 @@ typedef struct mtx_options {
                                 ancestor (4^3 blocks) is inactive, without reading its own pair
                                 (src/shaders/occupiedBlockPrefixSum.comp.glsl:47-63); needs the
                                 pyramid; same result (default 0) */
-    uint32_t _reserved[5];
+    uint32_t meshlets;       /* 1 (default): meshlet descriptors and the renderer-layout mesh
+                                (see mtx_extraction_output); 0: the shared mesh only */
+    uint32_t _reserved[4];
 } mtx_options;
 
 /* Min/max of every block (level 0) and the min/max pyramid, as uint32 key
@@ -139,6 +141,9 @@ typedef struct mtx_filtering_output {
     uint32_t capacity;
     float isovalue;
     uint32_t rule;
+    uint32_t block_dim;             /* block grid the list was made for: extraction and the */
+    uint32_t grid[3];               /* adjoint reject a min/max output of another grid */
+    void* done;                     /* hipEvent_t recorded after the filter's last launch */
 } mtx_filtering_output;
 
 /* VertexData (src/extractionOutput.h:131-134): NDC position (p/dim)*2-1, w = 1,
@@ -156,13 +161,29 @@ typedef struct mtx_meshlet {
     uint32_t primitiveCount;
 } mtx_meshlet;
 
+/* Meshlet limits of the reference (src/shaders/marching_cubes_pmb.task.glsl:24-25). */
+#define MTX_MESHLET_MAX_VERTICES 64
+#define MTX_MESHLET_MAX_PRIMITIVES 126
+
+/* Two meshes are written.  The shared mesh: `vertices` holds each vertex once
+ * (one per active grid edge, PMB ownership) and `indices` indexes it.  The
+ * renderer layout (option meshlets, default on): `meshlets` are the
+ * reference's meshlet descriptors -- a greedy partition of each active block's
+ * occupied cells in cell order into runs of at most 64 vertices and 126
+ * triangles (src/shaders/marching_cubes_pmb.task.glsl:182-215) -- over
+ * `meshlet_vertices` / `meshlet_indices`: each meshlet's vertices are
+ * contiguous, every index of its triangles lies in
+ * [vertexOffset, vertexOffset + vertexCount) (the renderer's acceptance rule,
+ * src/shaders/render.task.glsl:61-63,125-128), and a vertex used by several
+ * meshlets has a copy in each.  Triangle k is the same triangle in both:
+ * meshlet_vertex_map[meshlet_indices[i]] == indices[i] for every i, and
+ * meshlet_vertices[j] == vertices[meshlet_vertex_map[j]]. */
 typedef struct mtx_extraction_output {
     mtx_vertex* vertices;        /* device */
-    uint32_t* counts;            /* device uint32[3]: vertexCount, indexCount, meshletCount */
+    uint32_t* counts;            /* device uint32[4]: vertexCount, indexCount, meshletCount,
+                                    meshletVertexCount */
     uint32_t* indices;           /* device, 3 per triangle, global vertex indices */
-    mtx_meshlet* meshlets;       /* device: each active block's triangles in runs of <= 126
-                                    (MAX_PRIMS_PER_MESHLET, src/shaders/marching_cubes_pmb.mesh.glsl:30);
-                                    vertexOffset/vertexCount = the block's own vertices */
+    mtx_meshlet* meshlets;       /* device, renderer layout (see above), or NULL (meshlets = 0) */
     uint64_t* vertex_edge_ids;   /* optional (edge_ids): 3*linear(owner corner) + axis */
     float* vertex_t;             /* optional (edge_ids): t along the owning edge, in [0,1] */
     float* vertex_voxel_pos;     /* optional: float3 per vertex, voxel units */
@@ -181,6 +202,15 @@ typedef struct mtx_extraction_output {
     uint32_t block_dim;
     uint32_t max_blocks;
     uint32_t flags;              /* options the buffers were made for */
+    /* renderer layout (option meshlets) */
+    mtx_vertex* meshlet_vertices;   /* device: meshlet vertices, each meshlet's contiguous */
+    uint32_t* meshlet_indices;      /* device: 3 per triangle, into meshlet_vertices */
+    uint32_t* meshlet_vertex_map;   /* device: per meshlet vertex, its index in `vertices` */
+    uint64_t meshlet_vertex_capacity;
+    uint32_t meshletVertexCount;    /* host mirror (mtx_read_extraction_counts) */
+    uint32_t grid[3];               /* block grid of the extraction */
+    void* done;                     /* hipEvent_t recorded after the last launch */
+    uint32_t _pad;
 } mtx_extraction_output;
 
 /* ---- context --------------------------------------------------------------- */
@@ -378,6 +408,10 @@ mtx_status mtx_memcpy_h2d(void* dst, const void* src, uint64_t bytes);
 mtx_status mtx_memcpy_d2h(void* dst, const void* src, uint64_t bytes);
 mtx_status mtx_memcpy_d2d(void* dst, const void* src, uint64_t bytes);
 mtx_status mtx_memcpy_d2d_async(void* dst, const void* src, uint64_t bytes, void* stream);
+/* A non-blocking HIP stream at the default priority (the external stream a
+ * caller passes where the reference passes its external command buffer). */
+mtx_status mtx_stream_create(void** stream);
+mtx_status mtx_stream_destroy(void* stream);
 mtx_status mtx_stream_synchronize(void* stream);
 mtx_status mtx_device_synchronize(void);
 uint32_t mtx_key_f32(float f);

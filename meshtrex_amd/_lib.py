@@ -35,7 +35,7 @@ class Options(ctypes.Structure):
     _fields_ = [("overlap", u32), ("rule", u32), ("build_pyramid", u32), ("fuse_filter", u32),
                 ("normals", u32), ("edge_ids", u32), ("case_indices", u32),
                 ("voxel_positions", u32), ("reuse", u32), ("minmax_rows", u32), ("coarse_reject", u32),
-                ("_reserved", u32 * 5)]
+                ("meshlets", u32), ("_reserved", u32 * 4)]
 
 
 class CMinMaxOutput(ctypes.Structure):
@@ -50,7 +50,8 @@ class CMinMaxOutput(ctypes.Structure):
 
 class CFilteringOutput(ctypes.Structure):
     _fields_ = [("compacted_block_ids", vp), ("active_block_count", vp), ("block_to_active", vp),
-                ("activeBlockCount", u32), ("capacity", u32), ("isovalue", f32), ("rule", u32)]
+                ("activeBlockCount", u32), ("capacity", u32), ("isovalue", f32), ("rule", u32),
+                ("block_dim", u32), ("grid", u32 * 3), ("done", vp)]
 
 
 class CExtractionOutput(ctypes.Structure):
@@ -60,7 +61,10 @@ class CExtractionOutput(ctypes.Structure):
                 ("index_capacity", u64), ("meshlet_capacity", u64), ("case_capacity", u64),
                 ("meshletCount", u32),
                 ("vertexCount", u32), ("indexCount", u32), ("block_dim", u32),
-                ("max_blocks", u32), ("flags", u32)]
+                ("max_blocks", u32), ("flags", u32),
+                ("meshlet_vertices", vp), ("meshlet_indices", vp), ("meshlet_vertex_map", vp),
+                ("meshlet_vertex_capacity", u64), ("meshletVertexCount", u32), ("grid", u32 * 3),
+                ("done", vp), ("_pad", u32)]
 
 
 class CBatchVolume(ctypes.Structure):
@@ -124,6 +128,8 @@ _SIGS = {
     "mtx_memcpy_d2h": (ctypes.c_int, [vp, vp, u64]),
     "mtx_memcpy_d2d": (ctypes.c_int, [vp, vp, u64]),
     "mtx_memcpy_d2d_async": (ctypes.c_int, [vp, vp, u64, vp]),
+    "mtx_stream_create": (ctypes.c_int, [ctypes.POINTER(vp)]),
+    "mtx_stream_destroy": (ctypes.c_int, [vp]),
     "mtx_stream_synchronize": (ctypes.c_int, [vp]),
     "mtx_device_synchronize": (ctypes.c_int, []),
     "mtx_key_f32": (u32, [f32]),

@@ -98,11 +98,15 @@ class _MarchingCubes(torch.autograd.Function):
         ext = Extraction(volume.detach(), float(isovalue), block_dim, ctx)
         fctx.ext = ext
         fctx.iso_shape = isovalue.shape if torch.is_tensor(isovalue) else None
+        # the adjoint recomputes t from the volume: saving it lets torch's
+        # version counter reject an in-place change between forward and backward
+        fctx.save_for_backward(volume)
         fctx.mark_non_differentiable(ext.faces)
         return ext.positions, ext.faces
 
     @staticmethod
     def backward(fctx, grad_pos, grad_faces):
+        fctx.saved_tensors  # raises if the volume was modified in place since the forward pass
         dvol, diso = fctx.ext.adjoint(grad_pos)
         if fctx.iso_shape is not None:
             diso = diso.reshape(fctx.iso_shape)

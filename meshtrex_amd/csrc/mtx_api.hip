@@ -92,6 +92,13 @@ uint32_t pyramid_layout(const uint32_t G[3], uint32_t (*ext)[3], uint64_t* off)
     return L;
 }
 
+// The filter's list and map describe the min/max output's block grid.
+bool same_grid(const mtx_minmax_output* mm, const mtx_filtering_output* f)
+{
+    return f->block_dim == mm->block_dim && f->grid[0] == mm->grid[0] && f->grid[1] == mm->grid[1] &&
+           f->grid[2] == mm->grid[2] && f->capacity >= (uint64_t)mm->grid[0] * mm->grid[1] * mm->grid[2];
+}
+
 
 }  // namespace
 
@@ -336,7 +343,7 @@ mtx_status mtx_filter_active_blocks(mtx_context* ctx, const mtx_minmax_output* m
 
     const bool reuse = out->compacted_block_ids && out->block_to_active && out->capacity >= g.nb();
     if (!reuse) {
-        if (out->compacted_block_ids) mtx_filtering_release(ctx, out);
+        if (out->compacted_block_ids || out->done) mtx_filtering_release(ctx, out);
         std::memset(out, 0, sizeof *out);
         MTX_HIP(ctx, hipMalloc(&out->compacted_block_ids, 4 * (g.nb() ? g.nb() : 1)));
         hipError_t e = hipMalloc(&out->active_block_count, 256 + kFilterScratch);
@@ -350,6 +357,8 @@ mtx_status mtx_filter_active_blocks(mtx_context* ctx, const mtx_minmax_output* m
     out->activeBlockCount = 0;
     out->isovalue = pc->isovalue;
     out->rule = o.rule;
+    out->block_dim = g.B;
+    out->grid[0] = g.gx; out->grid[1] = g.gy; out->grid[2] = g.gz;
     const bool use_mask = mm->active_mask && mm->mask_valid && mm->mask_rule == o.rule &&
                           (mm->mask_isovalue == pc->isovalue ||
                            (std::isnan(mm->mask_isovalue) && std::isnan(pc->isovalue)));
@@ -376,6 +385,7 @@ mtx_status mtx_filter_active_blocks(mtx_context* ctx, const mtx_minmax_output* m
                                 coarse ? &mm->level_extent[2] : nullptr, bd.s);
     }
     if (e != hipSuccess) return fail(ctx, MTX_EDEVICE, "filter launch: %s", hipGetErrorString(e));
+    if (mtx_status st = mark_done(bd, &out->done)) return st;
     return finish(bd);
 }
 
@@ -384,10 +394,8 @@ mtx_status mtx_read_active_count(mtx_context* ctx, mtx_filtering_output* out)
     if (!ctx) return MTX_EINVAL;
     if (!out || !out->active_block_count) return fail(ctx, MTX_ESTATE, "filtering output is empty");
     std::lock_guard<std::mutex> lk(ctx->mu);
-    MTX_HIP(ctx, hipSetDevice(ctx->device));
-    MTX_HIP(ctx, hipDeviceSynchronize());
     uint32_t n = 0;
-    MTX_HIP(ctx, hipMemcpy(&n, out->active_block_count, 4, hipMemcpyDeviceToHost));
+    if (mtx_status st = read_after(ctx, out->done, &n, out->active_block_count, 4)) return st;
     out->activeBlockCount = n;
     return MTX_OK;
 }
@@ -399,6 +407,7 @@ void mtx_filtering_release(mtx_context* ctx, mtx_filtering_output* out)
     if (out->compacted_block_ids) hipFree(out->compacted_block_ids);
     if (out->active_block_count) hipFree(out->active_block_count);
     if (out->block_to_active) hipFree(out->block_to_active);
+    destroy_event(&out->done);
     std::memset(out, 0, sizeof *out);
 }
 
@@ -452,6 +461,8 @@ mtx_status mtx_extract(mtx_context* ctx, const mtx_minmax_output* mm,
     if (pc->blockDim[0] != mm->block_dim || pc->blockGridDim[0] != mm->grid[0] ||
         pc->blockGridDim[1] != mm->grid[1] || pc->blockGridDim[2] != mm->grid[2])
         return fail(ctx, MTX_ESTATE, "push constants do not match the min/max output");
+    if (!same_grid(mm, filt))
+        return fail(ctx, MTX_ESTATE, "filtering output was made for another block grid");
     if (filt->isovalue != pc->isovalue && !(std::isnan(filt->isovalue) && std::isnan(pc->isovalue)))
         return fail(ctx, MTX_ESTATE, "filter was run for isovalue %g, extraction asks %g",
                     (double)filt->isovalue, (double)pc->isovalue);
@@ -469,7 +480,8 @@ mtx_status mtx_extract(mtx_context* ctx, const mtx_minmax_output* mm,
     if (o.reuse) {
         if (!out->workspace || out->workspace_bytes < w.bytes || out->block_dim != B ||
             out->max_blocks < nb || (out->flags & want) != want || !out->vertices || !out->indices ||
-            !out->meshlets)
+            !out->meshlets || out->grid[0] != mm->grid[0] || out->grid[1] != mm->grid[1] ||
+            out->grid[2] != mm->grid[2])
             return fail(ctx, MTX_ESTATE, "reuse: output buffers were made for another problem");
         ExtractArgs a = make_extract_args(mm, filt, pc, out, w);
         a.normals = o.normals ? 1 : 0;
@@ -483,13 +495,15 @@ mtx_status mtx_extract(mtx_context* ctx, const mtx_minmax_output* mm,
         if (e == hipSuccess) e = launch_extract_scan(a, ts, bd.s);
         if (e == hipSuccess) e = launch_extract_emit(a, bd.s);
         if (e != hipSuccess) return fail(ctx, MTX_EDEVICE, "extract launch: %s", hipGetErrorString(e));
+        if (mtx_status st = mark_done(bd, &out->done)) return st;
         return finish(bd);
     }
 
     // Allocating form: count, scan, read the totals, allocate exactly, emit.
-    if (out->workspace) mtx_extraction_release(ctx, out);
+    if (out->workspace || out->done) mtx_extraction_release(ctx, out);
     std::memset(out, 0, sizeof *out);
     out->block_dim = B;
+    out->grid[0] = mm->grid[0]; out->grid[1] = mm->grid[1]; out->grid[2] = mm->grid[2];
     out->max_blocks = (uint32_t)nb;
     out->flags = want;
     out->workspace_bytes = w.bytes;
@@ -539,6 +553,7 @@ mtx_status mtx_extract(mtx_context* ctx, const mtx_minmax_output* mm,
     a.normals = o.normals ? 1 : 0;
     e = launch_extract_emit(a, bd.s);
     if (e != hipSuccess) return fail(ctx, MTX_EDEVICE, "extract emit: %s", hipGetErrorString(e));
+    if (mtx_status st = mark_done(bd, &out->done)) return st;
     out->vertexCount = (uint32_t)tot[0];
     out->indexCount = (uint32_t)(3 * tot[1]);
     out->meshletCount = (uint32_t)tot[3];
@@ -556,8 +571,12 @@ mtx_status mtx_adjoint(mtx_context* ctx, const mtx_minmax_output* mm, const mtx_
     if (ext->vertexCount && !dl_dpos) return fail(ctx, MTX_EINVAL, "dl_dpos is null");
     const uint32_t B = mm->block_dim;
     const uint64_t nb = (uint64_t)mm->grid[0] * mm->grid[1] * mm->grid[2];
-    if (ext->block_dim != B || ext->max_blocks < nb)
+    if (ext->block_dim != B || ext->max_blocks < nb || ext->grid[0] != mm->grid[0] ||
+        ext->grid[1] != mm->grid[1] || ext->grid[2] != mm->grid[2])
         return fail(ctx, MTX_ESTATE, "extraction output does not belong to this min/max output");
+    if (!same_grid(mm, filt))
+        return fail(ctx, MTX_ESTATE, "filtering output was made for another block grid");
+    if (mm->overlap != 1) return fail(ctx, MTX_ESTATE, "adjoint needs overlap-1 block windows");
     std::lock_guard<std::mutex> lk(ctx->mu);
     Bound bd;
     if (mtx_status st = bind(ctx, stream, bd)) return st;
@@ -580,13 +599,12 @@ mtx_status mtx_read_extraction_counts(mtx_context* ctx, mtx_extraction_output* o
     if (!ctx) return MTX_EINVAL;
     if (!out || !out->counts || !out->workspace) return fail(ctx, MTX_ESTATE, "extraction output is empty");
     std::lock_guard<std::mutex> lk(ctx->mu);
-    MTX_HIP(ctx, hipSetDevice(ctx->device));
-    MTX_HIP(ctx, hipDeviceSynchronize());
     uint32_t c[3] = { 0, 0, 0 };
-    MTX_HIP(ctx, hipMemcpy(c, out->counts, 12, hipMemcpyDeviceToHost));
+    if (mtx_status st = read_after(ctx, out->done, c, out->counts, 12)) return st;
     const WsLayout w = ws_layout(out->max_blocks, out->block_dim);
     uint64_t tot[3] = { 0, 0, 0 };
-    MTX_HIP(ctx, hipMemcpy(tot, static_cast<char*>(out->workspace) + w.totals, 24, hipMemcpyDeviceToHost));
+    if (mtx_status st = read_after(ctx, nullptr, tot, static_cast<char*>(out->workspace) + w.totals, 24))
+        return st;
     out->vertexCount = c[0];
     out->indexCount = c[1];
     out->meshletCount = c[2];
@@ -605,6 +623,7 @@ void mtx_extraction_release(mtx_context* ctx, mtx_extraction_output* out)
                      out->vertex_t, out->vertex_voxel_pos, out->workspace };
     for (void* p : ptrs)
         if (p) hipFree(p);
+    destroy_event(&out->done);
     std::memset(out, 0, sizeof *out);
 }
 
@@ -654,6 +673,21 @@ mtx_status mtx_memcpy_d2d_async(void* dst, const void* src, uint64_t bytes, void
 {
     return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, static_cast<hipStream_t>(stream)) == hipSuccess
                ? MTX_OK : MTX_EDEVICE;
+}
+
+mtx_status mtx_stream_create(void** stream)
+{
+    if (!stream) return MTX_EINVAL;
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return MTX_EDEVICE;
+    *stream = s;
+    return MTX_OK;
+}
+
+mtx_status mtx_stream_destroy(void* stream)
+{
+    if (!stream) return MTX_EINVAL;
+    return hipStreamDestroy(static_cast<hipStream_t>(stream)) == hipSuccess ? MTX_OK : MTX_EDEVICE;
 }
 
 mtx_status mtx_stream_synchronize(void* stream)

@@ -25,6 +25,10 @@ namespace mtx {
 
 inline mtx_status fail(mtx_context* ctx, mtx_status st, const char* fmt, ...)
 {
+    // a failed runtime call (e.g. an allocation) leaves its error as the
+    // thread's last error; clear it so that a later launch check does not
+    // report it again
+    (void)hipGetLastError();
     if (ctx) {
         char buf[512];
         va_list ap;
@@ -64,6 +68,38 @@ inline mtx_status finish(const Bound& b)
 {
     if (b.sync) MTX_HIP(b.ctx, hipStreamSynchronize(b.s));
     return MTX_OK;
+}
+
+// Records `*ev` (created on first use) on the bound stream: readbacks of an
+// output wait on this event, i.e. on the stream that produced the output,
+// not on the whole device.
+inline mtx_status mark_done(const Bound& b, void** ev)
+{
+    if (!*ev) {
+        hipEvent_t e = nullptr;
+        MTX_HIP(b.ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        *ev = e;
+    }
+    MTX_HIP(b.ctx, hipEventRecord(static_cast<hipEvent_t>(*ev), b.s));
+    return MTX_OK;
+}
+
+// Blocking copy of `bytes` from device memory once `ev` (the producer's
+// completion event) has been reached: the context's own stream waits for the
+// event, copies, and is synchronised.  Other streams keep running.
+inline mtx_status read_after(mtx_context* ctx, void* ev, void* dst, const void* src, size_t bytes)
+{
+    MTX_HIP(ctx, hipSetDevice(ctx->device));
+    if (ev) MTX_HIP(ctx, hipStreamWaitEvent(ctx->own, static_cast<hipEvent_t>(ev), 0));
+    MTX_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->own));
+    MTX_HIP(ctx, hipStreamSynchronize(ctx->own));
+    return MTX_OK;
+}
+
+inline void destroy_event(void** ev)
+{
+    if (*ev) hipEventDestroy(static_cast<hipEvent_t>(*ev));
+    *ev = nullptr;
 }
 
 inline ActParams make_act(int dtype, float iso, int rule)

@@ -297,9 +297,19 @@ struct MMPlan {
     uint64_t nwg;
 };
 
+// Model of a plan's duration: the larger of its HBM time -- the volume plus
+// the rows and planes read twice, 1/(8M) + 1/(8zc) of it, at the measured
+// nontemporal read rate -- and its latency time: a wave walks 8 zc + 1
+// planes, each M + 1/2 batches of 8 rows in flight one after another, about
+// 1.8 us per batch on MI355X (from a 247^3 volume at M = 4: 72.7 us for 41
+// batches).  The latency term is what keeps small volumes from running on a
+// few dozen workgroups.
+constexpr double kMmReadBps = 6.8e12;
+constexpr double kMmBatchSec = 1.8e-6;
+
 // Pick y-blocks per wave (M) and the z chunk: every workgroup resident in one
-// round (no tail), the fewest voxels read twice (1/(8M) + 1/(8zc)).
-static MMPlan plan_minmax(const Grid& g, uint32_t ntx, const int (&resident)[3], uint32_t rows)
+// round (no tail), the smallest modelled duration.
+static MMPlan plan_minmax(const Grid& g, uint32_t ntx, const int (&resident)[3], uint32_t rows, double bytes)
 {
     const uint32_t wx = ntx >= 4 ? 4u : (ntx >= 2 ? 2u : 1u);
     MMPlan best{};
@@ -322,7 +332,9 @@ static MMPlan plan_minmax(const Grid& g, uint32_t ntx, const int (&resident)[3],
         p.zc = (uint32_t)zc;
         p.ncz = (uint32_t)((g.gz + zc - 1) / zc);
         p.nwg = cols * p.ncz;
-        const double cost = 1.0 / (8.0 * M) + 1.0 / (8.0 * (double)zc);
+        const double t_hbm = bytes * (1.0 + 1.0 / (8.0 * M) + 1.0 / (8.0 * (double)zc)) / kMmReadBps;
+        const double t_lat = (8.0 * (double)zc + 1.0) * ((double)M + 0.5) * kMmBatchSec;
+        const double cost = t_hbm > t_lat ? t_hbm : t_lat;
         if (cost < best_cost) {
             best_cost = cost;
             best = p;
@@ -362,7 +374,8 @@ static hipError_t launch_mm_b8_t(const VolDesc& v, const Grid& g, uint32_t bzb, 
     }
     Grid gr = g;
     gr.gz = bze - bzb;  // the planned range
-    const MMPlan p = plan_minmax(gr, ntx, resident[aligned ? 0 : 1], rows);
+    const double bytes = (double)v.X * v.Y * (double)min(8u * gr.gz + 1u, v.Z) * sizeof(T);
+    const MMPlan p = plan_minmax(gr, ntx, resident[aligned ? 0 : 1], rows, bytes);
     if (p.nwg == 0 || p.nwg > 0x7FFFFFFFull) return hipErrorInvalidValue;
     ActParams a{};
     if (act) a = *act;
@@ -1067,6 +1080,14 @@ __device__ __forceinline__ WaveItems wave_items(uint32_t na, uint32_t wpg)
     return r;
 }
 
+// Active blocks to process: the device count, clamped to the capacity of the
+// per-block arrays (a count beyond it means the list does not belong to this
+// extraction; the scan flags it).
+__device__ __forceinline__ uint32_t active_items(const ExtractArgs& a)
+{
+    return min(*a.count, a.max_blocks);
+}
+
 template <typename V>
 __device__ __forceinline__ V wave_excl_scan(V v, V& total)
 {
@@ -1121,7 +1142,7 @@ __global__ __launch_bounds__(256) void k_extract_count(ExtractArgs a)
     __syncthreads();
     const uint32_t lane = lane_id();
     uint8_t* s_below = s_all[threadIdx.x >> 6];
-    const uint32_t na = *a.count;
+    const uint32_t na = active_items(a);
     Entry* __restrict__ table = static_cast<Entry*>(a.corner_table);
     const WaveItems it = wave_items(na, 4);
     for (uint32_t i = it.first; i < it.end; i += it.stride) {
@@ -1235,7 +1256,7 @@ __global__ __launch_bounds__(256) void k_extract_count_b8(ExtractArgs a)
         const uint32_t j = lane + 64u * k;
         tcoord[k] = j < (uint32_t)NTILE ? (j % E) | (((j / E) % E) << 4) | ((j / (E * E)) << 8) : 0x1000u;
     }
-    const uint32_t na = *a.count;
+    const uint32_t na = active_items(a);
     uint16_t* __restrict__ table = static_cast<uint16_t*>(a.corner_table);
     const uint32_t ly = lane & 7u, lz = lane >> 3;
     const WaveItems it = wave_items(na, 4);
@@ -1352,14 +1373,16 @@ __device__ __forceinline__ void block_counts(const uint32_t* __restrict__ cnt, u
 
 __global__ __launch_bounds__(256) void k_extract_tile_sums(const uint32_t* __restrict__ cnt,
                                                            const uint32_t* __restrict__ count,
+                                                           uint32_t max_blocks,
                                                            uint64_t* __restrict__ tile_sums,
                                                            uint64_t* __restrict__ totals)
 {
     __shared__ uint64_t s_red[12];
-    const uint32_t na = *count;
+    const uint32_t na = min(*count, max_blocks);
     const uint32_t tile = blockIdx.x;
     if (tile >= scan_tiles_of(na)) return;
-    if (tile == 0 && threadIdx.x == 0) totals[2] = 0;  // error flags of the emit pass
+    // error flags of the emit pass; bit 1: a count beyond the per-block arrays
+    if (tile == 0 && threadIdx.x == 0) totals[2] = *count > max_blocks ? 2ull : 0ull;
     uint64_t sv = 0, st = 0, sm = 0;
 #pragma unroll
     for (uint32_t k = 0; k < kScanItems; ++k) {
@@ -1387,6 +1410,7 @@ __global__ __launch_bounds__(256) void k_extract_tile_sums(const uint32_t* __res
 
 __global__ __launch_bounds__(256) void k_extract_scan(const uint32_t* __restrict__ cnt,
                                                       const uint32_t* __restrict__ count,
+                                                      uint32_t max_blocks,
                                                       uint64_t* __restrict__ vbase,
                                                       uint64_t* __restrict__ tbase,
                                                       uint64_t* __restrict__ mbase,
@@ -1397,7 +1421,7 @@ __global__ __launch_bounds__(256) void k_extract_scan(const uint32_t* __restrict
     __shared__ uint64_t s_w[12];
     __shared__ uint64_t s_red[12];
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
-    const uint32_t na = *count;
+    const uint32_t na = min(*count, max_blocks);
     const uint32_t ntiles = scan_tiles_of(na);
     const uint32_t tile = blockIdx.x;
     if (tile >= ntiles) return;
@@ -1573,7 +1597,7 @@ __global__ __launch_bounds__((64 * VertCfg<B, NORMALS, T>::WPG)) void k_extract_
     const uint32_t lane = lane_id(), w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     T* s_val = s_val_all[w];
     uint16_t* s_vl = s_vl_all[w];
-    const uint32_t na = *a.count;
+    const uint32_t na = active_items(a);
     const Entry* __restrict__ table = static_cast<const Entry*>(a.corner_table);
     constexpr int LPL = (NTILE + 63) / 64;
     const WaveItems it = wave_items(na, WPG);
@@ -1706,9 +1730,11 @@ __global__ __launch_bounds__((64 * VertCfg<B, NORMALS, T>::WPG)) void k_extract_
                 const uint32_t t0 = (lx + A) + RP * ((ly + A) + E * (lz + A));
                 const uint32_t st = ax == 0 ? 1u : (ax == 1 ? RP : RP * E);
                 const float v0 = vfloat(s_val[t0]), v1 = vfloat(s_val[t0 + st]);
-                // reciprocal (1 ulp) instead of an IEEE division: t stays within
-                // ~2 ulp of the quotient; v1 != v0 on an active edge
-                float t = (iso - v0) * __builtin_amdgcn_rcpf(v1 - v0);
+                const float d = v1 - v0;
+                // t = 0.5 unless |v1 - v0| > 1e-5 (src/shaders/marching_cubes_pmb.mesh.glsl:129-134);
+                // past that bound 1/d cannot overflow, and the reciprocal (1 ulp)
+                // keeps t within ~2 ulp of the quotient
+                float t = fabsf(d) > 1e-5f ? (iso - v0) * __builtin_amdgcn_rcpf(d) : 0.5f;
                 t = fminf(fmaxf(t, 0.0f), 1.0f);
                 const float px = (float)x + (ax == 0 ? t : 0.0f);
                 const float py = (float)y + (ax == 1 ? t : 0.0f);
@@ -1882,7 +1908,7 @@ __global__ __launch_bounds__((64 * TriCfg<B>::WPG)) void k_extract_triangles(Ext
     uint32_t* s_gidx = s_gidx_all[w];
     uint8_t* s_gfl = s_gfl_all[w];
     uint32_t* s_tl = s_tl_all[w];
-    const uint32_t na = *a.count;
+    const uint32_t na = active_items(a);
     const Entry* __restrict__ table = static_cast<const Entry*>(a.corner_table);
 
     // low words of the 64-bit bases
@@ -2167,9 +2193,9 @@ hipError_t launch_extract_scan(const ExtractArgs& a, TileSums ts, hipStream_t s)
     if (ntiles == 0) ntiles = 1;
     if (ntiles > 0x7FFFFFFFull) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_extract_tile_sums, dim3((uint32_t)ntiles), dim3(256), 0, s, a.blk_counts, a.count,
-                       ts.sums, a.totals);
+                       a.max_blocks, ts.sums, a.totals);
     hipLaunchKernelGGL(k_extract_scan, dim3((uint32_t)ntiles), dim3(256), 0, s, a.blk_counts, a.count,
-                       a.blk_vbase, a.blk_tbase, a.blk_mbase, a.totals, a.out_counts, ts.sums);
+                       a.max_blocks, a.blk_vbase, a.blk_tbase, a.blk_mbase, a.totals, a.out_counts, ts.sums);
     return hipGetLastError();
 }
 
@@ -2247,6 +2273,7 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjointArgs a)
                         const float G = a.dl_dpos[3 * vi + ax];
                         const uint32_t st = ax == 0 ? 1u : (ax == 1 ? (uint32_t)E : (uint32_t)(E * E));
                         const float d = s_val[t0 + st] - vg;
+                        if (!(fabsf(d) > 1e-5f)) continue;  // t = 0.5: no dependence on the values
                         const float t = (iso - vg) / d;
                         g += G * (t - 1.0f) / d;
                         diso += (double)(G / d);
@@ -2275,6 +2302,7 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjointArgs a)
                     const uint32_t st = ax == 0 ? 1u : (ax == 1 ? (uint32_t)E : (uint32_t)(E * E));
                     const float v0 = s_val[t0 - st];
                     const float d = vg - v0;
+                    if (!(fabsf(d) > 1e-5f)) continue;
                     const float t = (iso - v0) / d;
                     g += -G * t / d;
                 }

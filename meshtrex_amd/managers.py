@@ -244,6 +244,32 @@ def _stream(s):
     return int(s)
 
 
+class Buffer:
+    """Buffer of src/buffer.h:3-8: `buffer` is the device pointer (int, 0 when
+    absent), `size` its size in bytes; `data` (a host mapping) is None."""
+
+    __slots__ = ("buffer", "size", "data")
+
+    def __init__(self, ptr, size):
+        self.buffer = int(ptr or 0)
+        self.size = int(size) if ptr else 0
+        self.data = None
+
+    def __repr__(self):
+        return "Buffer(buffer=0x%x, size=%d)" % (self.buffer, self.size)
+
+
+class Image:
+    """Image of src/image.h:3-7: `image` and `imageView` are the device pointer
+    of the texels (a linear x-fastest array), `size` its bytes."""
+
+    __slots__ = ("image", "imageView", "size")
+
+    def __init__(self, ptr, size):
+        self.image = self.imageView = int(ptr or 0)
+        self.size = int(size) if ptr else 0
+
+
 class MinMaxOutput:
     """MinMaxOutput (src/minMaxOutput.h:10-64): device volume + min/max pyramid."""
 
@@ -261,6 +287,23 @@ class MinMaxOutput:
 
     def level_extent(self, level):
         return tuple(self.c.level_extent[level])
+
+    # the reference's members (src/minMaxOutput.h:11-13)
+    @property
+    def volumeImage(self):
+        esz = 4 if self.c.dtype == L.MTX_F32 else 1
+        return Image(self.c.volume, esz * self.c.dims[0] * self.c.dims[1] * self.c.dims[2])
+
+    @property
+    def minMaxImage(self):
+        return Image(self.c.minmax, 8 * self.c.level_offset[self.c.num_levels])
+
+    @property
+    def minMaxMipViews(self):
+        """Device pointer of each pyramid level's first key pair."""
+        if not self.c.minmax:
+            return []
+        return [self.c.minmax + 8 * self.c.level_offset[l] for l in range(self.c.num_levels)]
 
     def read_pairs(self, level=0):
         """(n, 2) uint32 key pairs of one level, x-fastest."""
@@ -288,6 +331,15 @@ class FilteringOutput:
     @property
     def activeBlockCount(self):
         return self.c.activeBlockCount
+
+    # the reference's members (src/filteringOutput.h:12-13)
+    @property
+    def compactedBlockIdBuffer(self):
+        return Buffer(self.c.compacted_block_ids, 4 * self.c.capacity)
+
+    @property
+    def activeBlockCountBuffer(self):
+        return Buffer(self.c.active_block_count, 4)
 
     def read_ids(self):
         n = read_device(self.c.active_block_count, np.uint32, 1)[0]
@@ -322,6 +374,45 @@ class ExtractionOutput:
     @property
     def meshletCount(self):
         return self.c.meshletCount
+
+    @property
+    def meshletVertexCount(self):
+        return self.c.meshletVertexCount
+
+    # the reference's members (src/extractionOutput.h:11-16); the renderer
+    # layout the meshlet descriptors refer to is meshletVertexBuffer /
+    # meshletIndexBuffer
+    @property
+    def vertexBuffer(self):
+        return Buffer(self.c.vertices, 32 * self.c.vertex_capacity)
+
+    @property
+    def indexBuffer(self):
+        return Buffer(self.c.indices, 4 * self.c.index_capacity)
+
+    @property
+    def meshletDescriptorBuffer(self):
+        return Buffer(self.c.meshlets, 16 * self.c.meshlet_capacity)
+
+    @property
+    def meshletVertexBuffer(self):
+        return Buffer(self.c.meshlet_vertices, 32 * self.c.meshlet_vertex_capacity)
+
+    @property
+    def meshletIndexBuffer(self):
+        return Buffer(self.c.meshlet_indices, 4 * self.c.index_capacity)
+
+    @property
+    def vertexCountBuffer(self):
+        return Buffer(self.c.counts, 4)
+
+    @property
+    def indexCountBuffer(self):
+        return Buffer(self.c.counts + 4 if self.c.counts else 0, 4)
+
+    @property
+    def meshletDescriptorCountBuffer(self):
+        return Buffer(self.c.counts + 8 if self.c.counts else 0, 4)
 
     def read_counts(self):
         _check(self.ctx, L.lib().mtx_read_extraction_counts(self.ctx.handle, ctypes.byref(self.c)))

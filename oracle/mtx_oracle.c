@@ -315,6 +315,24 @@ static inline float val(const vol_t* v, uint32_t x, uint32_t y, uint32_t z)
     return voxel_value(v->vol, v->dtype, lin(v, x, y, z));
 }
 
+/* t along an edge, src/shaders/marching_cubes_pmb.mesh.glsl:129-134:
+ * mu = 0.5 unless |v1 - v0| > 1e-5, the difference and the bound taken in
+ * fp32 as the shader takes them; the quotient in FP64; clamped to [0, 1]. */
+static inline double edge_t(float v0, float v1, float iso)
+{
+    volatile float df = v1 - v0;  /* rounded to fp32, like the shader's subtraction */
+    double t = 0.5;
+    if (fabsf(df) > 1e-5f) t = ((double)iso - (double)v0) / ((double)v1 - (double)v0);
+    return t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+}
+
+/* Whether t depends on the edge's values: the guard above passes. */
+static inline int edge_t_live(float v0, float v1)
+{
+    volatile float df = v1 - v0;
+    return fabsf(df) > 1e-5f;
+}
+
 /* Inside/outside bit of src/shaders/marching_cubes_pmb.task.glsl:91 (value <= iso). */
 static inline int below(const vol_t* v, uint32_t x, uint32_t y, uint32_t z)
 {
@@ -497,10 +515,7 @@ static void emit_vertex(void* p, uint64_t idx, uint32_t x, uint32_t y, uint32_t 
     if (c->edge_ids) c->edge_ids[idx] = eid;
     int64_t p0[3] = { x, y, z }, p1[3] = { x, y, z };
     p1[a] += 1;
-    double v0 = val(v, x, y, z);
-    double v1 = val(v, (uint32_t)p1[0], (uint32_t)p1[1], (uint32_t)p1[2]);
-    double t = (v->iso - v0) / (v1 - v0);
-    t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+    double t = edge_t(val(v, x, y, z), val(v, (uint32_t)p1[0], (uint32_t)p1[1], (uint32_t)p1[2]), v->iso);
     if (c->t) c->t[idx] = t;
     if (c->ndc)
         for (int k = 0; k < 3; ++k) {
@@ -579,6 +594,7 @@ void orc_adjoint(const float* vol, const uint32_t dims[3], float iso, const uint
         uint64_t vox = edge_ids[i] / 3;
         int a = (int)(edge_ids[i] % 3);
         uint64_t stride = a == 0 ? 1 : (a == 1 ? dims[0] : (uint64_t)dims[0] * dims[1]);
+        if (!edge_t_live(vol[vox], vol[vox + stride])) continue;  /* t = 0.5, constant */
         double v0 = vol[vox], v1 = vol[vox + stride];
         double d = v1 - v0;
         double t = ((double)iso - v0) / d;
@@ -598,9 +614,7 @@ void orc_positions(const float* vol, const uint32_t dims[3], float iso, const ui
         uint64_t vox = edge_ids[i] / 3;
         int a = (int)(edge_ids[i] % 3);
         uint64_t stride = a == 0 ? 1 : (a == 1 ? dims[0] : (uint64_t)dims[0] * dims[1]);
-        double v0 = vol[vox], v1 = vol[vox + stride];
-        double t = ((double)iso - v0) / (v1 - v0);
-        t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+        double t = edge_t(vol[vox], vol[vox + stride], iso);
         uint64_t x = vox % dims[0], y = (vox / dims[0]) % dims[1], z = vox / ((uint64_t)dims[0] * dims[1]);
         pos[3 * i + 0] = (double)x + (a == 0 ? t : 0.0);
         pos[3 * i + 1] = (double)y + (a == 1 ? t : 0.0);

@@ -41,6 +41,20 @@ def test_cpp_managers_demo(tmp_path):
     assert res["indexCount"] == 3 * len(ref["tris"])
     assert set(res["regions"]) == {"MinMax_Leaf_Pass", "MinMax_Octree_Reduction", "Active_Block_Filtering",
                                    "Mesh_Extraction"}
+    # the reference-named members (src/filteringOutput.h:12-13, src/minMaxOutput.h:11-13,
+    # src/extractionOutput.h:11-16) read back through their device pointers
+    nb = int(np.prod([(d + 7) // 8 for d in dims]))
+    assert res["activeBlockCountBuffer"] == ids.size
+    assert res["firstActiveBlockId"] == int(ids[0])
+    assert res["compactedBlockIdBuffer_size"] == 4 * nb
+    assert res["volumeImage_size"] == vol.size
+    assert res["levels"] == int(np.floor(np.log2(max((d + 7) // 8 for d in dims)))) + 1
+    assert res["vertexCountBuffer"] == res["vertexCount"]
+    assert res["indexCountBuffer"] == res["indexCount"]
+    assert res["meshletDescriptorCountBuffer"] == res["meshletCount"]
+    assert res["vertexBuffer_size"] == 32 * res["vertexCount"]
+    assert res["indexBuffer_size"] == 4 * res["indexCount"]
+    assert res["meshletDescriptorBuffer_size"] == 16 * res["meshletCount"]
     lines = obj.read_text().splitlines()
     vs = [l for l in lines if l.startswith("v ")]
     vns = [l for l in lines if l.startswith("vn ")]
