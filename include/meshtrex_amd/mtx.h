@@ -242,6 +242,22 @@ mtx_status mtx_build_pyramid(mtx_context* ctx, mtx_minmax_output* mm, void* stre
 mtx_status mtx_filter_active_blocks(mtx_context* ctx, const mtx_minmax_output* mm,
                                     const mtx_push_constants* pc, const mtx_options* opt,
                                     void* stream, mtx_filtering_output* out);
+/* mtx_filter_active_blocks into caller-provided buffers, allocating nothing:
+ * no hipMalloc, hipFree or allocating library call.  No counterpart in the
+ * reference; it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation).  Same
+ * results as mtx_filter_active_blocks: `compacted_block_ids` (room for one
+ * uint32 per block), `active_block_count` (one uint32), `block_to_active`
+ * (one uint32 per block).  The compaction's per-tile counts live in a static
+ * device array of 2^18 tiles (1 MiB on each GPU the library runs on, held
+ * until the process ends); a grid needing more tiles (over 2^29 blocks
+ * filtered from min/max pairs, or over 2^24 words of the fused activity mask)
+ * is rejected with MTX_EINVAL.  Every call on one GPU shares that array, so
+ * calls that overlap on one GPU race; nothing in it has to start at zero. */
+mtx_status mtx_filter_active_blocks_noalloc(mtx_context* ctx, const mtx_minmax_output* mm,
+                                            const mtx_push_constants* pc, const mtx_options* opt,
+                                            void* stream, uint32_t* compacted_block_ids,
+                                            uint32_t* active_block_count, uint32_t* block_to_active);
 /* Blocking 4-byte readback of the active count into out->activeBlockCount. */
 mtx_status mtx_read_active_count(mtx_context* ctx, mtx_filtering_output* out);
 void mtx_filtering_release(mtx_context* ctx, mtx_filtering_output* out);

@@ -113,6 +113,9 @@ _SIGS = {
     "mtx_filter_active_blocks": (ctypes.c_int, [vp, ctypes.POINTER(CMinMaxOutput),
                                                 ctypes.POINTER(PushConstants), ctypes.POINTER(Options),
                                                 vp, ctypes.POINTER(CFilteringOutput)]),
+    "mtx_filter_active_blocks_noalloc": (ctypes.c_int, [vp, ctypes.POINTER(CMinMaxOutput),
+                                                        ctypes.POINTER(PushConstants), ctypes.POINTER(Options),
+                                                        vp, vp, vp, vp]),
     "mtx_read_active_count": (ctypes.c_int, [vp, ctypes.POINTER(CFilteringOutput)]),
     "mtx_filtering_release": (None, [vp, ctypes.POINTER(CFilteringOutput)]),
     "mtx_extract": (ctypes.c_int, [vp, ctypes.POINTER(CMinMaxOutput), ctypes.POINTER(CFilteringOutput),

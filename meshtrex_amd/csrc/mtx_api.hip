@@ -17,6 +17,44 @@ using namespace mtx;
 // for grids of more than 2^17 tiles).
 constexpr uint64_t kFilterScratch = 4ull * 131072;
 
+// Scratch of mtx_filter_active_blocks_noalloc: the compaction's per-tile
+// active counts (one uint32 per tile, written by the count launch and read by
+// the scatter launch, so nothing has to start at zero).  2^18 tiles cover
+// grids of up to 2^29 blocks on the pairs path (2048 blocks per tile) and
+// 2^24 words of the fused mask (64 words per tile).
+constexpr uint32_t kNoallocTiles = 1u << 18;
+
+}  // namespace
+
+// One copy per GPU, held until the process ends; every call on a GPU shares
+// it, so calls that overlap on one GPU race.  In host code `scratch` is only a
+// stand-in for the symbol: its device address comes from hipGetSymbolAddress
+// (scratch_address below) and only that address reaches a kernel.
+static __device__ __align__(256) uint32_t scratch[kNoallocTiles];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices] = {};
+
+// This device's address of `scratch`, resolved once per device with the
+// device current (the first mtx_context_create on it).
+hipError_t scratch_address(int device, void** out)
+{
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch[device] = p;
+    }
+    *out = g_scratch[device];
+    return hipSuccess;
+}
+
 mtx_options defaults()
 {
     mtx_options o;
@@ -92,6 +130,44 @@ uint32_t pyramid_layout(const uint32_t G[3], uint32_t (*ext)[3], uint64_t* off)
     return L;
 }
 
+// Argument checks shared by the filter entry points; fills the options and
+// the block grid.
+mtx_status filter_checks(mtx_context* ctx, const mtx_minmax_output* mm, const mtx_push_constants* pc,
+                         const mtx_options* opt, mtx_options& o, Grid& g)
+{
+    if (!mm || !pc) return fail(ctx, MTX_EINVAL, "null argument");
+    if (!mm->minmax) return fail(ctx, MTX_ESTATE, "min/max output is empty");
+    o = opt ? *opt : defaults();
+    if (o.rule > 1) return fail(ctx, MTX_EINVAL, "unknown activity rule %u", o.rule);
+    g = Grid{ mm->block_dim, mm->grid[0], mm->grid[1], mm->grid[2] };
+    if (pc->blockGridDim[0] != g.gx || pc->blockGridDim[1] != g.gy || pc->blockGridDim[2] != g.gz ||
+        pc->blockDim[0] != g.B)
+        return fail(ctx, MTX_ESTATE, "push constants do not match the min/max output");
+    return MTX_OK;
+}
+
+// Whether the min/max pass's fused activity mask holds this isovalue and rule
+// (then the filter compacts the mask instead of re-reading the pairs).
+bool mask_usable(const mtx_minmax_output* mm, const mtx_push_constants* pc, uint32_t rule)
+{
+    return mm->active_mask && mm->mask_valid && mm->mask_rule == rule &&
+           (mm->mask_isovalue == pc->isovalue || (std::isnan(mm->mask_isovalue) && std::isnan(pc->isovalue)));
+}
+
+// The filter's two launches (tile counts, then the ordered scatter of IDs and
+// the map); the tile counts go to `ts`.
+hipError_t launch_filter(const mtx_minmax_output* mm, const Grid& g, const mtx_push_constants* pc,
+                         const mtx_options& o, bool use_mask, uint32_t* ids, uint32_t* count, uint32_t* map,
+                         TileSums ts, hipStream_t s)
+{
+    if (use_mask) return launch_compact_mask(mm->active_mask, g, mm->mask_words_per_row, ids, count, map, ts, s);
+    // coarse reject from pyramid level 2 (an ancestor's range contains its blocks')
+    const bool coarse = o.coarse_reject && mm->levels_built > 2;
+    return launch_filter_pairs(mm->minmax, g, make_act((int)mm->dtype, pc->isovalue, (int)o.rule), ids, count, map,
+                               ts, coarse ? mm->minmax + 2 * mm->level_offset[2] : nullptr,
+                               coarse ? &mm->level_extent[2] : nullptr, s);
+}
+
 // The filter's list and map describe the min/max output's block grid.
 bool same_grid(const mtx_minmax_output* mm, const mtx_filtering_output* f)
 {
@@ -160,6 +236,12 @@ mtx_status mtx_context_create(int device, mtx_context** out)
     if (!c) return MTX_ENOMEM;
     c->device = device;
     if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return MTX_EDEVICE;
+    }
+    // set-up of mtx_filter_active_blocks_noalloc: this device's scratch address
+    if (scratch_address(device, &c->scratch) != hipSuccess) {
+        hipStreamDestroy(c->own);
         delete c;
         return MTX_EDEVICE;
     }
@@ -329,14 +411,10 @@ mtx_status mtx_filter_active_blocks(mtx_context* ctx, const mtx_minmax_output* m
                                     void* stream, mtx_filtering_output* out)
 {
     if (!ctx) return MTX_EINVAL;
-    if (!mm || !pc || !out) return fail(ctx, MTX_EINVAL, "null argument");
-    if (!mm->minmax) return fail(ctx, MTX_ESTATE, "min/max output is empty");
-    const mtx_options o = opt ? *opt : defaults();
-    if (o.rule > 1) return fail(ctx, MTX_EINVAL, "unknown activity rule %u", o.rule);
-    const Grid g{ mm->block_dim, mm->grid[0], mm->grid[1], mm->grid[2] };
-    if (pc->blockGridDim[0] != g.gx || pc->blockGridDim[1] != g.gy || pc->blockGridDim[2] != g.gz ||
-        pc->blockDim[0] != g.B)
-        return fail(ctx, MTX_ESTATE, "push constants do not match the min/max output");
+    if (!out) return fail(ctx, MTX_EINVAL, "null argument");
+    mtx_options o;
+    Grid g;
+    if (mtx_status st = filter_checks(ctx, mm, pc, opt, o, g)) return st;
     std::lock_guard<std::mutex> lk(ctx->mu);
     Bound bd;
     if (mtx_status st = bind(ctx, stream, bd)) return st;
@@ -359,9 +437,7 @@ mtx_status mtx_filter_active_blocks(mtx_context* ctx, const mtx_minmax_output* m
     out->rule = o.rule;
     out->block_dim = g.B;
     out->grid[0] = g.gx; out->grid[1] = g.gy; out->grid[2] = g.gz;
-    const bool use_mask = mm->active_mask && mm->mask_valid && mm->mask_rule == o.rule &&
-                          (mm->mask_isovalue == pc->isovalue ||
-                           (std::isnan(mm->mask_isovalue) && std::isnan(pc->isovalue)));
+    const bool use_mask = mask_usable(mm, pc, o.rule);
     const uint32_t tiles = filter_tiles(g.nb(), use_mask, mm->mask_words_per_row, g);
     if (4ull * tiles > kFilterScratch) {
         // the tile counts live behind the count word; grow them for very large grids
@@ -372,20 +448,37 @@ mtx_status mtx_filter_active_blocks(mtx_context* ctx, const mtx_minmax_output* m
         out->active_block_count = big;
     }
     TileSums ts{ reinterpret_cast<uint64_t*>(reinterpret_cast<char*>(out->active_block_count) + 256) };
-    hipError_t e;
-    if (use_mask)
-        e = launch_compact_mask(mm->active_mask, g, mm->mask_words_per_row, out->compacted_block_ids,
-                                out->active_block_count, out->block_to_active, ts, bd.s);
-    else {
-        // coarse reject from pyramid level 2 (an ancestor's range contains its blocks')
-        const bool coarse = o.coarse_reject && mm->levels_built > 2;
-        e = launch_filter_pairs(mm->minmax, g, make_act((int)mm->dtype, pc->isovalue, (int)o.rule),
-                                out->compacted_block_ids, out->active_block_count, out->block_to_active, ts,
-                                coarse ? mm->minmax + 2 * mm->level_offset[2] : nullptr,
-                                coarse ? &mm->level_extent[2] : nullptr, bd.s);
-    }
+    const hipError_t e = launch_filter(mm, g, pc, o, use_mask, out->compacted_block_ids, out->active_block_count,
+                                       out->block_to_active, ts, bd.s);
     if (e != hipSuccess) return fail(ctx, MTX_EDEVICE, "filter launch: %s", hipGetErrorString(e));
     if (mtx_status st = mark_done(bd, &out->done)) return st;
+    return finish(bd);
+}
+
+mtx_status mtx_filter_active_blocks_noalloc(mtx_context* ctx, const mtx_minmax_output* mm,
+                                            const mtx_push_constants* pc, const mtx_options* opt,
+                                            void* stream, uint32_t* compacted_block_ids,
+                                            uint32_t* active_block_count, uint32_t* block_to_active)
+{
+    if (!ctx) return MTX_EINVAL;
+    if (!compacted_block_ids || !active_block_count || !block_to_active)
+        return fail(ctx, MTX_EINVAL, "null output buffer");
+    mtx_options o;
+    Grid g;
+    if (mtx_status st = filter_checks(ctx, mm, pc, opt, o, g)) return st;
+    const bool use_mask = mask_usable(mm, pc, o.rule);
+    const uint32_t tiles = filter_tiles(g.nb(), use_mask, mm->mask_words_per_row, g);
+    if (tiles > kNoallocTiles)
+        return fail(ctx, MTX_EINVAL, "grid of %llu blocks needs %u compaction tiles; the static scratch holds %u",
+                    (unsigned long long)g.nb(), tiles, kNoallocTiles);
+    if (!ctx->scratch) return fail(ctx, MTX_EDEVICE, "no scratch address for device %d", ctx->device);
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    Bound bd;
+    if (mtx_status st = bind(ctx, stream, bd)) return st;
+    const TileSums ts{ static_cast<uint64_t*>(ctx->scratch) };
+    const hipError_t e = launch_filter(mm, g, pc, o, use_mask, compacted_block_ids, active_block_count,
+                                       block_to_active, ts, bd.s);
+    if (e != hipSuccess) return fail(ctx, MTX_EDEVICE, "filter launch: %s", hipGetErrorString(e));
     return finish(bd);
 }
 

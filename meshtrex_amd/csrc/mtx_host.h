@@ -17,6 +17,7 @@ struct mtx_context {
     hipStream_t own = nullptr;
     hipStream_t copy = nullptr;           // host-to-device uploads (created on first use)
     std::vector<hipEvent_t> slab_events;  // one per uploaded slab in flight
+    void* scratch = nullptr;              // this device's address of the static scratch array
     std::string last_error;
     std::mutex mu;
 };
