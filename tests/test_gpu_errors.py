@@ -276,7 +276,7 @@ def test_tiny_and_subnormal_gaps_follow_the_reference_rule(ctx):
     t = ex.read_t().astype(np.float64)
     assert np.all(np.abs(t - ref["t"]) <= 1e-6)
     # the guarded edges: exactly 0.5
-    e = ref["edge_ids"]
+    e = ref["edge_ids"].astype(np.int64)
     vox, ax = e // 3, e % 3
     st = np.array([1, dims[0], dims[0] * dims[1]])[ax]
     d32 = (vol[vox + st] - vol[vox]).astype(np.float32)
