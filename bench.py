@@ -181,12 +181,16 @@ def bench_plain(args, ctx, stream):
     cb.free()
 
     nv, ni, nm = ex.read_counts()
+    nmv = ex.meshletVertexCount
     na = int(filt.read_ids().size)
     nb = ((S + 7) // 8) ** 3
     ntri = ni // 3
     leaf_bytes = 4 * nvox + 8 * nb
     mmf_bytes = 4 * nvox + 8 * nb + 4 * na + 4
     ext_bytes = na * 8 ** 3 * 4 + 32 * nv + 12 * ntri + 16 * nm
+    # everything the extraction writes: the shared mesh, the meshlet descriptors
+    # and the renderer layout (meshlet vertices, their map, meshlet indices)
+    ext_out_bytes = 32 * nv + 12 * ntri + 16 * nm + (32 + 4) * nmv + 12 * ntri
     leaf_gbs = leaf_bytes / (stages["minmax_leaf"] * 1e-3) / 1e9
     mmf_gbs = mmf_bytes / ((stages["minmax_leaf"] + stages["filter"]) * 1e-3) / 1e9
     res = {
@@ -195,7 +199,7 @@ def bench_plain(args, ctx, stream):
                                "N(0,0.05^2)), isovalue 0, 8^3-cell blocks; min-max + pyramid + "
                                "filter + PMB extraction with normals and meshlets" % S,
                    "volume_bytes": 4 * nvox, "blocks": nb, "active_blocks": na,
-                   "vertices": nv, "triangles": ntri, "meshlets": nm},
+                   "vertices": nv, "triangles": ntri, "meshlets": nm, "meshlet_vertices": nmv},
         "roofline": {"bound": "hbm", "achieved": round(leaf_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(leaf_gbs / HBM_PEAK_GBS, 4), "traffic": parse_traffic(),
                      "kernel": "k_minmax_b8<float> (block min/max + fused activity mask)",
@@ -209,7 +213,10 @@ def bench_plain(args, ctx, stream):
                                    "GBps": round(mmf_gbs, 1), "frac_of_peak": round(mmf_gbs / HBM_PEAK_GBS, 4),
                                    "target_frac": 0.70},
             "extraction": {"algorithmic_bytes": ext_bytes, "ms": round(stages["extract"], 4),
-                           "GBps": round(ext_bytes / (stages["extract"] * 1e-3) / 1e9, 1)},
+                           "GBps": round(ext_bytes / (stages["extract"] * 1e-3) / 1e9, 1),
+                           "bytes_in_plus_all_outputs": na * 8 ** 3 * 4 + ext_out_bytes,
+                           "GBps_all_outputs": round((na * 8 ** 3 * 4 + ext_out_bytes) /
+                                                     (stages["extract"] * 1e-3) / 1e9, 1)},
             "triangles_per_s": round(ntri / (ms * 1e-3), 1),
             "wall_s_timed": round(wall, 4),
             "generate_s": round(gen_s, 3),
@@ -309,7 +316,8 @@ def bench_batched(args, ctx, stream):
     # the same units through the single-volume path, preallocated, stream-ordered
     lib, h = L.lib(), ctx.handle
     o_mm = makeOptions(build_pyramid=0, reuse=1)
-    o_ex_alloc, o_ex = makeOptions(), makeOptions(reuse=1)
+    # the batch writes the shared mesh only (no meshlets): so do these
+    o_ex_alloc, o_ex = makeOptions(meshlets=0), makeOptions(reuse=1, meshlets=0)
     plan = []
     for buf, vol, dims, kind, isos in vols:
         cv = vol._c()
@@ -407,8 +415,8 @@ def bench_adjoint(args, ctx):
     specs = workloads.differentiable_specs(args.adjoint_volumes)
     items = []
     o_mm = makeOptions(build_pyramid=0, reuse=1)
-    o_alloc = makeOptions(voxel_positions=1, normals=0)
-    o_ex = makeOptions(voxel_positions=1, normals=0, reuse=1)
+    o_alloc = makeOptions(voxel_positions=1, normals=0, meshlets=0)
+    o_ex = makeOptions(voxel_positions=1, normals=0, reuse=1, meshlets=0)
     for dims, seed in specs:
         vol = torch.empty(dims[::-1], dtype=torch.float32, device=dev)
         d3 = (L.u32 * 3)(*dims)

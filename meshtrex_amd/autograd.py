@@ -58,7 +58,7 @@ class Extraction:
                                                  ctypes.byref(self.mm.c)))
         _check(self.ctx, lib.mtx_filter_active_blocks(h, ctypes.byref(self.mm.c), ctypes.byref(self.pc), None, s,
                                                       ctypes.byref(self.filt.c)))
-        opts = makeOptions(voxel_positions=1, edge_ids=int(edge_ids), normals=int(normals))
+        opts = makeOptions(voxel_positions=1, edge_ids=int(edge_ids), normals=int(normals), meshlets=0)
         _check(self.ctx, lib.mtx_extract(h, ctypes.byref(self.mm.c), ctypes.byref(self.filt.c),
                                          ctypes.byref(self.pc), ctypes.byref(opts), s, ctypes.byref(self.ex.c)))
         nv, ni = self.ex.c.vertexCount, self.ex.c.indexCount

@@ -148,4 +148,34 @@ static inline MTX_HD unsigned mtx_case_edge_mask(unsigned c)
     return m;
 }
 
+/* Owner index of an edge, 0..11: u = 4 axis + d_p + 2 d_q, where (p, q) are
+ * the two axes other than the edge's, in increasing order, and d its owner
+ * corner's offset from the cell's minimum corner.  A cell's meshlet vertices
+ * are taken in ascending u (the order of owner corners along the axes). */
+static inline MTX_CE MTX_HD unsigned mtx_edge_uidx(unsigned e)
+{
+    unsigned o = mtx_edge_owner(e);
+    unsigned a = o >> 3;
+    unsigned p = a == 0u ? 1u : 0u, q = a == 2u ? 1u : 2u;
+    return 4u * a + ((o >> p) & 1u) + 2u * ((o >> q) & 1u);
+}
+
+/* The edge of owner index u (inverse of mtx_edge_uidx). */
+static inline MTX_CE MTX_HD unsigned mtx_uidx_edge(unsigned u)
+{
+    unsigned e = 0;
+    for (unsigned k = 0; k < 12u; ++k)
+        if (mtx_edge_uidx(k) == u) e = k;
+    return e;
+}
+
+/* Active edges of case c as owner-index bits (bit u). */
+static inline MTX_HD unsigned mtx_case_umask(unsigned c)
+{
+    unsigned m = 0;
+    for (unsigned e = 0; e < 12u; ++e)
+        m |= (((c >> mtx_edge_c0(e)) ^ (c >> mtx_edge_c1(e))) & 1u) << mtx_edge_uidx(e);
+    return m;
+}
+
 #endif /* MTX_MC_CASES_H */

@@ -17,6 +17,9 @@ using namespace mtx;
 // for grids of more than 2^17 tiles).
 constexpr uint64_t kFilterScratch = 4ull * 131072;
 
+// mtx_extraction_output.flags bit: the buffers include the meshlet outputs.
+constexpr uint32_t kWantMeshlets = 16u;
+
 // Scratch of mtx_filter_active_blocks_noalloc: the compaction's per-tile
 // active counts (one uint32 per tile, written by the count launch and read by
 // the scatter launch, so nothing has to start at zero).  2^18 tiles cover
@@ -206,6 +209,7 @@ void mtx_options_default(mtx_options* o)
     o->build_pyramid = 1;
     o->fuse_filter = 1;
     o->normals = 1;
+    o->meshlets = 1;
 }
 
 void mtx_push_constants_init(mtx_push_constants* pc, const uint32_t dims[3], uint32_t block_dim,
@@ -522,6 +526,7 @@ static ExtractArgs make_extract_args(const mtx_minmax_output* mm, const mtx_filt
     a.blk_vbase = reinterpret_cast<uint64_t*>(ws + w.vbase);
     a.blk_tbase = reinterpret_cast<uint64_t*>(ws + w.tbase);
     a.blk_mbase = reinterpret_cast<uint64_t*>(ws + w.mbase);
+    a.blk_sbase = reinterpret_cast<uint64_t*>(ws + w.sbase);
     a.corner_table = ws + w.table;
     a.totals = reinterpret_cast<uint64_t*>(ws + w.totals);
     a.cases = reinterpret_cast<uint8_t*>(ws + w.cases);
@@ -530,10 +535,18 @@ static ExtractArgs make_extract_args(const mtx_minmax_output* mm, const mtx_filt
     a.edge_ids = out->vertex_edge_ids;
     a.vox_pos = out->vertex_voxel_pos;
     a.edge_t = out->vertex_t;
-    a.meshlets = reinterpret_cast<uint4*>(out->meshlets);
+    // meshlets when the output was made for them
+    if (out->flags & kWantMeshlets) {
+        a.starts = reinterpret_cast<uint32_t*>(ws + w.starts);
+        a.meshlets = reinterpret_cast<uint4*>(out->meshlets);
+        a.mverts = reinterpret_cast<float*>(out->meshlet_vertices);
+        a.mindices = out->meshlet_indices;
+        a.mvmap = out->meshlet_vertex_map;
+        a.mcap = out->meshlet_capacity;
+        a.mvcap = out->meshlet_vertex_capacity;
+    }
     a.vcap = out->vertex_capacity;
     a.icap = out->index_capacity;
-    a.mcap = out->meshlet_capacity;
     a.out_counts = out->counts;
     a.max_blocks = out->max_blocks;
     return a;
@@ -563,7 +576,7 @@ mtx_status mtx_extract(mtx_context* ctx, const mtx_minmax_output* mm,
     const uint32_t B = mm->block_dim;
     const uint64_t nb = (uint64_t)mm->grid[0] * mm->grid[1] * mm->grid[2];
     const uint32_t want = (o.normals ? 1u : 0u) | (o.edge_ids ? 2u : 0u) |
-                          (o.voxel_positions ? 8u : 0u);
+                          (o.voxel_positions ? 8u : 0u) | (o.meshlets ? kWantMeshlets : 0u);
     const WsLayout w = ws_layout(nb, B);
 
     std::lock_guard<std::mutex> lk(ctx->mu);
@@ -573,10 +586,11 @@ mtx_status mtx_extract(mtx_context* ctx, const mtx_minmax_output* mm,
     if (o.reuse) {
         if (!out->workspace || out->workspace_bytes < w.bytes || out->block_dim != B ||
             out->max_blocks < nb || (out->flags & want) != want || !out->vertices || !out->indices ||
-            !out->meshlets || out->grid[0] != mm->grid[0] || out->grid[1] != mm->grid[1] ||
-            out->grid[2] != mm->grid[2])
+            out->grid[0] != mm->grid[0] || out->grid[1] != mm->grid[1] || out->grid[2] != mm->grid[2])
             return fail(ctx, MTX_ESTATE, "reuse: output buffers were made for another problem");
-        ExtractArgs a = make_extract_args(mm, filt, pc, out, w);
+        mtx_extraction_output view = *out;   // this call's options decide what is written
+        if (!o.meshlets) view.flags &= ~kWantMeshlets;
+        ExtractArgs a = make_extract_args(mm, filt, pc, &view, w);
         a.normals = o.normals ? 1 : 0;
         if (!o.edge_ids) {
             a.edge_ids = nullptr;
@@ -616,14 +630,14 @@ mtx_status mtx_extract(mtx_context* ctx, const mtx_minmax_output* mm,
     e = hipMemsetAsync(out->counts, 0, 256, bd.s);
     if (e == hipSuccess) e = launch_extract_count(a, bd.s);
     if (e == hipSuccess) e = launch_extract_scan(a, ts, bd.s);
-    uint64_t tot[4] = { 0, 0, 0, 0 };
-    if (e == hipSuccess) e = hipMemcpyAsync(tot, a.totals, 32, hipMemcpyDeviceToHost, bd.s);
+    uint64_t tot[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
+    if (e == hipSuccess) e = hipMemcpyAsync(tot, a.totals, 64, hipMemcpyDeviceToHost, bd.s);
     if (e == hipSuccess) e = hipStreamSynchronize(bd.s);
     if (e != hipSuccess) {
         mtx_extraction_release(ctx, out);
         return fail(ctx, MTX_EDEVICE, "extract count/scan: %s", hipGetErrorString(e));
     }
-    if (tot[0] > 0xFFFFFFFFull || 3 * tot[1] > 0xFFFFFFFFull) {
+    if (tot[0] > 0xFFFFFFFFull || 3 * tot[1] > 0xFFFFFFFFull || tot[4] > 0xFFFFFFFFull) {
         mtx_extraction_release(ctx, out);
         return fail(ctx, MTX_ECAPACITY, "mesh too large for 32-bit indices (%llu vertices, %llu triangles)",
                     (unsigned long long)tot[0], (unsigned long long)tot[1]);
@@ -631,9 +645,13 @@ mtx_status mtx_extract(mtx_context* ctx, const mtx_minmax_output* mm,
     out->vertex_capacity = tot[0];
     out->index_capacity = 3 * tot[1];
     out->meshlet_capacity = tot[3];
+    out->meshlet_vertex_capacity = tot[4];
     e = hipMalloc(&out->vertices, sizeof(mtx_vertex) * (tot[0] ? tot[0] : 1));
-    if (e == hipSuccess) e = hipMalloc(&out->meshlets, sizeof(mtx_meshlet) * (tot[3] ? tot[3] : 1));
     if (e == hipSuccess) e = hipMalloc(&out->indices, 4 * (tot[1] ? 3 * tot[1] : 1));
+    if (e == hipSuccess && o.meshlets) e = hipMalloc(&out->meshlets, sizeof(mtx_meshlet) * (tot[3] ? tot[3] : 1));
+    if (e == hipSuccess && o.meshlets) e = hipMalloc(&out->meshlet_vertices, sizeof(mtx_vertex) * (tot[4] ? tot[4] : 1));
+    if (e == hipSuccess && o.meshlets) e = hipMalloc(&out->meshlet_vertex_map, 4 * (tot[4] ? tot[4] : 1));
+    if (e == hipSuccess && o.meshlets) e = hipMalloc(&out->meshlet_indices, 4 * (tot[1] ? 3 * tot[1] : 1));
     if (e == hipSuccess && o.edge_ids) e = hipMalloc(&out->vertex_edge_ids, 8 * (tot[0] ? tot[0] : 1));
     if (e == hipSuccess && o.edge_ids) e = hipMalloc(&out->vertex_t, 4 * (tot[0] ? tot[0] : 1));
     if (e == hipSuccess && o.voxel_positions) e = hipMalloc(&out->vertex_voxel_pos, 12 * (tot[0] ? tot[0] : 1));
@@ -650,6 +668,7 @@ mtx_status mtx_extract(mtx_context* ctx, const mtx_minmax_output* mm,
     out->vertexCount = (uint32_t)tot[0];
     out->indexCount = (uint32_t)(3 * tot[1]);
     out->meshletCount = (uint32_t)tot[3];
+    out->meshletVertexCount = (uint32_t)tot[4];
     return finish(bd);
 }
 
@@ -692,18 +711,23 @@ mtx_status mtx_read_extraction_counts(mtx_context* ctx, mtx_extraction_output* o
     if (!ctx) return MTX_EINVAL;
     if (!out || !out->counts || !out->workspace) return fail(ctx, MTX_ESTATE, "extraction output is empty");
     std::lock_guard<std::mutex> lk(ctx->mu);
-    uint32_t c[3] = { 0, 0, 0 };
-    if (mtx_status st = read_after(ctx, out->done, c, out->counts, 12)) return st;
+    uint32_t c[4] = { 0, 0, 0, 0 };
+    if (mtx_status st = read_after(ctx, out->done, c, out->counts, 16)) return st;
     const WsLayout w = ws_layout(out->max_blocks, out->block_dim);
-    uint64_t tot[3] = { 0, 0, 0 };
-    if (mtx_status st = read_after(ctx, nullptr, tot, static_cast<char*>(out->workspace) + w.totals, 24))
+    uint64_t tot[5] = { 0, 0, 0, 0, 0 };
+    if (mtx_status st = read_after(ctx, nullptr, tot, static_cast<char*>(out->workspace) + w.totals, 40))
         return st;
     out->vertexCount = c[0];
     out->indexCount = c[1];
     out->meshletCount = c[2];
+    out->meshletVertexCount = c[3];
     if (tot[2] & 2) return fail(ctx, MTX_ESTATE, "active block list does not match the volume");
-    if (tot[2] & 1) return fail(ctx, MTX_ECAPACITY, "output buffers too small: need %llu vertices, %llu indices",
-                                (unsigned long long)tot[0], (unsigned long long)(3 * tot[1]));
+    if (tot[2] & 4) return fail(ctx, MTX_EDEVICE, "meshlet cut of the triangle pass differs from the count pass");
+    if (tot[2] & 1)
+        return fail(ctx, MTX_ECAPACITY,
+                    "output buffers too small: need %llu vertices, %llu indices, %llu meshlets, %llu meshlet vertices",
+                    (unsigned long long)tot[0], (unsigned long long)(3 * tot[1]), (unsigned long long)tot[3],
+                    (unsigned long long)tot[4]);
     return MTX_OK;
 }
 
@@ -713,7 +737,8 @@ void mtx_extraction_release(mtx_context* ctx, mtx_extraction_output* out)
     if (ctx) hipSetDevice(ctx->device);
     // case_indices points into the workspace
     void* ptrs[] = { out->vertices, out->counts, out->indices, out->meshlets, out->vertex_edge_ids,
-                     out->vertex_t, out->vertex_voxel_pos, out->workspace };
+                     out->vertex_t, out->vertex_voxel_pos, out->workspace, out->meshlet_vertices,
+                     out->meshlet_indices, out->meshlet_vertex_map };
     for (void* p : ptrs)
         if (p) hipFree(p);
     destroy_event(&out->done);

@@ -244,6 +244,7 @@ hipError_t class_enqueue(const ClassPlan& c, hipStream_t s)
     a.blk_vbase = reinterpret_cast<uint64_t*>(c.d_ws + c.w.vbase);
     a.blk_tbase = reinterpret_cast<uint64_t*>(c.d_ws + c.w.tbase);
     a.blk_mbase = reinterpret_cast<uint64_t*>(c.d_ws + c.w.mbase);
+    a.blk_sbase = reinterpret_cast<uint64_t*>(c.d_ws + c.w.sbase);
     a.corner_table = c.d_ws + c.w.table;
     a.cases = reinterpret_cast<uint8_t*>(c.d_ws + c.w.cases);
     a.totals = reinterpret_cast<uint64_t*>(c.d_ws + c.w.totals);

@@ -66,6 +66,19 @@ __device__ __forceinline__ V wave_incl_scan(V v)
     return v;
 }
 
+// Maximum of v over the lanes before this one (0 for lane 0).
+__device__ __forceinline__ uint32_t wave_excl_max(uint32_t v)
+{
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t t = __shfl_up(v, d, 64);
+        if (lane >= (uint32_t)d) v = max(v, t);
+    }
+    const uint32_t p = __shfl_up(v, 1, 64);
+    return lane == 0 ? 0u : p;
+}
+
 template <typename V>
 __device__ __forceinline__ V wave_sum(V v)
 {

@@ -119,26 +119,31 @@ inline uint64_t align256(uint64_t x) { return (x + 255) & ~255ull; }
 
 // Extraction workspace layout (offsets in bytes).
 struct WsLayout {
-    uint64_t counts, vbase, tbase, mbase, table, cases, adj, totals, scan, bytes;
+    uint64_t counts, vbase, tbase, mbase, sbase, table, cases, starts, adj, totals, scan, bytes;
     uint32_t scan_tiles;
 };
+
+// Words of a block's meshlet start bits (one bit per cell).
+inline uint64_t start_words(uint32_t B) { return ((uint64_t)B * B * B + 31) / 32; }
 
 inline WsLayout ws_layout(uint64_t max_blocks, uint32_t B)
 {
     WsLayout w{};
     uint64_t o = 0;
-    w.counts = o; o = align256(o + 8 * max_blocks);
+    w.counts = o; o = align256(o + 16 * max_blocks);   // {vertices, triangles, meshlets, meshlet vertices}
     w.vbase = o;  o = align256(o + 8 * max_blocks);
     w.tbase = o;  o = align256(o + 8 * max_blocks);
     w.mbase = o;  o = align256(o + 8 * max_blocks);
+    w.sbase = o;  o = align256(o + 8 * max_blocks);
     w.table = o;  o = align256(o + corner_entry_bytes(B) * (uint64_t)B * B * B * max_blocks);
     w.cases = o;  o = align256(o + (uint64_t)B * B * B * max_blocks);
+    w.starts = o; o = align256(o + 4 * start_words(B) * max_blocks);
     w.adj = o;    o = align256(o + 8ull * 4 * adjoint_workgroups(max_blocks));
-    w.totals = o; o = align256(o + 8 * 4);
+    w.totals = o; o = align256(o + 8 * 8);
     uint64_t tiles = (max_blocks + scan_tile_items() - 1) / scan_tile_items();
     if (tiles == 0) tiles = 1;
     w.scan_tiles = (uint32_t)tiles;
-    w.scan = o;   o = align256(o + 8 * (3 * tiles + 1));
+    w.scan = o;   o = align256(o + 8 * (4 * tiles + 1));
     w.bytes = o;
     return w;
 }

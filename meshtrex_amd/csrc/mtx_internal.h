@@ -140,12 +140,16 @@ struct ExtractArgs {
     const uint32_t* ids;        // active block IDs (ascending)
     const uint32_t* count;      // device active count
     const uint32_t* blk_map;    // dense block ID -> active index (0xFFFFFFFF: inactive)
-    uint32_t* blk_counts;       // per active block: {nv, nt}
+    uint32_t* blk_counts;       // per active block: {vertices, triangles, meshlets, meshlet vertices}
     void* corner_table;         // per active block: B^3 entries (prefix<<3 | flags)
     uint64_t* blk_vbase;        // per active block vertex base
     uint64_t* blk_tbase;        // per active block triangle base
     uint64_t* blk_mbase;        // per active block meshlet base
+    uint64_t* blk_sbase;        // per active block meshlet-vertex base
+    uint32_t* starts;           // per active block: meshlet start bits, one per cell ((B^3+31)/32
+                                // words); nullptr: no meshlets
     uint64_t* totals;           // [0]=vertices [1]=triangles [2]=error flags [3]=meshlets
+                                // [4]=meshlet vertices
     uint8_t* cases;             // per active block: B^3 case bytes (0 for cells outside the volume)
     // outputs
     float* vertices;            // 8 floats per vertex (position xyzw, normal xyzw)
@@ -153,8 +157,12 @@ struct ExtractArgs {
     uint64_t* edge_ids;         // optional
     float* vox_pos;             // optional: 3 floats per vertex, voxel units
     float* edge_t;              // optional: t along the owning edge
-    uint4* meshlets;            // optional: {vertexOffset, indexOffset, vertexCount, primitiveCount}
+    uint4* meshlets;            // with starts: {vertexOffset, indexOffset, vertexCount, primitiveCount}
+    float* mverts;              // with starts: meshlet vertices, 8 floats each
+    uint32_t* mindices;         // with starts: 3 per triangle, into mverts
+    uint32_t* mvmap;            // with starts: per meshlet vertex, its vertex index
     uint64_t vcap, icap, mcap;  // capacities (vertices, indices, meshlets)
+    uint64_t mvcap;             // capacity of mverts / mvmap
     uint32_t* out_counts;       // optional device uint32[3]: vertexCount, indexCount, meshletCount
     int normals;
     uint32_t max_blocks;        // capacity of the per-block arrays

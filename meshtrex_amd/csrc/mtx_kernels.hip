@@ -1041,9 +1041,6 @@ template <int B> struct EB {
 bool extract_supported_block(uint32_t B) { return B >= 1 && B <= 16; }
 size_t corner_entry_bytes(uint32_t B) { return B <= 13 ? 2 : 4; }
 
-// Triangles per meshlet descriptor: MAX_PRIMS_PER_MESHLET of
-// src/shaders/marching_cubes_pmb.mesh.glsl:30.
-constexpr uint32_t kMeshletPrims = 126;
 
 // Orders LDS accesses between the lanes of one wave: the LDS unit executes a
 // wave's instructions in program order, so only compiler reordering must be
@@ -1126,6 +1123,271 @@ __device__ __forceinline__ ItemVol item_vol(const ExtractArgs& a, uint32_t i, ui
         r.map = a.blk_map;
     }
     return r;
+}
+
+// ---------------------------------------------------------------------------
+// meshlets (renderer layout)
+//
+// Per active block, the occupied cells (valid, at least one triangle) in cell
+// order are cut greedily into meshlets of at most 64 distinct vertices and 126
+// triangles, as src/shaders/marching_cubes_pmb.task.glsl:182-215 cuts them
+// (a cell that would take the meshlet past either limit starts the next one);
+// each meshlet carries its own copy of every vertex its triangles use, in
+// order of first use: cell order, then owner index u (mtx_edge_uidx) within a
+// cell.
+//
+// A cell c of a meshlet that starts at cell S adds the active edges that no
+// cell of [S, c) uses.  The other cells using an edge of c are its neighbours
+// in three fixed directions; the nearest of them with a smaller index lies
+// back(u, F) cells before c (F: which neighbours of c exist in the block, bits
+// 2a / 2a+1 for -1 / +1 along axis a).  So c adds exactly the edges with
+// back(u, F) > c - S.  back takes one of nine values (1, B-1, B, B+1, B^2-B,
+// B^2-1, B^2, B^2+1, B^2+B) or none; the mask of added edges therefore
+// depends on F and on the interval of c - S between those values only, and is
+// read from a table of 64 x 10 entries.
+// ---------------------------------------------------------------------------
+
+// MAX_VERTS_PER_MESHLET, MAX_PRIMS_PER_MESHLET: src/shaders/marching_cubes_pmb.task.glsl:24-25
+constexpr uint32_t kMeshletVerts = 64;
+constexpr uint32_t kMeshletTris = 126;
+constexpr uint32_t kBackNone = 0xFFFFu;
+constexpr uint32_t kBackClasses = 10;
+
+template <int B>
+__host__ __device__ constexpr uint32_t back_dist(int i)
+{
+    return i == 0 ? 1u : i == 1 ? B - 1u : i == 2 ? (uint32_t)B : i == 3 ? B + 1u : i == 4 ? (uint32_t)(B * B - B)
+         : i == 5 ? (uint32_t)(B * B - 1) : i == 6 ? (uint32_t)(B * B) : i == 7 ? (uint32_t)(B * B + 1)
+         : (uint32_t)(B * B + B);
+}
+
+// Interval of k among the possible back distances: how many are <= k.
+template <int B>
+__host__ __device__ constexpr uint32_t back_class(uint32_t k)
+{
+    uint32_t n = 0;
+    for (int i = 0; i < 9; ++i) n += back_dist<B>(i) <= k ? 1u : 0u;
+    return n;
+}
+
+// A k of class cls, or kBackNone if no k has it.
+template <int B>
+__host__ __device__ constexpr uint32_t back_class_rep(uint32_t cls)
+{
+    for (uint32_t k = 0; k <= (uint32_t)(B * B + B); ++k)
+        if (back_class<B>(k) == cls) return k;
+    return kBackNone;
+}
+
+// Neighbour bits of cell (lx, ly, lz) of a block whose valid cells along each
+// axis are [0, vx) x [0, vy) x [0, vz): bit 2a, the cell at -1 along axis a
+// is in the block; bit 2a+1, the cell at +1 is in the block and valid.
+__device__ __forceinline__ uint32_t cell_nbrs(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t vx, uint32_t vy,
+                                              uint32_t vz)
+{
+    return (lx > 0 ? 1u : 0u) | (lx + 1 < vx ? 2u : 0u) | (ly > 0 ? 4u : 0u) | (ly + 1 < vy ? 8u : 0u) |
+           (lz > 0 ? 16u : 0u) | (lz + 1 < vz ? 32u : 0u);
+}
+
+// The three other cells using edge u of a cell: offsets (xp, xq) along the
+// edge's two perpendicular axes p < q, and the edge's owner index in them.
+// Visits each with its cell-index delta (negative: an earlier cell).
+template <int B, typename F>
+__device__ __forceinline__ void for_sharing_cells(uint32_t u, uint32_t nb, F&& visit)
+{
+    const uint32_t ax = u >> 2, dp = u & 1u, dq = (u >> 1) & 1u;
+    const uint32_t p = ax == 0 ? 1u : 0u, q = ax == 2 ? 1u : 2u;
+    const int sp = p == 0 ? 1 : B, sq = q == 1 ? B : B * B;
+#pragma unroll
+    for (uint32_t i = 0; i < 2; ++i)
+#pragma unroll
+        for (uint32_t j = 0; j < 2; ++j) {
+            if (i == dp && j == dq) continue;
+            const int xp = (int)dp - (int)i, xq = (int)dq - (int)j;
+            const bool okp = xp == 0 || ((nb >> (2 * p + (xp > 0 ? 1u : 0u))) & 1u);
+            const bool okq = xq == 0 || ((nb >> (2 * q + (xq > 0 ? 1u : 0u))) & 1u);
+            if (okp && okq) visit(xp * sp + xq * sq, 4u * ax + i + 2u * j);
+        }
+}
+
+// back(u, nb): cells back to the nearest earlier cell using the edge, or kBackNone.
+template <int B>
+__device__ __forceinline__ uint32_t share_back(uint32_t u, uint32_t nb)
+{
+    uint32_t best = kBackNone;
+    for_sharing_cells<B>(u, nb, [&](int delta, uint32_t) {
+        if (delta < 0) best = min(best, (uint32_t)(-delta));
+    });
+    return best;
+}
+
+// The cell of a meshlet that first uses edge u of a cell c (the meshlet holds
+// cells c - k .. c): back = c minus that cell, u1 = the edge's owner index there.
+template <int B>
+__device__ __forceinline__ void first_user(uint32_t u, uint32_t nb, uint32_t k, uint32_t& back, uint32_t& u1)
+{
+    back = 0;
+    u1 = u;
+    for_sharing_cells<B>(u, nb, [&](int delta, uint32_t uo) {
+        if (delta < 0 && (uint32_t)(-delta) <= k && (uint32_t)(-delta) > back) {
+            back = (uint32_t)(-delta);
+            u1 = uo;
+        }
+    });
+}
+
+// Workgroup tables: active-edge masks of the 256 cases in owner-index bits,
+// and the masks of edges a cell adds (64 neighbour patterns x 10 classes).
+template <int B>
+__device__ __forceinline__ void meshlet_tables(uint16_t* s_emu, uint16_t* s_newtab)
+{
+    for (uint32_t c = threadIdx.x; c < 256u; c += blockDim.x) s_emu[c] = (uint16_t)mtx_case_umask(c);
+    for (uint32_t nb = threadIdx.x; nb < 64u; nb += blockDim.x) {
+        uint32_t back[12];
+#pragma unroll
+        for (uint32_t u = 0; u < 12u; ++u) back[u] = share_back<B>(u, nb);
+#pragma unroll
+        for (uint32_t cls = 0; cls < kBackClasses; ++cls) {
+            const uint32_t k = back_class_rep<B>(cls);   // folded: cls is a constant of the unrolled loop
+            uint32_t m = 0;
+#pragma unroll
+            for (uint32_t u = 0; u < 12u; ++u) m |= back[u] > k ? 1u << u : 0u;
+            s_newtab[nb * kBackClasses + cls] = (uint16_t)m;
+        }
+    }
+}
+
+// Vertices a cell adds to a meshlet that started k cells before it.
+template <int B>
+__device__ __forceinline__ uint32_t added_edges(uint32_t emu, uint32_t nb, uint32_t k, const uint16_t* s_newtab)
+{
+    const uint32_t cls = k >= (uint32_t)(B * B + B) ? 9u : back_class<B>(k);
+    return emu & s_newtab[nb * kBackClasses + cls];
+}
+
+// Greedy cut of one block into meshlets.  The wave's lanes hold cells
+// lane * CPL + j, j < CPL; cell(j, cs, nb) gives cell j's case (0 if invalid)
+// and neighbour bits.  Returns the block's meshlet count and meshlet-vertex
+// count (wave-uniform) and this lane's start bits (bit j: a meshlet starts at
+// cell lane * CPL + j; cell 0 starts the first one, the cells before the
+// first occupied one holding nothing).  Each step cuts one meshlet: the
+// vertices and triangles every cell from the meshlet's start on would add,
+// a wave scan of them, and the first cell whose running sums pass a limit.
+// For up to 8 cells per lane each cell's case-derived data is read once and
+// kept packed: edge bits (12) | neighbour bits << 12 | triangles << 18 |
+// edges no earlier cell of the block uses << 21.
+template <int B, int CPL, typename Cell>
+__device__ __forceinline__ void meshlet_cut(Cell&& cell, const uint8_t* s_ntri, const uint16_t* s_emu,
+                                            const uint16_t* s_newtab, uint32_t& nm, uint32_t& ns, uint64_t& sbits)
+{
+    constexpr uint32_t NC = B * B * B;
+    constexpr bool KEEP = CPL <= 8;
+    constexpr uint32_t KFAR = (uint32_t)(B * B + B);   // at or past it every earlier sharer is in the meshlet
+    const uint32_t lane = lane_id();
+    nm = 0;
+    ns = 0;
+    sbits = 0;
+    uint32_t info[KEEP ? CPL : 1];
+    uint32_t any = 0;
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) {
+        uint32_t cs, nb;
+        cell(j, cs, nb);
+        const uint32_t p = lane * CPL + j < NC ? (uint32_t)s_ntri[cs] : 0u;
+        any |= p;
+        if constexpr (KEEP) {
+            const uint32_t emu = p ? (uint32_t)s_emu[cs] : 0u;
+            const uint32_t ninf = __popc(emu & s_newtab[nb * kBackClasses + kBackClasses - 1]);
+            info[j] = emu | (nb << 12) | (p << 18) | (ninf << 21);
+        }
+    }
+    if (__ballot(any != 0) == 0) return;
+    if (lane == 0) sbits = 1;
+    uint32_t S = 0;
+    // vertices (v) and triangles (p) cell j adds to the meshlet starting at S
+    auto vp = [&](int j, uint32_t& v, uint32_t& p) {
+        const uint32_t c = lane * CPL + j;
+        v = 0;
+        p = 0;
+        if (c >= NC || c < S) return;
+        const uint32_t k = c - S;
+        if constexpr (KEEP) {
+            const uint32_t e = info[j];
+            p = (e >> 18) & 7u;
+            if (k >= KFAR) v = e >> 21;
+            else if (p) v = __popc(added_edges<B>(e & 0xFFFu, (e >> 12) & 63u, k, s_newtab));
+        } else {
+            uint32_t cs, nb;
+            cell(j, cs, nb);
+            p = s_ntri[cs];
+            if (p) v = __popc(added_edges<B>(s_emu[cs], nb, k, s_newtab));
+        }
+    };
+    for (;;) {
+        uint32_t lv = 0, lp = 0, vpk = 0, ppk = 0;
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
+            uint32_t v, p;
+            vp(j, v, p);
+            lv += v;
+            lp += p;
+            if constexpr (KEEP) {
+                vpk |= v << (4 * j);
+                ppk |= p << (3 * j);
+            }
+        }
+        uint32_t tv, tp;
+        uint32_t rv = wave_excl_scan<uint32_t>(lv, tv);
+        uint32_t rp = wave_excl_scan<uint32_t>(lp, tp);
+        uint32_t cut = 0xFFFFFFFFu, before = 0;
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
+            uint32_t v, p;
+            if constexpr (KEEP) {
+                v = (vpk >> (4 * j)) & 15u;
+                p = (ppk >> (3 * j)) & 7u;
+            } else {
+                vp(j, v, p);
+            }
+            rv += v;
+            rp += p;
+            if (cut == 0xFFFFFFFFu && (rv > kMeshletVerts || rp > kMeshletTris)) {
+                cut = lane * CPL + j;
+                before = rv - v;
+            }
+        }
+        const uint64_t bal = __ballot(cut != 0xFFFFFFFFu);
+        ++nm;
+        if (!bal) {
+            ns += tv;
+            break;
+        }
+        const int src = __builtin_ctzll(bal);
+        const uint32_t b = __shfl(cut, src, 64);
+        ns += __shfl(before, src, 64);
+        if (lane == b / CPL) sbits |= 1ull << (b % CPL);
+        S = b;
+    }
+}
+
+// Words of a block's meshlet start bits (one bit per cell).
+template <int B> struct StartBits {
+    static constexpr uint32_t NW = (B * B * B + 31) / 32;
+};
+
+// This lane's start bits (cells lane * CPL .. lane * CPL + CPL - 1, CPL <= 64)
+// from a block's start words.
+template <int B, int CPL>
+__device__ __forceinline__ uint64_t lane_start_bits(const uint32_t* __restrict__ w, uint32_t lane)
+{
+    constexpr uint32_t NW = StartBits<B>::NW, NC = B * B * B;
+    const uint32_t first = lane * CPL;
+    if (first >= NC) return 0;
+    const uint32_t w0 = first >> 5, sh = first & 31u;
+    const uint64_t lo = (uint64_t)w[w0] | (w0 + 1 < NW ? (uint64_t)w[w0 + 1] << 32 : 0ull);
+    uint64_t v = lo >> sh;
+    if (sh && w0 + 2 < NW) v |= (uint64_t)w[w0 + 2] << (64 - sh);
+    return CPL >= 64 ? v : (v & ((1ull << CPL) - 1ull));
 }
 
 template <int B, typename T, bool BATCH>
@@ -1224,10 +1486,8 @@ __global__ __launch_bounds__(256) void k_extract_count(ExtractArgs a)
             for (int j = 0; j < CPL; ++j)
                 if (lane * CPL + j < (uint32_t)NC) dst[j] = ent[j];
         }
-        if (lane == 0) {
-            a.blk_counts[2 * (uint64_t)i] = tv;
-            a.blk_counts[2 * (uint64_t)i + 1] = tt;
-        }
+        // meshlet counts: 0 here, the cut kernel writes them when meshlets are made
+        if (lane == 0) *reinterpret_cast<uint4*>(a.blk_counts + 4 * (uint64_t)i) = make_uint4(tv, tt, 0u, 0u);
         wave_lds_sync();
     }
 }
@@ -1342,19 +1602,17 @@ __global__ __launch_bounds__(256) void k_extract_count_b8(ExtractArgs a)
         }
         *reinterpret_cast<uint4*>(table + (uint64_t)i * NC + lane * 8u) = *reinterpret_cast<const uint4*>(ent);
         *reinterpret_cast<uint2*>(a.cases + (uint64_t)i * NC + lane * 8u) = make_uint2(cases0, cases1);
-        if (lane == 0) {
-            a.blk_counts[2 * (uint64_t)i] = tv;
-            a.blk_counts[2 * (uint64_t)i + 1] = tt;
-        }
+        if (lane == 0) *reinterpret_cast<uint4*>(a.blk_counts + 4 * (uint64_t)i) = make_uint4(tv, tt, 0u, 0u);
         wave_lds_sync();
     }
 }
 
-// Exclusive scan of per-block {vertices, triangles, meshlets} into bases, in
-// two launches (reduce-then-scatter, as the compaction): tile sums, then each
-// tile adds up its predecessors' sums and writes its blocks' bases.  Tile
-// sums: 3 words per tile.
+// Exclusive scan of the per-block counts {vertices, triangles, meshlets,
+// meshlet vertices} into bases, in two launches (reduce-then-scatter, as the
+// compaction): tile sums, then each tile adds up its predecessors' sums and
+// writes its blocks' bases.  Tile sums: 4 words per tile.
 constexpr uint32_t kScanItems = 4;
+constexpr uint32_t kScanQ = 4;   // quantities scanned
 uint32_t scan_tile_items() { return 256 * kScanItems; }
 
 __device__ __forceinline__ uint32_t scan_tiles_of(uint32_t na)
@@ -1364,11 +1622,13 @@ __device__ __forceinline__ uint32_t scan_tiles_of(uint32_t na)
 }
 
 __device__ __forceinline__ void block_counts(const uint32_t* __restrict__ cnt, uint32_t na, uint64_t i,
-                                             uint64_t& v, uint64_t& t, uint64_t& m)
+                                             uint64_t (&q)[kScanQ])
 {
-    v = i < na ? cnt[2 * i] : 0;
-    t = i < na ? cnt[2 * i + 1] : 0;
-    m = (t + kMeshletPrims - 1) / kMeshletPrims;
+    const uint4 c = i < na ? *reinterpret_cast<const uint4*>(cnt + 4 * i) : make_uint4(0, 0, 0, 0);
+    q[0] = c.x;
+    q[1] = c.y;
+    q[2] = c.z;
+    q[3] = c.w;
 }
 
 __global__ __launch_bounds__(256) void k_extract_tile_sums(const uint32_t* __restrict__ cnt,
@@ -1377,125 +1637,115 @@ __global__ __launch_bounds__(256) void k_extract_tile_sums(const uint32_t* __res
                                                            uint64_t* __restrict__ tile_sums,
                                                            uint64_t* __restrict__ totals)
 {
-    __shared__ uint64_t s_red[12];
+    __shared__ uint64_t s_red[4 * kScanQ];
     const uint32_t na = min(*count, max_blocks);
     const uint32_t tile = blockIdx.x;
     if (tile >= scan_tiles_of(na)) return;
     // error flags of the emit pass; bit 1: a count beyond the per-block arrays
     if (tile == 0 && threadIdx.x == 0) totals[2] = *count > max_blocks ? 2ull : 0ull;
-    uint64_t sv = 0, st = 0, sm = 0;
+    uint64_t sum[kScanQ] = { 0, 0, 0, 0 };
 #pragma unroll
     for (uint32_t k = 0; k < kScanItems; ++k) {
-        uint64_t v, t, m;
-        block_counts(cnt, na, (uint64_t)tile * 256 * kScanItems + threadIdx.x * kScanItems + k, v, t, m);
-        sv += v;
-        st += t;
-        sm += m;
+        uint64_t q[kScanQ];
+        block_counts(cnt, na, (uint64_t)tile * 256 * kScanItems + threadIdx.x * kScanItems + k, q);
+#pragma unroll
+        for (uint32_t r = 0; r < kScanQ; ++r) sum[r] += q[r];
     }
-    sv = wave_sum<uint64_t>(sv);
-    st = wave_sum<uint64_t>(st);
-    sm = wave_sum<uint64_t>(sm);
     const uint32_t w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-        s_red[w] = sv;
-        s_red[4 + w] = st;
-        s_red[8 + w] = sm;
+#pragma unroll
+    for (uint32_t r = 0; r < kScanQ; ++r) {
+        sum[r] = wave_sum<uint64_t>(sum[r]);
+        if ((threadIdx.x & 63) == 0) s_red[4 * r + w] = sum[r];
     }
     __syncthreads();
-    if (threadIdx.x < 3) {
+    if (threadIdx.x < kScanQ) {
         const uint64_t* r = s_red + 4 * threadIdx.x;
-        tile_sums[3 * (uint64_t)tile + threadIdx.x] = r[0] + r[1] + r[2] + r[3];
+        tile_sums[kScanQ * (uint64_t)tile + threadIdx.x] = r[0] + r[1] + r[2] + r[3];
     }
 }
 
+// Bases: vbase, tbase, mbase, sbase (meshlet vertices); totals: [0] vertices,
+// [1] triangles, [3] meshlets, [4] meshlet vertices; out_counts: vertexCount,
+// indexCount, meshletCount, meshletVertexCount (saturated to 32 bits).
 __global__ __launch_bounds__(256) void k_extract_scan(const uint32_t* __restrict__ cnt,
                                                       const uint32_t* __restrict__ count,
                                                       uint32_t max_blocks,
                                                       uint64_t* __restrict__ vbase,
                                                       uint64_t* __restrict__ tbase,
                                                       uint64_t* __restrict__ mbase,
+                                                      uint64_t* __restrict__ sbase,
                                                       uint64_t* __restrict__ totals,
                                                       uint32_t* __restrict__ out_counts,
                                                       const uint64_t* __restrict__ tile_sums)
 {
-    __shared__ uint64_t s_w[12];
-    __shared__ uint64_t s_red[12];
+    __shared__ uint64_t s_w[4 * kScanQ];
+    __shared__ uint64_t s_red[4 * kScanQ];
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
     const uint32_t na = min(*count, max_blocks);
     const uint32_t ntiles = scan_tiles_of(na);
     const uint32_t tile = blockIdx.x;
     if (tile >= ntiles) return;
     // exclusive prefix of this tile: its predecessors' sums
-    uint64_t pv = 0, pt = 0, pm = 0;
+    uint64_t pre[kScanQ] = { 0, 0, 0, 0 };
     for (uint32_t k = threadIdx.x; k < tile; k += 256) {
-        pv += tile_sums[3 * (uint64_t)k];
-        pt += tile_sums[3 * (uint64_t)k + 1];
-        pm += tile_sums[3 * (uint64_t)k + 2];
+#pragma unroll
+        for (uint32_t r = 0; r < kScanQ; ++r) pre[r] += tile_sums[kScanQ * (uint64_t)k + r];
     }
-    pv = wave_sum<uint64_t>(pv);
-    pt = wave_sum<uint64_t>(pt);
-    pm = wave_sum<uint64_t>(pm);
-    if (lane == 0) {
-        s_red[w] = pv;
-        s_red[4 + w] = pt;
-        s_red[8 + w] = pm;
+#pragma unroll
+    for (uint32_t r = 0; r < kScanQ; ++r) {
+        pre[r] = wave_sum<uint64_t>(pre[r]);
+        if (lane == 0) s_red[4 * r + w] = pre[r];
     }
-    uint64_t v[kScanItems], t[kScanItems], m[kScanItems];
-    uint64_t sv = 0, st = 0, sm = 0;
+    uint64_t q[kScanItems][kScanQ];
+    uint64_t sum[kScanQ] = { 0, 0, 0, 0 };
 #pragma unroll
     for (uint32_t k = 0; k < kScanItems; ++k) {
-        block_counts(cnt, na, (uint64_t)tile * 256 * kScanItems + threadIdx.x * kScanItems + k, v[k], t[k], m[k]);
-        sv += v[k];
-        st += t[k];
-        sm += m[k];
+        block_counts(cnt, na, (uint64_t)tile * 256 * kScanItems + threadIdx.x * kScanItems + k, q[k]);
+#pragma unroll
+        for (uint32_t r = 0; r < kScanQ; ++r) sum[r] += q[k][r];
     }
-    const uint64_t iv = wave_incl_scan<uint64_t>(sv);
-    const uint64_t it = wave_incl_scan<uint64_t>(st);
-    const uint64_t im = wave_incl_scan<uint64_t>(sm);
-    if (lane == 63) {
-        s_w[w] = iv;
-        s_w[4 + w] = it;
-        s_w[8 + w] = im;
+    uint64_t inc[kScanQ];
+#pragma unroll
+    for (uint32_t r = 0; r < kScanQ; ++r) {
+        inc[r] = wave_incl_scan<uint64_t>(sum[r]);
+        if (lane == 63) s_w[4 * r + w] = inc[r];
     }
     __syncthreads();
-    const uint64_t exv = s_red[0] + s_red[1] + s_red[2] + s_red[3];
-    const uint64_t ext = s_red[4] + s_red[5] + s_red[6] + s_red[7];
-    const uint64_t exm = s_red[8] + s_red[9] + s_red[10] + s_red[11];
-    uint64_t ov = 0, ot = 0, om = 0, av = 0, at = 0, am = 0;
+    uint64_t run[kScanQ], total[kScanQ];
 #pragma unroll
-    for (uint32_t j = 0; j < 4; ++j) {
-        ov += j < w ? s_w[j] : 0;
-        ot += j < w ? s_w[4 + j] : 0;
-        om += j < w ? s_w[8 + j] : 0;
-        av += s_w[j];
-        at += s_w[4 + j];
-        am += s_w[8 + j];
+    for (uint32_t r = 0; r < kScanQ; ++r) {
+        const uint64_t ex = s_red[4 * r] + s_red[4 * r + 1] + s_red[4 * r + 2] + s_red[4 * r + 3];
+        uint64_t ow = 0, all = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+            ow += j < w ? s_w[4 * r + j] : 0;
+            all += s_w[4 * r + j];
+        }
+        run[r] = ex + ow + inc[r] - sum[r];
+        total[r] = ex + all;
     }
     if (threadIdx.x == 0 && tile == ntiles - 1) {
-        totals[0] = exv + av;
-        totals[1] = ext + at;
-        totals[3] = exm + am;
+        totals[0] = total[0];
+        totals[1] = total[1];
+        totals[3] = total[2];
+        totals[4] = total[3];
         if (out_counts) {
-            const uint64_t V = exv + av, I = 3 * (ext + at), M = exm + am;
-            out_counts[0] = V > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)V;
-            out_counts[1] = I > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)I;
-            out_counts[2] = M > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)M;
+            const uint64_t c4[4] = { total[0], 3 * total[1], total[2], total[3] };
+#pragma unroll
+            for (int r = 0; r < 4; ++r) out_counts[r] = c4[r] > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)c4[r];
         }
     }
-    uint64_t rv = exv + ov + iv - sv;
-    uint64_t rt = ext + ot + it - st;
-    uint64_t rm = exm + om + im - sm;
 #pragma unroll
     for (uint32_t k = 0; k < kScanItems; ++k) {
         const uint64_t i = (uint64_t)tile * 256 * kScanItems + threadIdx.x * kScanItems + k;
         if (i < na) {
-            vbase[i] = rv;
-            tbase[i] = rt;
-            mbase[i] = rm;
+            vbase[i] = run[0];
+            tbase[i] = run[1];
+            mbase[i] = run[2];
+            sbase[i] = run[3];
         }
-        rv += v[k];
-        rt += t[k];
-        rm += m[k];
+#pragma unroll
+        for (uint32_t r = 0; r < kScanQ; ++r) run[r] += q[k][r];
     }
 }
 
@@ -1602,7 +1852,7 @@ __global__ __launch_bounds__((64 * VertCfg<B, NORMALS, T>::WPG)) void k_extract_
     constexpr int LPL = (NTILE + 63) / 64;
     const WaveItems it = wave_items(na, WPG);
     for (uint32_t i = it.first; i < it.end; i += it.stride) {
-        const uint32_t bnv = a.blk_counts[2 * (uint64_t)i];
+        const uint32_t bnv = a.blk_counts[4 * (uint64_t)i];
         if (bnv == 0) continue;
         const uint64_t vb = a.blk_vbase[i];
         if (vb + bnv > a.vcap) {
@@ -1916,8 +2166,8 @@ __global__ __launch_bounds__((64 * TriCfg<B>::WPG)) void k_extract_triangles(Ext
     const uint32_t* __restrict__ tb32 = reinterpret_cast<const uint32_t*>(a.blk_tbase);
     const uint32_t* __restrict__ mb32 = reinterpret_cast<const uint32_t*>(a.blk_mbase);
     auto stage1 = [&](TB& P, uint32_t i) {
-        P.bnv = a.blk_counts[2 * (uint64_t)i];
-        P.bnt = a.blk_counts[2 * (uint64_t)i + 1];
+        P.bnv = a.blk_counts[4 * (uint64_t)i];
+        P.bnt = a.blk_counts[4 * (uint64_t)i + 1];
         P.vb = vb32[2 * (uint64_t)i];
         P.tb = a.blk_tbase[i];
         P.mb = mb32[2 * (uint64_t)i];
@@ -1976,10 +2226,9 @@ __global__ __launch_bounds__((64 * TriCfg<B>::WPG)) void k_extract_triangles(Ext
         const uint32_t inx = i + it.stride;
         const bool more = inx < it.end;
         if (more) stage1(nxt, inx);
-        const uint32_t bnt = cur.bnt, bnv = cur.bnv;
-        const uint32_t bnm = (bnt + kMeshletPrims - 1) / kMeshletPrims;
+        const uint32_t bnt = cur.bnt;
         bool work = bnt != 0;
-        if (work && (3 * (cur.tb + bnt) > a.icap || (a.meshlets && (uint64_t)cur.mb + bnm > a.mcap))) {
+        if (work && 3 * (cur.tb + bnt) > a.icap) {
             if (lane == 0) atomicOr(reinterpret_cast<unsigned long long*>(&a.totals[2]), 1ull);
             work = false;
         }
@@ -2069,19 +2318,361 @@ __global__ __launch_bounds__((64 * TriCfg<B>::WPG)) void k_extract_triangles(Ext
             // an active edge whose owner block is not in the active list: the
             // list does not belong to this volume/isovalue
             if (bad & 0xF0u) atomicOr(reinterpret_cast<unsigned long long*>(&a.totals[2]), 2ull);
-            if (a.meshlets) {
-                for (uint32_t k = lane; k < bnm; k += 64u) {
-                    const uint32_t prims = min(kMeshletPrims, bnt - k * kMeshletPrims);
-                    a.meshlets[(uint64_t)cur.mb + k] = make_uint4(cur.vb - cur.uv,
-                                                                  3u * ((uint32_t)tb - cur.ut + k * kMeshletPrims),
-                                                                  bnv, prims);
-                }
-            }
             wave_lds_sync();
         }
         if (more) stage3(nxt);
         cur = nxt;
         i = inx;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// meshlet passes
+//
+// Cut (between the count pass and the scan): one wave per active block reads
+// the block's case indices, cuts its occupied cells into meshlets and writes
+// the block's meshlet and meshlet-vertex counts (the scan turns them into
+// bases) and its start bits.  Emit (after the triangle pass): one wave per
+// active block rebuilds, from the start bits and the cases, each cell's added
+// edges and first meshlet vertex; then lists the block's triangles in output
+// order and, for each triangle vertex, finds the cell of the meshlet that
+// first uses its edge: the meshlet index is that cell's first meshlet vertex
+// plus the edge's rank among the edges the cell adds, and the vertex-map entry
+// of that meshlet vertex is the triangle's own (shared-mesh) index.  Finally
+// the meshlet descriptors.
+// ---------------------------------------------------------------------------
+
+// This lane's CPL cell cases of active block i (0 for cells outside the volume).
+template <int B>
+struct LaneCases {
+    static constexpr int CPL = EB<B>::CPL, NC = EB<B>::NC, NCW = (CPL + 3) / 4;
+    uint32_t w[NCW];
+    __device__ __forceinline__ void load(const uint8_t* __restrict__ cases, uint64_t i, uint32_t lane)
+    {
+        if constexpr (CPL == 8) {
+            const uint2 c2 = *reinterpret_cast<const uint2*>(cases + i * NC + lane * 8u);
+            w[0] = c2.x;
+            w[1] = c2.y;
+        } else {
+#pragma unroll
+            for (int k = 0; k < NCW; ++k) w[k] = 0;
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) {
+                const uint32_t c = lane * CPL + j;
+                if (c < (uint32_t)NC) w[j / 4] |= (uint32_t)cases[i * NC + c] << (8 * (j & 3));
+            }
+        }
+    }
+    __device__ __forceinline__ uint32_t cs(int j) const { return (w[j / 4] >> (8 * (j & 3))) & 0xFFu; }
+};
+
+// Origin and valid-cell extents (10 bits per axis) of block `id`.
+__device__ __forceinline__ void block_cells(const ExtractArgs& a, uint32_t id, uint32_t& vlim)
+{
+    const uint32_t gx = a.grid.gx, gy = a.grid.gy, B = a.grid.B;
+    const uint32_t ox = (id % gx) * B, oy = ((id / gx) % gy) * B, oz = (id / (gx * gy)) * B;
+    const uint32_t vx = a.vol.X - 1 > ox ? min(a.vol.X - 1 - ox, B) : 0u;
+    const uint32_t vy = a.vol.Y - 1 > oy ? min(a.vol.Y - 1 - oy, B) : 0u;
+    const uint32_t vz = a.vol.Z - 1 > oz ? min(a.vol.Z - 1 - oz, B) : 0u;
+    vlim = vx | (vy << 10) | (vz << 20);
+}
+
+template <int B>
+__device__ __forceinline__ uint32_t nbrs_of_cell(uint32_t c, uint32_t vlim)
+{
+    return cell_nbrs(c % B, (c / B) % B, c / (B * B), vlim & 1023u, (vlim >> 10) & 1023u, vlim >> 20);
+}
+
+template <int B>
+__global__ __launch_bounds__(256) void k_meshlet_cut(ExtractArgs a)
+{
+    constexpr int CPL = EB<B>::CPL;
+    constexpr uint32_t NW = StartBits<B>::NW;
+    __shared__ uint8_t s_ntri[256];
+    __shared__ uint16_t s_emu[256];
+    __shared__ uint16_t s_newtab[64 * kBackClasses];
+    __shared__ uint32_t s_sb_all[4][NW];
+    for (uint32_t c = threadIdx.x; c < 256u; c += blockDim.x) s_ntri[c] = (uint8_t)mtx_word_ntri(mtx_tri_words[c]);
+    meshlet_tables<B>(s_emu, s_newtab);
+    __syncthreads();
+    const uint32_t lane = lane_id();
+    uint32_t* s_sb = s_sb_all[threadIdx.x >> 6];
+    const uint32_t na = active_items(a);
+    const WaveItems it = wave_items(na, 4);
+    // the next block's ID and cases load while this block is cut
+    uint32_t id_n = 0;
+    LaneCases<B> lc_n;
+    if (it.first < it.end) {
+        id_n = a.ids[it.first];
+        lc_n.load(a.cases, it.first, lane);
+    }
+    for (uint32_t i = it.first; i < it.end; i += it.stride) {
+        const uint32_t id = id_n;
+        const LaneCases<B> lc = lc_n;
+        if (i + it.stride < it.end) {
+            id_n = a.ids[i + it.stride];
+            lc_n.load(a.cases, i + it.stride, lane);
+        }
+        uint32_t vlim;
+        block_cells(a, id, vlim);
+        uint32_t nm, ns;
+        uint64_t sb;
+        meshlet_cut<B, CPL>(
+            [&](int j, uint32_t& cs, uint32_t& nb) {
+                cs = lc.cs(j);
+                nb = nbrs_of_cell<B>(lane * CPL + j, vlim);
+            },
+            s_ntri, s_emu, s_newtab, nm, ns, sb);
+        if constexpr (CPL == 8 && NW == 16) {
+            // lane l's 8 cells are byte l of the block's 64
+            reinterpret_cast<uint8_t*>(a.starts)[(uint64_t)i * 64 + lane] = (uint8_t)sb;
+        } else {
+            for (uint32_t k = lane; k < NW; k += 64u) s_sb[k] = 0;
+            wave_lds_sync();
+            for (uint64_t m = sb; m; m &= m - 1) {
+                const uint32_t c = lane * CPL + (uint32_t)__builtin_ctzll(m);
+                atomicOr(&s_sb[c >> 5], 1u << (c & 31u));
+            }
+            wave_lds_sync();
+            for (uint32_t k = lane; k < NW; k += 64u) a.starts[(uint64_t)i * NW + k] = s_sb[k];
+            wave_lds_sync();
+        }
+        if (lane == 0) *reinterpret_cast<uint2*>(a.blk_counts + 4 * (uint64_t)i + 2) = make_uint2(nm, ns);
+    }
+}
+
+template <int B> struct EmitCfg {
+    static constexpr int NC = B * B * B;
+    static constexpr int LIST = 256;             // triangles listed at a time
+    // list entry: cell index (CB bits) | decoded triangle (case * 5 + k) << CB
+    // [| B <= 8: the cell's distance back to its meshlet's first cell << CB + 11]
+    static constexpr int CB = B <= 8 ? 9 : 12;
+    static constexpr bool KIN = B <= 8;
+    static constexpr int MAXM = NC / 5 + 2;      // every meshlet but a block's last has >= 5 cells
+    static constexpr int TL = MAXM > LIST ? MAXM : LIST;
+    static constexpr int WAVE_LDS = NC * 4 + (KIN ? 0 : NC * 2) + TL * 4 + 3 * LIST * 4;
+    static constexpr int WG_LDS = 256 + 512 + 64 * (int)kBackClasses * 2 + 256 * 5 * 2;
+    static constexpr int WPG = WAVE_LDS * 4 + WG_LDS <= 65536 ? 4 : (WAVE_LDS * 2 + WG_LDS <= 65536 ? 2 : 1);
+};
+
+// One block's inputs to the emit pass (loaded while the previous block runs).
+template <int B>
+struct EmitBlock {
+    uint4 cnt;          // vertices, triangles, meshlets, meshlet vertices
+    uint64_t tb;        // first triangle
+    uint32_t mb, msb;   // first meshlet, first meshlet vertex
+    uint32_t id;
+    uint64_t sbits;     // this lane's start bits
+    LaneCases<B> lc;
+};
+
+template <int B>
+__global__ __launch_bounds__(64 * EmitCfg<B>::WPG) void k_meshlet_emit(ExtractArgs a)
+{
+    using C = EmitCfg<B>;
+    using EBk = EmitBlock<B>;
+    constexpr int NC = EB<B>::NC, CPL = EB<B>::CPL, CB = C::CB;
+    constexpr uint32_t LIST = C::LIST, NW = StartBits<B>::NW;
+    constexpr bool KIN = C::KIN;
+    constexpr bool PACKN = CPL * 3 <= 32;
+    constexpr int WPG = C::WPG;
+    __shared__ uint8_t s_ntri[256];
+    __shared__ uint16_t s_emu[256];
+    __shared__ uint16_t s_newtab[64 * kBackClasses];
+    __shared__ uint16_t s_tu[256 * 5];          // owner indices of each decoded triangle's vertices
+    // per wave: each cell's first meshlet vertex in the block (20 bits) | its
+    // added edges << 20; the cells' distance back to their meshlet's start when
+    // the list entry cannot hold it; the triangle list (then the descriptors'
+    // inputs); the listed triangles' shared-mesh indices
+    __shared__ uint32_t s_mc_all[WPG][NC];
+    __shared__ uint16_t s_mk_all[KIN ? 1 : WPG][KIN ? 1 : NC];
+    __shared__ uint32_t s_tl_all[WPG][C::TL];
+    __shared__ __align__(16) uint32_t s_idx_all[WPG][3 * LIST];
+    for (uint32_t c = threadIdx.x; c < 256u; c += blockDim.x) s_ntri[c] = (uint8_t)mtx_word_ntri(mtx_tri_words[c]);
+    for (uint32_t x = threadIdx.x; x < 256u * 5u; x += blockDim.x) {
+        const uint64_t wd = mtx_tri_words[x / 5u];
+        const uint32_t k = x % 5u;
+        uint32_t tu = 0;
+        if (k < mtx_word_ntri(wd))
+            for (uint32_t j = 0; j < 3u; ++j) tu |= mtx_edge_uidx(mtx_word_edge(wd, k, j)) << (4u * j);
+        s_tu[x] = (uint16_t)tu;
+    }
+    meshlet_tables<B>(s_emu, s_newtab);
+    __syncthreads();
+    const uint32_t lane = lane_id(), w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    uint32_t* s_mc = s_mc_all[w];
+    uint16_t* s_mk = s_mk_all[KIN ? 0 : w];
+    uint32_t* s_tl = s_tl_all[w];
+    uint32_t* s_idx = s_idx_all[w];
+    const uint32_t* __restrict__ mb32 = reinterpret_cast<const uint32_t*>(a.blk_mbase);
+    const uint32_t* __restrict__ sb32 = reinterpret_cast<const uint32_t*>(a.blk_sbase);
+    auto load = [&](EBk& P, uint32_t i) {
+        P.cnt = *reinterpret_cast<const uint4*>(a.blk_counts + 4 * (uint64_t)i);
+        P.tb = a.blk_tbase[i];
+        P.mb = mb32[2 * (uint64_t)i];
+        P.msb = sb32[2 * (uint64_t)i];
+        P.id = a.ids[i];
+        if constexpr (CPL == 8 && NW == 16)
+            P.sbits = reinterpret_cast<const uint8_t*>(a.starts)[(uint64_t)i * 64 + lane];
+        else
+            P.sbits = lane_start_bits<B, CPL>(a.starts + (uint64_t)i * NW, lane);
+        P.lc.load(a.cases, i, lane);
+    };
+    // the indices of triangles [c0, c0 + n) of a block go to LDS with direct
+    // loads (4 bytes per lane, completion on the vector-memory counter)
+    auto stage_indices = [&](uint64_t tb, uint32_t c0, uint32_t n) {
+        const uint32_t nd = 3 * n;
+        const uint32_t* src = a.indices + 3 * (tb + c0);
+        for (uint32_t k = 0; k < nd; k += 64u)
+            if (k + lane < nd) lds_direct_load4(src + k + lane, s_idx + k);
+    };
+    const uint32_t na = active_items(a);
+    const WaveItems it = wave_items(na, WPG);
+    EBk nxt;
+    if (it.first < it.end) load(nxt, it.first);
+    for (uint32_t i = it.first; i < it.end; i += it.stride) {
+        const EBk cur = nxt;
+        if (i + it.stride < it.end) load(nxt, i + it.stride);
+        const uint32_t bnt = cur.cnt.y, bnm = cur.cnt.z, bns = cur.cnt.w;
+        if (bnt == 0) continue;
+        const uint64_t tb = cur.tb;
+        const uint32_t mb = cur.mb, msb = cur.msb;
+        if (3 * (tb + bnt) > a.icap || (uint64_t)msb + bns > a.mvcap || (uint64_t)mb + bnm > a.mcap ||
+            bnm > (uint32_t)C::MAXM) {
+            if (lane == 0) atomicOr(reinterpret_cast<unsigned long long*>(&a.totals[2]), 1ull);
+            continue;
+        }
+        // the first list's indices load while the cells are set up
+        stage_indices(tb, 0, min(LIST, bnt));
+        uint32_t vlim;
+        block_cells(a, cur.id, vlim);
+        const uint64_t sbits = cur.sbits;
+        const uint32_t S0 =
+            wave_excl_max(sbits ? lane * CPL + 63u - (uint32_t)__builtin_clzll(sbits) : 0u);
+        // each cell's added edges and distance back to its meshlet's start,
+        // its triangle count; the block's meshlet vertices and triangles before it
+        uint32_t S = S0, cnt_v = 0, npk = 0, nt = 0;
+        auto added = [&](int j, uint32_t& k) -> uint32_t {
+            const uint32_t c = lane * CPL + j;
+            if ((sbits >> j) & 1ull) S = c;
+            k = c - S;
+            const uint32_t cs = cur.lc.cs(j);
+            if (c >= (uint32_t)NC || !s_ntri[cs]) return 0u;
+            return added_edges<B>(s_emu[cs], nbrs_of_cell<B>(c, vlim), k, s_newtab);
+        };
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
+            uint32_t k;
+            cnt_v += __popc(added(j, k));
+            const uint32_t n = s_ntri[cur.lc.cs(j)];
+            nt += n;
+            if (PACKN) npk |= n << (3 * j);
+        }
+        uint32_t ns, tt;
+        uint32_t slot = wave_excl_scan<uint32_t>(cnt_v, ns);
+        const uint32_t et = wave_excl_scan<uint32_t>(nt, tt);
+        const uint32_t nm = wave_sum<uint32_t>((uint32_t)__popcll(sbits));
+        if (ns != bns || nm != bnm || tt != bnt) {
+            // the cut and the counts disagree with this block's data
+            if (lane == 0) atomicOr(reinterpret_cast<unsigned long long*>(&a.totals[2]), 4ull);
+            __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the staged loads have landed
+            continue;
+        }
+        S = S0;
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
+            const uint32_t c = lane * CPL + j;
+            uint32_t k;
+            const uint32_t m = added(j, k);
+            if (c < (uint32_t)NC) {
+                s_mc[c] = slot | (m << 20);
+                if constexpr (!KIN) s_mk[c] = (uint16_t)k;
+            }
+            slot += __popc(m);
+        }
+        for (uint32_t c0 = 0; c0 < tt; c0 += LIST) {
+            const uint32_t n = min(LIST, tt - c0);
+            if (c0) stage_indices(tb, c0, n);
+            // list entries in output order: cell by cell, the case's triangles k = 0..4
+            uint32_t run = et - c0;
+            S = S0;
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) {
+                const uint32_t c = lane * CPL + j;
+                if ((sbits >> j) & 1ull) S = c;
+                if (c < (uint32_t)NC) {
+                    const uint32_t cs = cur.lc.cs(j);
+                    const uint32_t nn = PACKN ? (npk >> (3 * j)) & 7u : (uint32_t)s_ntri[cs];
+                    uint32_t ent = c | ((cs * 5u) << CB);
+                    if constexpr (KIN) ent |= (c - S) << (CB + 11);
+#pragma unroll
+                    for (uint32_t k = 0; k < 5u; ++k)
+                        if (k < nn && run + k < LIST) s_tl[run + k] = ent + (k << CB);
+                    run += nn;
+                }
+            }
+            __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the indices are in LDS
+            wave_lds_sync();
+            // one triangle vertex per lane: its edge's first user in the
+            // meshlet and the edge's rank there give the meshlet vertex; that
+            // vertex holds this triangle vertex (every triangle using it writes
+            // the same shared-mesh index)
+            uint32_t* dst = a.mindices + 3 * (tb + c0);
+            for (uint32_t v = lane; v < 3 * n; v += 64u) {
+                const uint32_t q = v / 3u, e3 = v - 3u * q;
+                const uint32_t e = s_tl[q];
+                const uint32_t c = e & ((1u << CB) - 1u);
+                const uint32_t u = ((uint32_t)s_tu[(e >> CB) & 2047u] >> (4 * e3)) & 15u;
+                const uint32_t kk = KIN ? e >> (CB + 11) : (uint32_t)s_mk[c];
+                uint32_t back, u1;
+                first_user<B>(u, nbrs_of_cell<B>(c, vlim), kk, back, u1);
+                const uint32_t mc = s_mc[c - back];
+                const uint32_t mv = msb + (mc & 0xFFFFFu) + __popc((mc >> 20) & ((1u << u1) - 1u));
+                a.mvmap[mv] = s_idx[v];
+                __builtin_nontemporal_store(mv, dst + v);
+            }
+            wave_lds_sync();
+        }
+        // descriptors: each meshlet's first meshlet vertex and first triangle
+        // within the block (16 bits each), staged at its index
+        uint32_t nmt;
+        uint32_t m = wave_excl_scan<uint32_t>((uint32_t)__popcll(sbits), nmt);
+        uint32_t tp = et;
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
+            const uint32_t c = lane * CPL + j;
+            if ((sbits >> j) & 1ull) s_tl[m++] = (s_mc[c] & 0xFFFFu) | (tp << 16);
+            tp += PACKN ? (npk >> (3 * j)) & 7u : (uint32_t)s_ntri[cur.lc.cs(j)];
+        }
+        wave_lds_sync();
+        const uint32_t tb_rel = (uint32_t)tb;
+        for (uint32_t k = lane; k < nm; k += 64u) {
+            const uint32_t A = s_tl[k];
+            const uint32_t Bn = k + 1 < nm ? s_tl[k + 1] : (bns | (bnt << 16));
+            a.meshlets[(uint64_t)mb + k] = make_uint4(msb + (A & 0xFFFFu), 3u * (tb_rel + (A >> 16)),
+                                                      (Bn & 0xFFFFu) - (A & 0xFFFFu), (Bn >> 16) - (A >> 16));
+        }
+        wave_lds_sync();
+    }
+}
+
+// Meshlet vertices: copies of the shared mesh's vertices through the meshlet
+// vertex map, two lanes per vertex (one 16-byte half each), so that a wave
+// reads its map entries and writes its copies as contiguous 1 KiB runs.  A map
+// entry past the vertex buffer (a block flagged over capacity) gives zeros.
+__global__ __launch_bounds__(256) void k_meshlet_vertices(const float4* __restrict__ vertices,
+                                                          const uint32_t* __restrict__ map,
+                                                          float4* __restrict__ mverts,
+                                                          const uint64_t* __restrict__ totals, uint64_t mvcap,
+                                                          uint64_t vcap)
+{
+    const uint64_t n = 2 * min(totals[4], mvcap);
+    for (uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (uint64_t)gridDim.x * 256) {
+        const uint64_t j = t >> 1;
+        const uint32_t v = map[j];
+        float4 x = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (v < vcap) x = vertices[2 * (uint64_t)v + (t & 1)];
+        __builtin_nontemporal_store(mtx_f4v{x.x, x.y, x.z, x.w}, reinterpret_cast<mtx_f4v*>(mverts + t));
     }
 }
 
@@ -2099,6 +2690,7 @@ constexpr uint32_t kWgPerCuSingleVertices = 10;
 constexpr uint32_t kWgPerCuSingleCount = 24;
 constexpr uint32_t kWgPerCuSingleTriangles = 10;
 constexpr uint32_t kWgPerCuBatch = 64;
+constexpr uint32_t kWgPerCuMeshlet = 8;
 
 template <typename K>
 static hipError_t launch_persistent(K kernel, uint32_t threads, const ExtractArgs& a, hipStream_t s,
@@ -2140,7 +2732,21 @@ static hipError_t launch_emit_b(const ExtractArgs& a, hipStream_t s)
                                           64 * VertCfg<B, false, uint8_t>::WPG, a, s, kWgPerCuSingleVertices);
     }
     if (e != hipSuccess) return e;
-    return launch_persistent(k_extract_triangles<B, BATCH>, 64 * TriCfg<B>::WPG, a, s, kWgPerCuSingleTriangles);
+    e = launch_persistent(k_extract_triangles<B, BATCH>, 64 * TriCfg<B>::WPG, a, s, kWgPerCuSingleTriangles);
+    if (e != hipSuccess || BATCH || !a.starts) return e;
+    // meshlets (not in the batched form)
+    e = launch_persistent(k_meshlet_emit<B>, 64 * EmitCfg<B>::WPG, a, s, kWgPerCuMeshlet);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_meshlet_vertices, dim3((uint32_t)device_cus() * 8), dim3(256), 0, s,
+                       reinterpret_cast<const float4*>(a.vertices), a.mvmap, reinterpret_cast<float4*>(a.mverts),
+                       a.totals, a.mvcap, a.vcap);
+    return hipGetLastError();
+}
+
+template <int B, bool BATCH>
+static hipError_t launch_cut_b(const ExtractArgs& a, hipStream_t s)
+{
+    return launch_persistent(k_meshlet_cut<B>, 256, a, s, kWgPerCuMeshlet);
 }
 
 #define MTX_DISPATCH_B(FN, ...)                                   \
@@ -2175,10 +2781,22 @@ static hipError_t launch_emit_b(const ExtractArgs& a, hipStream_t s)
 
 bool batch_supported_block(uint32_t B) { return B == 4 || B == 8 || B == 16; }
 
+static hipError_t launch_count_single(const ExtractArgs& a, hipStream_t s)
+{
+    MTX_DISPATCH_B(launch_count_b, a, s)
+}
+
+static hipError_t launch_cut(const ExtractArgs& a, hipStream_t s)
+{
+    MTX_DISPATCH_B(launch_cut_b, a, s)
+}
+
 hipError_t launch_extract_count(const ExtractArgs& a, hipStream_t s)
 {
     if (a.units) { MTX_DISPATCH_BATCH_B(launch_count_b, a, s) }
-    MTX_DISPATCH_B(launch_count_b, a, s)
+    const hipError_t e = launch_count_single(a, s);
+    if (e != hipSuccess || !a.starts) return e;
+    return launch_cut(a, s);   // meshlet counts and start bits, from the cases
 }
 
 hipError_t launch_extract_emit(const ExtractArgs& a, hipStream_t s)
@@ -2195,7 +2813,8 @@ hipError_t launch_extract_scan(const ExtractArgs& a, TileSums ts, hipStream_t s)
     hipLaunchKernelGGL(k_extract_tile_sums, dim3((uint32_t)ntiles), dim3(256), 0, s, a.blk_counts, a.count,
                        a.max_blocks, ts.sums, a.totals);
     hipLaunchKernelGGL(k_extract_scan, dim3((uint32_t)ntiles), dim3(256), 0, s, a.blk_counts, a.count,
-                       a.max_blocks, a.blk_vbase, a.blk_tbase, a.blk_mbase, a.totals, a.out_counts, ts.sums);
+                       a.max_blocks, a.blk_vbase, a.blk_tbase, a.blk_mbase, a.blk_sbase, a.totals, a.out_counts,
+                       ts.sums);
     return hipGetLastError();
 }
 

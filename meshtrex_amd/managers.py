@@ -435,6 +435,22 @@ class ExtractionOutput:
     def read_voxel_positions(self):
         return read_device(self.c.vertex_voxel_pos, np.float32, 3 * self.c.vertexCount).reshape(-1, 3)
 
+    def read_meshlets(self):
+        """(m, 4) uint32 meshlet descriptors: vertexOffset, indexOffset, vertexCount, primitiveCount."""
+        return read_device(self.c.meshlets, np.uint32, 4 * self.c.meshletCount).reshape(-1, 4)
+
+    def read_meshlet_vertices(self):
+        """(s, 8) float32: the renderer-layout vertices the meshlets refer to."""
+        return read_device(self.c.meshlet_vertices, np.float32, 8 * self.c.meshletVertexCount).reshape(-1, 8)
+
+    def read_meshlet_vertex_map(self):
+        """Per meshlet vertex, its index in the shared vertex buffer."""
+        return read_device(self.c.meshlet_vertex_map, np.uint32, self.c.meshletVertexCount)
+
+    def read_meshlet_indices(self):
+        """3 per triangle, into the meshlet vertices (same triangle order as read_indices)."""
+        return read_device(self.c.meshlet_indices, np.uint32, self.c.indexCount)
+
     def read_case_indices(self, n_active, block_dim):
         return read_device(self.c.case_indices, np.uint8, n_active * block_dim ** 3)
 

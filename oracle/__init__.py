@@ -64,6 +64,8 @@ def lib():
                                     ctypes.c_void_p]
         _lib.orc_minmax_legacy.argtypes = _lib.orc_minmax.argtypes
         _lib.orc_pyramid.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        _lib.orc_meshlets.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint32,
+                                      ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_void_p] * 3
     return _lib
 
 
@@ -181,6 +183,28 @@ def extract(vol, dims, B, iso, normals=True):
             out[k] = out[k][:nv]
     out["tris"] = out["tris"][:nt]
     return out
+
+
+def meshlets(vol, dims, B, iso):
+    """Renderer-layout meshlets (orc_meshlets): dict desc (m, 4) uint32
+    {vertexOffset, indexOffset, vertexCount, primitiveCount}, map (s,) uint32
+    meshlet vertex -> shared-mesh vertex, mind (t, 3) uint32 meshlet vertex of
+    each triangle vertex."""
+    vol = np.ascontiguousarray(vol)
+    nm, ns = ctypes.c_uint64(), ctypes.c_uint64()
+    r = lib().orc_meshlets(_p(vol), _dt(vol), _dims(dims), B, iso, ctypes.byref(nm), ctypes.byref(ns), None, None,
+                           None)
+    if r != 0:
+        raise RuntimeError("oracle meshlets failed: %d" % r)
+    _, nt = extract_count(vol, dims, B, iso)
+    desc = np.zeros((max(nm.value, 1), 4), np.uint32)
+    mp = np.zeros(max(ns.value, 1), np.uint32)
+    mind = np.zeros((max(nt, 1), 3), np.uint32)
+    r = lib().orc_meshlets(_p(vol), _dt(vol), _dims(dims), B, iso, ctypes.byref(nm), ctypes.byref(ns), _p(desc),
+                           _p(mp), _p(mind))
+    if r != 0:
+        raise RuntimeError("oracle meshlets failed: %d" % r)
+    return {"desc": desc[:nm.value], "map": mp[:ns.value], "mind": mind[:nt]}
 
 
 def case_indices(vol, dims, B, iso, ids):

@@ -24,6 +24,9 @@
  *                       normal  src/shaders/marching_cubes_pmb.mesh.glsl:111-120
  *                       with the zero-gradient guard of
  *                       src/shaders/marching_cubes.mesh.glsl:147-149,181-183
+ *   orc_meshlets        meshlet partition of src/shaders/marching_cubes_pmb.task.glsl:182-215
+ *                       over the renderer layout (self-contained vertex ranges,
+ *                       src/shaders/render.task.glsl:61-63,125-128)
  *   orc_adjoint         d(vertex)/d(voxel), d(vertex)/d(iso) of the edge
  *                       interpolation above, topology frozen
  *
@@ -575,6 +578,136 @@ int orc_extract(const void* vol, int dtype, const uint32_t dims[3], uint32_t B, 
     free(c.map.keys);
     free(c.map.vals);
     return c.missing ? -2 : 0;
+}
+
+/* ---- meshlets ------------------------------------------------------------ */
+
+/* Meshlets of the renderer layout.  Per block (ascending block ID), the
+ * occupied cells (valid, at least one triangle) in cell order are cut greedily
+ * as src/shaders/marching_cubes_pmb.task.glsl:182-215 cuts them: a cell
+ * starts a new meshlet when it is not the meshlet's first and adding it would
+ * take the meshlet past 64 vertices or 126 triangles.  A meshlet's vertex
+ * count is the number of distinct vertices its triangles use (each meshlet
+ * carries its own copy of each, so that every index of a meshlet falls in its
+ * own vertex range: src/shaders/render.task.glsl:61-63,125-128), taken in
+ * order of first use (cell order, then the cell's edges in ascending owner
+ * index mtx_edge_uidx).  Outputs (any may be NULL; sizes from a first call with NULLs):
+ * desc[4 m + 0..3] = vertexOffset, indexOffset, vertexCount, primitiveCount;
+ * map[j] = index in the shared mesh (orc_extract's vertex order) of meshlet
+ * vertex j; mind[3 t + k] = meshlet vertex of vertex k of triangle t (the
+ * shared mesh's triangle order). */
+#define ORC_MESHLET_VERTS 64u
+#define ORC_MESHLET_PRIMS 126u
+
+typedef struct {
+    uint64_t edge[ORC_MESHLET_VERTS];
+    uint32_t n, prims;
+    uint64_t vbase, tbase;
+} orc_meshlet_t;
+
+static int mlet_find(const orc_meshlet_t* m, uint64_t eid)
+{
+    for (uint32_t i = 0; i < m->n; ++i)
+        if (m->edge[i] == eid) return (int)i;
+    return -1;
+}
+
+int orc_meshlets(const void* vol, int dtype, const uint32_t dims[3], uint32_t B, float iso, uint64_t* n_meshlets,
+                 uint64_t* n_mverts, uint32_t* desc, uint32_t* map, uint32_t* mind)
+{
+    vol_t v = { vol, dtype, { dims[0], dims[1], dims[2] }, iso };
+    *n_meshlets = 0;
+    *n_mverts = 0;
+    if (dims[0] < 2 || dims[1] < 2 || dims[2] < 2) return 0;
+    emap_t em;
+    const uint64_t nv = for_each_vertex(&v, B, NULL, NULL);
+    if (emap_init(&em, nv)) return -1;
+    {
+        ext_ctx c;
+        memset(&c, 0, sizeof(c));
+        c.v = &v;
+        c.map = em;
+        for_each_vertex(&v, B, emit_vertex, &c);
+        em = c.map;
+    }
+    uint32_t G[3] = { ceil_div(dims[0], B), ceil_div(dims[1], B), ceil_div(dims[2], B) };
+    uint64_t nm = 0, ns = 0, tri = 0;
+    orc_meshlet_t m;
+    memset(&m, 0, sizeof m);
+    for (uint32_t bz = 0; bz < G[2]; ++bz)
+        for (uint32_t by = 0; by < G[1]; ++by)
+            for (uint32_t bx = 0; bx < G[0]; ++bx) {
+                int open = 0;
+                for (uint32_t lz = 0; lz < B; ++lz)
+                    for (uint32_t ly = 0; ly < B; ++ly)
+                        for (uint32_t lx = 0; lx < B; ++lx) {
+                            uint32_t x = bx * B + lx, y = by * B + ly, z = bz * B + lz;
+                            if (x + 1 >= dims[0] || y + 1 >= dims[1] || z + 1 >= dims[2]) continue;
+                            unsigned cs = cell_case(&v, x, y, z);
+                            uint64_t w = mtx_tri_words[cs];
+                            unsigned nt = mtx_word_ntri(w);
+                            if (!nt) continue;
+                            /* the cell's edges, ascending owner index, as global edge IDs */
+                            unsigned emask = mtx_case_edge_mask(cs);
+                            uint64_t ce[12];
+                            unsigned nce = 0, nnew = 0;
+                            for (unsigned u = 0; u < 12u; ++u) {
+                                unsigned e = mtx_uidx_edge(u);
+                                if (!((emask >> e) & 1u)) continue;
+                                unsigned o = mtx_edge_owner(e);
+                                uint64_t eid = 3 * lin(&v, x + (o & 1u), y + ((o >> 1) & 1u), z + ((o >> 2) & 1u)) +
+                                               (o >> 3);
+                                ce[nce++] = eid;
+                                if (!open || mlet_find(&m, eid) < 0) ++nnew;
+                            }
+                            if (open && (m.n + nnew > ORC_MESHLET_VERTS || m.prims + nt > ORC_MESHLET_PRIMS)) {
+                                if (desc) {
+                                    desc[4 * nm + 0] = (uint32_t)m.vbase;
+                                    desc[4 * nm + 1] = (uint32_t)(3 * m.tbase);
+                                    desc[4 * nm + 2] = m.n;
+                                    desc[4 * nm + 3] = m.prims;
+                                }
+                                ++nm;
+                                ns += m.n;
+                                open = 0;
+                            }
+                            if (!open) {
+                                m.n = 0;
+                                m.prims = 0;
+                                m.vbase = ns;
+                                m.tbase = tri;
+                                open = 1;
+                            }
+                            for (unsigned i = 0; i < nce; ++i)
+                                if (mlet_find(&m, ce[i]) < 0) {
+                                    if (map) map[m.vbase + m.n] = (uint32_t)emap_get(&em, ce[i]);
+                                    m.edge[m.n++] = ce[i];
+                                }
+                            for (unsigned k = 0; k < nt; ++k, ++tri)
+                                for (unsigned j = 0; j < 3; ++j) {
+                                    unsigned o = mtx_edge_owner(mtx_word_edge(w, k, j));
+                                    uint64_t eid = 3 * lin(&v, x + (o & 1u), y + ((o >> 1) & 1u), z + ((o >> 2) & 1u)) +
+                                                   (o >> 3);
+                                    if (mind) mind[3 * tri + j] = (uint32_t)(m.vbase + (uint64_t)mlet_find(&m, eid));
+                                }
+                            m.prims += nt;
+                        }
+                if (open) {
+                    if (desc) {
+                        desc[4 * nm + 0] = (uint32_t)m.vbase;
+                        desc[4 * nm + 1] = (uint32_t)(3 * m.tbase);
+                        desc[4 * nm + 2] = m.n;
+                        desc[4 * nm + 3] = m.prims;
+                    }
+                    ++nm;
+                    ns += m.n;
+                }
+            }
+    free(em.keys);
+    free(em.vals);
+    *n_meshlets = nm;
+    *n_mverts = ns;
+    return 0;
 }
 
 /* ---- adjoint ------------------------------------------------------------- */

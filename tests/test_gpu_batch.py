@@ -3,8 +3,8 @@
 Every unit of a batch -- one (volume, isovalue) pair -- must give exactly what
 the single-volume path gives for that volume and isovalue at the same block
 edge: the same active block IDs, vertex records (bit-identical floats), index
-buffer, meshlet descriptors and owning edge IDs, with indices and meshlet
-offsets relative to the unit's first vertex / index.  One unit per dtype is
+buffer and owning edge IDs, with indices relative to the unit's first
+vertex.  The batched form writes the shared mesh only: no meshlets.  One unit per dtype is
 also compared with the oracle directly.  Replays of the captured graph must
 repeat, follow new volume contents, and flag outputs past the planned
 capacity.
@@ -55,13 +55,12 @@ def plain(ctx, vol, dims, B, iso, edge_ids=True):
     mm = computeMinMaxMip(ctx, vol, pc)
     f = filterActiveBlocks(ctx, mm, pc)
     readActiveBlockCount(ctx, f)
-    ex = extractMeshletDescriptors(ctx, mm, f, pc, options=makeOptions(edge_ids=int(edge_ids)))
+    ex = extractMeshletDescriptors(ctx, mm, f, pc, options=makeOptions(edge_ids=int(edge_ids), meshlets=0))
     ex.read_counts()
     res = {
         "ids": f.read_ids(),
         "verts": ex.read_vertices(),
         "idx": ex.read_indices(),
-        "meshlets": read_device(ex.c.meshlets, np.uint32, 4 * ex.meshletCount).reshape(-1, 4),
         "eid": ex.read_edge_ids() if edge_ids else None,
     }
     ex.cleanup()
@@ -82,7 +81,7 @@ def check_units_against_plain(ctx, b, vols):
             v, i = b.unit_mesh(u, ranges)
             np.testing.assert_array_equal(v, ref["verts"])
             np.testing.assert_array_equal(i, ref["idx"])
-            np.testing.assert_array_equal(b.unit_meshlets(u, ranges), ref["meshlets"])
+            assert r["meshlet_count"] == 0
             np.testing.assert_array_equal(b.unit_edge_ids(u, ranges), ref["eid"])
             u += 1
     return ranges

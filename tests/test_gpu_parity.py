@@ -183,7 +183,45 @@ def check_extraction(ctx, data, dims, B, iso, rule=0, normals=True):
     n_active = f.read_ids().size
     cases = ex.read_case_indices(n_active, B)
     np.testing.assert_array_equal(cases, oracle.case_indices(data, dims, B, iso, f.read_ids()))
+    check_meshlets(ex, data, dims, B, iso)
     return ex
+
+
+def renderer_accepts(desc, mind, nverts):
+    """The reference renderer's acceptance rule, restated: a meshlet of at most
+    64 vertices and 126 primitives goes through whole
+    (src/shaders/render.task.glsl:61-63), and a triangle is dropped when an
+    index falls outside [vertexOffset, vertexOffset + vertexCount)
+    (:125-128).  Returns the number of triangles dropped."""
+    dropped = 0
+    for vo, io, vc, pc in desc.astype(np.int64):
+        assert vc <= 64 and pc <= 126
+        assert io % 3 == 0 and vo + vc <= nverts
+        t = mind[io // 3: io // 3 + pc].astype(np.int64)
+        dropped += int(np.sum(np.any((t < vo) | (t >= vo + vc), axis=1)))
+    return dropped
+
+
+def check_meshlets(ex, data, dims, B, iso):
+    """Renderer-layout meshlets against the oracle's greedy cut, bit for bit;
+    every triangle survives the renderer's rule; the meshlet mesh is the shared
+    mesh (map[meshlet_indices] == indices) and its vertices are copies."""
+    ref = oracle.meshlets(data, dims, B, iso)
+    desc = ex.read_meshlets()
+    assert ex.meshletCount == len(ref["desc"]) and ex.meshletVertexCount == ref["map"].size
+    np.testing.assert_array_equal(desc, ref["desc"])
+    mp = ex.read_meshlet_vertex_map()
+    np.testing.assert_array_equal(mp, ref["map"])
+    mind = ex.read_meshlet_indices().reshape(-1, 3)
+    np.testing.assert_array_equal(mind, ref["mind"])
+    np.testing.assert_array_equal(mp[mind], ex.read_indices().reshape(-1, 3))
+    assert renderer_accepts(desc, mind, mp.size) == 0
+    if mp.size:
+        np.testing.assert_array_equal(ex.read_meshlet_vertices(), ex.read_vertices()[mp])
+    # the meshlets cover the triangles in order, without gaps
+    if len(desc):
+        assert desc[0, 1] == 0 and np.all(desc[1:, 1] == desc[:-1, 1] + 3 * desc[:-1, 3])
+        assert int(desc[-1, 1]) + 3 * int(desc[-1, 3]) == ex.indexCount
 
 
 @pytest.mark.parametrize("B", [8, 4, 3, 16])
@@ -274,9 +312,8 @@ def test_extract_reuse_mode_and_capacity(ctx):
             ex.read_counts()
 
 
-def test_block_map_and_meshlets(ctx):
-    """Dense block -> active-index map, and meshlet descriptors: each active
-    block's triangles in runs of <= 126, over the block's own vertex range."""
+def test_block_map(ctx):
+    """Dense block -> active-index map, from the fused mask and from the pairs."""
     dims = (90, 77, 61)
     data = oracle.gen_blobs_u8(dims, 91, 10)
     B, iso = 8, 128.0
@@ -291,28 +328,27 @@ def test_block_map_and_meshlets(ctx):
         want = np.full(nb, 0xFFFFFFFF, np.uint32)
         want[ids] = np.arange(ids.size, dtype=np.uint32)
         np.testing.assert_array_equal(m, want)
-    ex = extractMeshletDescriptors(ctx, mm, f, pc, options=makeOptions(edge_ids=1))
-    nv, ni, nm = ex.read_counts()
-    ref = oracle.extract(data, dims, B, iso, normals=False)
-    # per-block vertex counts from owning edges, triangle counts from case indices
-    vox = ref["edge_ids"].astype(np.int64) // 3
-    X, Y = dims[0], dims[1]
-    owner_block = ((vox % X) // B) + G[0] * (((vox // X) % Y) // B + G[1] * ((vox // (X * Y)) // B))
-    pos = np.searchsorted(ids, owner_block)
-    assert np.all(ids[pos] == owner_block)
-    bnv = np.bincount(pos, minlength=ids.size)
-    ntri = np.array([int(np.sum(r >= 0)) // 3 for r in oracle.tri_table()])
-    bnt = ntri[oracle.case_indices(data, dims, B, iso, ids).reshape(ids.size, -1)].sum(axis=1)
-    vb = np.concatenate([[0], np.cumsum(bnv)[:-1]])
-    tb = np.concatenate([[0], np.cumsum(bnt)[:-1]])
-    exp = []
-    for i in range(ids.size):
-        for k in range(-(-int(bnt[i]) // 126)):
-            exp.append((vb[i], 3 * (tb[i] + 126 * k), bnv[i], min(126, bnt[i] - 126 * k)))
-    from meshtrex_amd.managers import read_device
-    got = read_device(ex.c.meshlets, np.uint32, 4 * nm).reshape(-1, 4)
-    assert nm == len(exp)
-    np.testing.assert_array_equal(got, np.array(exp, np.uint32).reshape(-1, 4))
+
+
+def test_meshlets_off_and_reuse(ctx):
+    """meshlets=0 writes the shared mesh only (no meshlet buffers); reuse mode
+    rewrites the same meshlets into the same buffers."""
+    dims = (70, 61, 53)
+    data = oracle.gen_blobs_u8(dims, 92, 10)
+    mm, pc = run_minmax(ctx, data, dims, 8, iso=128.0)
+    f = filterActiveBlocks(ctx, mm, pc)
+    ex0 = extractMeshletDescriptors(ctx, mm, f, pc, options=makeOptions(meshlets=0))
+    ex0.read_counts()
+    assert not ex0.c.meshlets and not ex0.c.meshlet_vertices and ex0.meshletCount == 0
+    ex = extractMeshletDescriptors(ctx, mm, f, pc)
+    ex.read_counts()
+    first = (ex.read_meshlets().copy(), ex.read_meshlet_indices().copy())
+    np.testing.assert_array_equal(ex0.read_indices(), ex.read_indices())
+    extractMeshletDescriptors(ctx, mm, f, pc, options=makeOptions(reuse=1), out=ex)
+    ex.read_counts()
+    np.testing.assert_array_equal(ex.read_meshlets(), first[0])
+    np.testing.assert_array_equal(ex.read_meshlet_indices(), first[1])
+    check_meshlets(ex, data, dims, 8, 128.0)
 
 
 @pytest.mark.parametrize("B,rule", [(8, 0), (8, 1), (4, 0), (5, 1)])

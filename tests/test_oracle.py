@@ -395,3 +395,121 @@ def test_adjoint_vs_central_differences():
     p = oracle.positions(v, dims, iso, e)
     owner = np.stack([vox_ % dims[0], (vox_ // dims[0]) % dims[1], vox_ // (dims[0] * dims[1])], 1)
     np.testing.assert_allclose(p[np.arange(e.size), ax] - owner[np.arange(e.size), ax], r["t"], atol=1e-12)
+
+
+# ---- meshlets -----------------------------------------------------------------
+#
+# The oracle's greedy cut (orc_meshlets) is pinned by properties that fix it
+# uniquely, derived from the triangle list and the case indices rather than
+# from a second greedy loop: meshlets are runs of consecutive cells of one
+# block covering the triangles in order; each holds at most 64 vertices and
+# 126 triangles; each but a block's last is maximal (its block's next cell
+# would take it past a limit); its vertices are exactly those its triangles
+# use, once each, in first-use order (cell order, then owner index u within
+# the cell); map[meshlet index] is the shared mesh's index.
+
+def uidx_of(dx, dy, dz, axis):
+    """Owner index: 4 axis + d_p + 2 d_q, (p, q) the other axes ascending."""
+    d = (dx, dy, dz)
+    p, q = [a for a in range(3) if a != axis]
+    return 4 * axis + d[p] + 2 * d[q]
+
+
+def triangle_cells(data, dims, B, iso, ntris):
+    """Global cell (x, y, z) and block ID of every triangle, from the case
+    indices of the active blocks (triangles come block by block, cell by cell)."""
+    G = oracle.grid(dims, B)
+    ids = np.arange(G[0] * G[1] * G[2], dtype=np.uint32)
+    cases = oracle.case_indices(data, dims, B, iso, ids).reshape(ids.size, -1)
+    ntri = np.array([int(np.sum(r >= 0)) // 3 for r in oracle.tri_table()])
+    cells, blocks = [], []
+    for b in range(ids.size):
+        bx, by, bz = b % G[0], (b // G[0]) % G[1], b // (G[0] * G[1])
+        for c in np.nonzero(ntri[cases[b]])[0]:
+            lx, ly, lz = c % B, (c // B) % B, c // (B * B)
+            for _ in range(ntri[cases[b][c]]):
+                cells.append((bx * B + lx, by * B + ly, bz * B + lz))
+                blocks.append(b)
+    assert len(cells) == ntris
+    return np.array(cells, np.int64).reshape(-1, 3), np.array(blocks, np.int64)
+
+
+def check_meshlet_contract(data, dims, B, iso):
+    ref = oracle.extract(data, dims, B, iso, normals=False)
+    m = oracle.meshlets(data, dims, B, iso)
+    tris, eid = ref["tris"].astype(np.int64), ref["edge_ids"].astype(np.int64)
+    desc, mp, mind = m["desc"].astype(np.int64), m["map"].astype(np.int64), m["mind"].astype(np.int64)
+    np.testing.assert_array_equal(mp[mind], tris)
+    if len(tris) == 0:
+        assert len(desc) == 0 and mp.size == 0
+        return
+    cells, blocks = triangle_cells(data, dims, B, iso, len(tris))
+    X, Y = dims[0], dims[1]
+    cell_lin = cells[:, 0] + X * (cells[:, 1] + Y * cells[:, 2])
+    # runs: contiguous triangles, contiguous meshlet vertices, whole cells, one block
+    assert desc[0, 1] == 0 and desc[0, 0] == 0
+    assert np.all(desc[1:, 1] == desc[:-1, 1] + 3 * desc[:-1, 3])
+    assert np.all(desc[1:, 0] == desc[:-1, 0] + desc[:-1, 2])
+    assert desc[-1, 1] + 3 * desc[-1, 3] == 3 * len(tris) and desc[-1, 0] + desc[-1, 2] == mp.size
+    starts = desc[:, 1] // 3
+    for k in range(len(desc)):
+        t0, t1 = starts[k], starts[k] + desc[k, 3]
+        vo, vc = desc[k, 0], desc[k, 2]
+        assert 1 <= vc <= 64 and 1 <= desc[k, 3] <= 126
+        assert len(set(blocks[t0:t1])) == 1
+        if t0 > 0:
+            assert cell_lin[t0] != cell_lin[t0 - 1]          # a cell is never split
+        # the renderer keeps every triangle
+        assert np.all((mind[t0:t1] >= vo) & (mind[t0:t1] < vo + vc))
+        # exactly the vertices the triangles use, once each
+        used = tris[t0:t1].reshape(-1)
+        assert sorted(mp[vo:vo + vc].tolist()) == sorted(set(used.tolist()))
+        # first-use order: first cell using the vertex, then its owner index in that cell
+        key = {}
+        for t in range(t0, t1):
+            for v in tris[t]:
+                if v in key:
+                    continue
+                e = eid[v]
+                vox, ax = e // 3, e % 3
+                o = np.array([vox % X, (vox // X) % Y, vox // (X * Y)]) - cells[t]
+                assert np.all((o >= 0) & (o <= 1))
+                key[v] = (cell_lin[t], uidx_of(o[0], o[1], o[2], ax))
+        assert mp[vo:vo + vc].tolist() == sorted(key, key=lambda v: key[v])
+        # maximal: the block's next cell would take the meshlet past a limit
+        if k + 1 < len(desc) and blocks[starts[k + 1]] == blocks[t0]:
+            n0 = starts[k + 1]
+            n1 = n0
+            while n1 < len(tris) and cell_lin[n1] == cell_lin[n0]:
+                n1 += 1
+            more = set(tris[n0:n1].reshape(-1).tolist()) - set(used.tolist())
+            assert vc + len(more) > 64 or desc[k, 3] + (n1 - n0) > 126
+
+
+@pytest.mark.parametrize("B", [8, 4, 5, 3])
+def test_meshlet_contract_blobs(B):
+    dims = (41, 37, 30)
+    check_meshlet_contract(oracle.gen_blobs_u8(dims, oracle.SEED_BASE + 9, 10), dims, B, 128.0)
+
+
+def test_meshlet_contract_noise_and_sdf():
+    # noise: dense cells, many meshlets cut by the vertex limit
+    rng = np.random.default_rng(3)
+    dims = (21, 19, 17)
+    check_meshlet_contract(rng.integers(0, 256, int(np.prod(dims)), dtype=np.uint8), dims, 8, 127.5)
+    dims = (44, 40, 36)
+    check_meshlet_contract(oracle.gen_sdf_f32(dims, 41, 6, 3, 0.15), dims, 8, 0.0)
+
+
+def test_meshlet_known_answers():
+    """One hot voxel in a field of 0 (iso 128): its 8 cells each give one
+    triangle; with the voxel at a block's corner the cells fall in 8 blocks,
+    so 8 meshlets of one triangle and 3 vertices each."""
+    dims = (17, 17, 17)
+    data = np.zeros(dims[::-1], np.uint8)
+    data[8, 8, 8] = 255
+    m = oracle.meshlets(data.reshape(-1), dims, 8, 128.0)
+    assert len(m["desc"]) == 8 and np.all(m["desc"][:, 2:] == [3, 1])
+    # an empty surface: no meshlets
+    m = oracle.meshlets(np.zeros(8 * 8 * 8, np.uint8), (8, 8, 8), 8, 128.0)
+    assert len(m["desc"]) == 0 and m["map"].size == 0
