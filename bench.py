@@ -1,18 +1,26 @@
 """Benchmark of the isosurface-extraction path on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
-                    [--skip-batched] [--skip-adjoint]
+                    [--skip-small] [--skip-batched] [--skip-adjoint] [--skip-ingest]
 
 Headline (`value`): a step is one pass of the path over a resident 1024^3
 float32 SDF (64 spheres + 32 tori, N(0, 0.05^2) noise, isovalue 0, 8^3-cell
 blocks): block min/max with the fused activity mask, the min/max pyramid,
 ordered active-block compaction and PMB marching cubes (count, scan, emit of
-vertices with normals, index buffer, meshlet descriptors), all on one stream
-with no host synchronisation, into buffers sized by a warm-up pass.  `value`
-is Gvoxels/s of the whole step.  `roofline` describes the block min/max
-kernel (the dominant, HBM-bound kernel) from HIP events around its launch on
-its own stream; `cpu_baseline` times the CPU oracle (scalar C, 1 thread) on a
-z-slab of the same volume on this host.
+vertices with normals, index buffer, meshlets), all on one stream with no
+host synchronisation, into buffers sized by a warm-up pass.  `value` is
+Gvoxels/s over the K timed steps (hipEvents bracketing them); each step's own
+events give the median beside it.  `roofline` describes the block min/max
+kernel (the dominant, HBM-bound kernel) from HIP events around its launch
+inside the timed steps; stage times come from a separate pass.
+`cpu_baseline` times the CPU oracle on a z-slab of the same volume on this
+host at T threads (1 thread beside it).
+
+`extra.small_volumes`: the 256^3 uint8 blob volume (isovalue 128) and the
+512^3 uint8 vessel volume (isovalue 80), same stages, median of >= 20 steps
+with a 512 MiB flush before each; the CPU oracle on the whole volumes at 1
+and T threads; the reference's own CPU min/max (compiled from its headers)
+against the GPU min/max in the same mode.
 
 `extra.batched`: 48 volumes of uneven dimensions (64-384 per axis, none a
 multiple of 4; uint8 blobs and float32 SDFs alternating) with 1-8 isovalues
@@ -60,32 +68,192 @@ def parse_traffic():
     return None
 
 
-def cpu_baseline(vol_ptr, dims, iso, planes):
-    """The CPU oracle (scalar C, 1 thread) on the first `planes` z-planes."""
+def cpu_threads():
+    """T for the threaded CPU runs: the CPUs this process may use, at most 16."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_path(host, dims, iso, threads):
+    """The CPU oracle's min/max, filter and extraction (with normals) over
+    z-slabs of 8-plane block rows, one slab per thread.  Each slab carries the
+    plane above its last block row, so its min/max pairs and its cells are the
+    whole volume's (only the x/y edges of that shared plane are extracted
+    twice).  Stages run one after another, each across all threads."""
     import numpy as np
     import oracle
-    from meshtrex_amd.managers import read_device
-    X, Y = dims[0], dims[1]
-    n = X * Y * planes
-    host = read_device(vol_ptr, np.float32, n)
-    sd = (X, Y, planes)
-    t0 = time.perf_counter()
-    pairs = oracle.minmax(host, sd, 8, 1)
-    ids = oracle.filter_ids(pairs, oracle.F32, iso)
-    ref = oracle.extract(host, sd, 8, iso, normals=True)
-    dt = time.perf_counter() - t0
-    return {"value": round(n / dt / 1e9, 5), "unit": "Gvoxels/s", "cores": 1, "kind": "port",
-            "sample": "z-slab %dx%dx%d (%d voxels) of the same 1024^3 f32 volume, iso %g: oracle "
-                      "min-max + filter + full extraction with normals (%d active blocks, %d "
-                      "triangles) in %.2f s" % (X, Y, planes, n, iso, ids.size, len(ref["tris"]), dt),
-            "triangles_per_s": round(len(ref["tris"]) / dt, 1)}
+    from concurrent.futures import ThreadPoolExecutor
+    X, Y, Z = dims
+    rows = -(-Z // 8)
+    cuts = np.linspace(0, rows, min(threads, rows) + 1).astype(int)
+    slabs, own = [], []
+    for a, b in zip(cuts[:-1], cuts[1:]):
+        z0, z1 = 8 * int(a), min(Z, 8 * int(b) + 1)
+        slabs.append((host[z0 * X * Y: z1 * X * Y], (X, Y, z1 - z0)))
+        own.append(-(-X // 8) * -(-Y // 8) * int(b - a))   # blocks of the slab's own rows
+    dt = oracle.U8 if host.dtype == np.uint8 else oracle.F32
+    with ThreadPoolExecutor(len(slabs)) as ex:
+        t0 = time.perf_counter()
+        pairs = list(ex.map(lambda sl: oracle.minmax(sl[0], sl[1], 8, 1), slabs))
+        t1 = time.perf_counter()
+        ids = list(ex.map(lambda pr: oracle.filter_ids(pr, dt, iso), pairs))
+        t2 = time.perf_counter()
+        meshes = list(ex.map(lambda sl: oracle.extract(sl[0], sl[1], 8, iso, normals=True), slabs))
+        t3 = time.perf_counter()
+    nvox = X * Y * Z
+    tris = sum(len(m["tris"]) for m in meshes)
+    return {"threads": len(slabs), "minmax_s": round(t1 - t0, 4), "filter_s": round(t2 - t1, 4),
+            "extract_s": round(t3 - t2, 4), "total_s": round(t3 - t0, 4),
+            "Gvoxels_per_s": round(nvox / (t3 - t0) / 1e9, 5), "triangles_per_s": round(tris / (t3 - t0), 1),
+            "triangles": tris, "active_blocks": sum(int(np.count_nonzero(x < n)) for x, n in zip(ids, own))}
+
+
+def cpu_both(host, dims, iso, T):
+    one = cpu_path(host, dims, iso, 1)
+    many = cpu_path(host, dims, iso, T) if T > 1 else one
+    return {"1": one, "T": many}
+
+
+class Path:
+    """One resident volume through the path -- block min/max with the fused
+    activity mask, the min/max pyramid, ordered compaction, PMB extraction
+    with normals (and meshlets unless meshlets=0) -- into outputs sized by a
+    first, allocating pass; one stream, no host synchronisation in a step."""
+
+    def __init__(self, ctx, stream, vol_ptr, dims, dtype, iso, meshlets=1):
+        from meshtrex_amd import _lib as L
+        from meshtrex_amd import hiprt
+        from meshtrex_amd.managers import (ExtractionOutput, FilteringOutput, MinMaxOutput, _check, makeOptions,
+                                           makePushConstants)
+        self.ctx, self.stream, self.dims = ctx, stream, dims
+        self.nvox = dims[0] * dims[1] * dims[2]
+        self.esize = 4 if dtype == "float32" else 1
+        self.pc = makePushConstants(dims, 8, iso)
+        self.cvol = L.CVolume()
+        self.cvol.data, self.cvol.dims[:], self.cvol.on_device = vol_ptr, dims, 1
+        self.cvol.dtype = L.MTX_F32 if dtype == "float32" else L.MTX_U8
+        self.o_mm = makeOptions(build_pyramid=0, reuse=1)
+        self.o_ex = makeOptions(reuse=1, meshlets=meshlets)
+        self.mm, self.filt, self.ex = MinMaxOutput(ctx), FilteringOutput(ctx), ExtractionOutput(ctx)
+        self.lib, self.h, self._check = L.lib(), ctx.handle, _check
+        self._call(self.lib.mtx_compute_min_max, ctypes.byref(self.cvol), ctypes.byref(self.pc),
+                   ctypes.byref(self.o_mm), stream, ctypes.byref(self.mm.c))
+        self._call(self.lib.mtx_build_pyramid, ctypes.byref(self.mm.c), stream)
+        self._call(self.lib.mtx_filter_active_blocks, ctypes.byref(self.mm.c), ctypes.byref(self.pc), None, stream,
+                   ctypes.byref(self.filt.c))
+        self._call(self.lib.mtx_extract, ctypes.byref(self.mm.c), ctypes.byref(self.filt.c), ctypes.byref(self.pc),
+                   ctypes.byref(makeOptions(meshlets=meshlets)), stream, ctypes.byref(self.ex.c))
+        hiprt.stream_sync(stream)
+
+    def _call(self, fn, *a):
+        self._check(self.ctx, fn(self.h, *a))
+
+    def step(self, ev=(None,) * 5):
+        """ev: events recorded before min/max, after min/max, after the
+        pyramid, after the filter and after the extraction (None: skipped)."""
+        s = self.stream
+        if ev[0]:
+            ev[0].record(s)
+        self._call(self.lib.mtx_compute_min_max, ctypes.byref(self.cvol), ctypes.byref(self.pc),
+                   ctypes.byref(self.o_mm), s, ctypes.byref(self.mm.c))
+        if ev[1]:
+            ev[1].record(s)
+        self._call(self.lib.mtx_build_pyramid, ctypes.byref(self.mm.c), s)
+        if ev[2]:
+            ev[2].record(s)
+        self._call(self.lib.mtx_filter_active_blocks, ctypes.byref(self.mm.c), ctypes.byref(self.pc), None, s,
+                   ctypes.byref(self.filt.c))
+        if ev[3]:
+            ev[3].record(s)
+        self._call(self.lib.mtx_extract, ctypes.byref(self.mm.c), ctypes.byref(self.filt.c), ctypes.byref(self.pc),
+                   ctypes.byref(self.o_ex), s, ctypes.byref(self.ex.c))
+        if ev[4]:
+            ev[4].record(s)
+
+    def counts(self):
+        nv, ni, nm = self.ex.read_counts()
+        self._check(self.ctx, self.lib.mtx_read_active_count(self.h, ctypes.byref(self.filt.c)))
+        return {"active_blocks": int(self.filt.activeBlockCount), "vertices": nv, "triangles": ni // 3,
+                "meshlets": nm, "meshlet_vertices": self.ex.meshletVertexCount}
+
+    def close(self):
+        self.ex.cleanup()
+        self.filt.cleanup()
+        self.mm.cleanup()
+
+
+def time_path(path, steps, warmup, flush=None):
+    """hipEvent times of `steps` steps after `warmup`: the bracket over all of
+    them, and each step's and its min/max launch's own events.  With `flush`,
+    a 512 MiB device write runs before each step, outside the step's events
+    (and inside the bracket, which then is not a per-step figure)."""
+    import statistics
+    from meshtrex_amd import hiprt
+    s = path.stream
+    for _ in range(warmup):
+        if flush:
+            flush()
+        path.step()
+    hiprt.stream_sync(s)
+    evs = [[hiprt.Event() for _ in range(3)] for _ in range(steps)]
+    start, stop = hiprt.Event(), hiprt.Event()
+    w0 = time.perf_counter()
+    start.record(s)
+    for k in range(steps):
+        if flush:
+            flush()
+        path.step((evs[k][0], evs[k][1], None, None, evs[k][2]))
+    stop.record(s)
+    stop.sync()
+    wall = time.perf_counter() - w0
+    per = [e[2].ms_since(e[0]) for e in evs]
+    mm = [e[1].ms_since(e[0]) for e in evs]
+    return {"bracket_ms": stop.ms_since(start) / steps, "median_ms": statistics.median(per),
+            "mean_ms": statistics.fmean(per), "min_ms": min(per), "max_ms": max(per),
+            "minmax_mean_ms": statistics.fmean(mm), "minmax_median_ms": statistics.median(mm), "wall_s": wall}
+
+
+def stage_times(path, steps):
+    """Mean time of each stage, from events between the stages (a pass of its own)."""
+    from meshtrex_amd import hiprt
+    evs = [[hiprt.Event() for _ in range(5)] for _ in range(steps)]
+    for k in range(steps):
+        path.step(evs[k])
+    evs[-1][4].sync()
+    names = ("minmax_leaf", "pyramid", "filter", "extract")
+    return {n: round(sum(e[i + 1].ms_since(e[i]) for e in evs) / steps, 4) for i, n in enumerate(names)}
+
+
+def make_flush(stream):
+    """A 512 MiB device copy: evicts the 256 MiB Infinity Cache between steps."""
+    from meshtrex_amd import _lib as L
+    from meshtrex_amd.managers import DeviceBuffer
+    n = 512 << 20
+    a, b = DeviceBuffer(n), DeviceBuffer(n)
+
+    def flush():
+        assert L.synth().mtx_bench_copy(b.ptr, a.ptr, n, stream) == 0
+    flush.buffers = (a, b)
+    return flush
 
 
 def bench_plain(args, ctx, stream):
     from meshtrex_amd import _lib as L
     from meshtrex_amd import hiprt
-    from meshtrex_amd.managers import (DeviceBuffer, ExtractionOutput, FilteringOutput, MinMaxOutput, _check,
-                                       makeOptions, makePushConstants)
+    from meshtrex_amd.managers import DeviceBuffer
     S = args.size
     dims = (S, S, S)
     nvox = S ** 3
@@ -95,64 +263,17 @@ def bench_plain(args, ctx, stream):
     rc = L.synth().mtx_synth_sdf_f32(vol.ptr, d3, SEED, 64, 32, S / 1024.0, 0.05, stream)
     assert rc == 0, rc
     gen_s = time.perf_counter() - t0
-    iso = 0.0
-    pc = makePushConstants(dims, 8, iso)
-    cvol = L.CVolume()
-    cvol.data, cvol.dims[:], cvol.dtype, cvol.on_device = vol.ptr, dims, L.MTX_F32, 1
-    o_mm = makeOptions(build_pyramid=0, reuse=1)
-    o_ex = makeOptions(reuse=1)
-    mm, filt, ex = MinMaxOutput(ctx), FilteringOutput(ctx), ExtractionOutput(ctx)
-    lib = L.lib()
-    h = ctx.handle
+    path = Path(ctx, stream, vol.ptr, dims, "float32", 0.0, meshlets=1)
+    t = time_path(path, args.steps, args.warmup)
+    stages = stage_times(path, args.steps)
+    cnt = path.counts()
+    ms = t["bracket_ms"]
 
-    # sizing pass: allocating extraction (one host sync inside)
-    _check(ctx, lib.mtx_compute_min_max(h, ctypes.byref(cvol), ctypes.byref(pc), ctypes.byref(o_mm),
-                                        stream, ctypes.byref(mm.c)))
-    _check(ctx, lib.mtx_build_pyramid(h, ctypes.byref(mm.c), stream))
-    _check(ctx, lib.mtx_filter_active_blocks(h, ctypes.byref(mm.c), ctypes.byref(pc), None, stream,
-                                             ctypes.byref(filt.c)))
-    _check(ctx, lib.mtx_extract(h, ctypes.byref(mm.c), ctypes.byref(filt.c), ctypes.byref(pc),
-                                ctypes.byref(makeOptions()), stream, ctypes.byref(ex.c)))
-    hiprt.stream_sync(stream)
-
-    def step(events=None):
-        if events:
-            events[0].record(stream)
-        _check(ctx, lib.mtx_compute_min_max(h, ctypes.byref(cvol), ctypes.byref(pc), ctypes.byref(o_mm),
-                                            stream, ctypes.byref(mm.c)))
-        if events:
-            events[1].record(stream)
-        _check(ctx, lib.mtx_build_pyramid(h, ctypes.byref(mm.c), stream))
-        if events:
-            events[2].record(stream)
-        _check(ctx, lib.mtx_filter_active_blocks(h, ctypes.byref(mm.c), ctypes.byref(pc), None, stream,
-                                                 ctypes.byref(filt.c)))
-        if events:
-            events[3].record(stream)
-        _check(ctx, lib.mtx_extract(h, ctypes.byref(mm.c), ctypes.byref(filt.c), ctypes.byref(pc),
-                                    ctypes.byref(o_ex), stream, ctypes.byref(ex.c)))
-        if events:
-            events[4].record(stream)
-
-    for _ in range(args.warmup):
-        step()
-    hiprt.stream_sync(stream)
-
-    per_step = [[hiprt.Event() for _ in range(5)] for _ in range(args.steps)]
-    start, stop = hiprt.Event(), hiprt.Event()
-    wall0 = time.perf_counter()
-    start.record(stream)
-    for k in range(args.steps):
-        step(per_step[k])
-    stop.record(stream)
-    stop.sync()
-    wall = time.perf_counter() - wall0
-    total_ms = stop.ms_since(start)
-    ms = total_ms / args.steps
-    stages = {"minmax_leaf": 0.0, "pyramid": 0.0, "filter": 0.0, "extract": 0.0}
-    for e in per_step:
-        for i, name in enumerate(stages):
-            stages[name] += e[i + 1].ms_since(e[i]) / args.steps
+    # the same step writing the shared mesh only (no meshlets)
+    p0 = Path(ctx, stream, vol.ptr, dims, "float32", 0.0, meshlets=0)
+    t0m = time_path(p0, args.steps, args.warmup)
+    st0 = stage_times(p0, args.steps)
+    p0.close()
 
     # achievable HBM rates on this GPU: a read-only stream over the 4 GiB volume
     # and a plain 1 GiB device copy (read + write), same stream
@@ -180,19 +301,19 @@ def bench_plain(args, ctx, stream):
     ca.free()
     cb.free()
 
-    nv, ni, nm = ex.read_counts()
-    nmv = ex.meshletVertexCount
-    na = int(filt.read_ids().size)
+    nv, ntri, nm, nmv, na = (cnt["vertices"], cnt["triangles"], cnt["meshlets"], cnt["meshlet_vertices"],
+                             cnt["active_blocks"])
     nb = ((S + 7) // 8) ** 3
-    ntri = ni // 3
     leaf_bytes = 4 * nvox + 8 * nb
     mmf_bytes = 4 * nvox + 8 * nb + 4 * na + 4
     ext_bytes = na * 8 ** 3 * 4 + 32 * nv + 12 * ntri + 16 * nm
     # everything the extraction writes: the shared mesh, the meshlet descriptors
     # and the renderer layout (meshlet vertices, their map, meshlet indices)
     ext_out_bytes = 32 * nv + 12 * ntri + 16 * nm + (32 + 4) * nmv + 12 * ntri
-    leaf_gbs = leaf_bytes / (stages["minmax_leaf"] * 1e-3) / 1e9
-    mmf_gbs = mmf_bytes / ((stages["minmax_leaf"] + stages["filter"]) * 1e-3) / 1e9
+    # the roofline kernel's time: its own events, inside the timed steps
+    leaf_gbs = leaf_bytes / (t["minmax_mean_ms"] * 1e-3) / 1e9
+    mmf_ms = stages["minmax_leaf"] + stages["filter"]
+    mmf_gbs = mmf_bytes / (mmf_ms * 1e-3) / 1e9
     res = {
         "ms": ms, "nvox": nvox, "dims": dims,
         "config": {"workload": "plain single extraction, %d^3 float32 SDF (64 spheres + 32 tori + "
@@ -203,26 +324,112 @@ def bench_plain(args, ctx, stream):
         "roofline": {"bound": "hbm", "achieved": round(leaf_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(leaf_gbs / HBM_PEAK_GBS, 4), "traffic": parse_traffic(),
                      "kernel": "k_minmax_b8<float> (block min/max + fused activity mask)",
+                     "kernel_ms": round(t["minmax_mean_ms"], 4),
                      "algorithmic_bytes_per_launch": leaf_bytes,
                      "achievable_read_GBps": round(read_gbs, 1),
                      "frac_of_achievable_read": round(leaf_gbs / read_gbs, 4),
                      "achievable_copy_GBps": round(copy_gbs, 1)},
         "extra": {
-            "stage_ms": {k: round(v, 4) for k, v in stages.items()},
-            "minmax_plus_filter": {"bytes": mmf_bytes, "ms": round(stages["minmax_leaf"] + stages["filter"], 4),
+            "step_ms": {k: round(t[k], 4) for k in ("bracket_ms", "median_ms", "mean_ms", "min_ms", "max_ms")},
+            "Gvoxels_per_s_at_median": round(nvox / (t["median_ms"] * 1e-3) / 1e9, 3),
+            "stage_ms": stages,
+            "stage_ms_note": "a separate pass with events between the stages, after the timed steps",
+            "minmax_plus_filter": {"bytes": mmf_bytes, "ms": round(mmf_ms, 4),
                                    "GBps": round(mmf_gbs, 1), "frac_of_peak": round(mmf_gbs / HBM_PEAK_GBS, 4),
                                    "target_frac": 0.70},
-            "extraction": {"algorithmic_bytes": ext_bytes, "ms": round(stages["extract"], 4),
+            "extraction": {"algorithmic_bytes": ext_bytes, "ms": stages["extract"],
                            "GBps": round(ext_bytes / (stages["extract"] * 1e-3) / 1e9, 1),
                            "bytes_in_plus_all_outputs": na * 8 ** 3 * 4 + ext_out_bytes,
                            "GBps_all_outputs": round((na * 8 ** 3 * 4 + ext_out_bytes) /
                                                      (stages["extract"] * 1e-3) / 1e9, 1)},
+            "shared_mesh_only": {"ms": round(t0m["bracket_ms"], 4), "median_ms": round(t0m["median_ms"], 4),
+                                 "Gvoxels_per_s": round(nvox / (t0m["bracket_ms"] * 1e-3) / 1e9, 3),
+                                 "extract_ms": st0["extract"],
+                                 "note": "the same step with meshlets=0: vertices, normals and the index "
+                                         "buffer, no meshlet descriptors or renderer layout"},
             "triangles_per_s": round(ntri / (ms * 1e-3), 1),
-            "wall_s_timed": round(wall, 4),
+            "wall_s_timed": round(t["wall_s"], 4),
             "generate_s": round(gen_s, 3),
         },
-        "_keep": (vol, mm, filt, ex),
+        "_keep": (vol, path),
     }
+    return res
+
+
+def bench_small(args, ctx, stream, kind, dims, seed, iso, flush, name, workload):
+    """A workload that fits the 256 MiB Infinity Cache: median of >= 20 steps,
+    each after a 512 MiB flush."""
+    from meshtrex_amd import workloads
+    buf, vol = workloads.generate(dims, kind, seed, stream)
+    path = Path(ctx, stream, buf.ptr, dims, vol.volume_type, iso, meshlets=1)
+    steps = max(20, args.steps)
+    t = time_path(path, steps, 3, flush=flush)
+    cnt = path.counts()
+    path.close()
+    nvox = path.nvox
+    med = t["median_ms"]
+    log("%s: median %.4f ms/step, %.1f Gvox/s" % (name, med, nvox / (med * 1e-3) / 1e9))
+    out = {"workload": workload, "steps": steps, "warmup": 3,
+           "median_ms": round(med, 4), "mean_ms": round(t["mean_ms"], 4), "min_ms": round(t["min_ms"], 4),
+           "Gvoxels_per_s": round(nvox / (med * 1e-3) / 1e9, 3),
+           "triangles_per_s": round(cnt["triangles"] / (med * 1e-3), 1),
+           "minmax_median_ms": round(t["minmax_median_ms"], 4),
+           "flush": "512 MiB device copy before each step, outside the step's events", **cnt}
+    return out, buf
+
+
+def bench_reference_minmax(ctx, stream, buf, dims, flush):
+    """The reference's own CPU min/max + active count (src/testMinMax.h:28-135,
+    220-242; compiled from the reference's headers) on the 256^3 blob volume,
+    beside the GPU min/max in the same mode (4^3 blocks, no overlap)."""
+    import statistics
+    import subprocess
+    import tempfile
+    import numpy as np
+    from meshtrex_amd import _lib as L
+    from meshtrex_amd import hiprt
+    from meshtrex_amd.managers import MinMaxOutput, makeOptions, makePushConstants, read_device
+    drv = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+    res = {}
+    if os.path.exists(drv):
+        host = read_device(buf.ptr, np.uint8, int(np.prod(dims)))
+        with tempfile.TemporaryDirectory() as d:
+            f = os.path.join(d, "blobs256.raw")
+            host.tofile(f)
+            r = subprocess.run([drv, "time", f, "128", "21"], capture_output=True, text=True, timeout=120)
+        if r.returncode == 0 and r.stdout.startswith("SECONDS"):
+            w = r.stdout.split()
+            res["reference_cpu"] = {"ms": round(1e3 * float(w[1]), 3), "active_blocks": int(w[3]), "threads": 1,
+                                    "what": "computeMinMaxFromFile (file read included) + "
+                                            "computeActiveBlockCountCPU, median of 21 runs"}
+        else:
+            res["reference_cpu"] = {"error": (r.stdout + r.stderr)[-300:]}
+    else:
+        res["reference_cpu"] = {"error": "oracle/_ref/ref_driver not built"}
+    pc = makePushConstants(dims, 4, 128.0)
+    cv = L.CVolume()
+    cv.data, cv.dims[:], cv.dtype, cv.on_device = buf.ptr, dims, L.MTX_U8, 1
+    o = makeOptions(build_pyramid=0, reuse=1, overlap=0, fuse_filter=0)
+    mm = MinMaxOutput(ctx)
+    lib = L.lib()
+    assert lib.mtx_compute_min_max(ctx.handle, ctypes.byref(cv), ctypes.byref(pc), ctypes.byref(o), stream,
+                                   ctypes.byref(mm.c)) == 0
+    per = []
+    for _ in range(21):
+        flush()
+        e0, e1 = hiprt.Event(), hiprt.Event()
+        e0.record(stream)
+        assert lib.mtx_compute_min_max(ctx.handle, ctypes.byref(cv), ctypes.byref(pc), ctypes.byref(o), stream,
+                                       ctypes.byref(mm.c)) == 0
+        e1.record(stream)
+        e1.sync()
+        per.append(e1.ms_since(e0))
+    mm.cleanup()
+    res["gpu_minmax_same_mode"] = {"ms": round(statistics.median(per), 4),
+                                   "what": "mtx_compute_min_max, 4^3 blocks, no overlap, median of 21 after a "
+                                           "512 MiB flush each"}
+    if "ms" in res["reference_cpu"]:
+        res["speedup"] = round(res["reference_cpu"]["ms"] / res["gpu_minmax_same_mode"]["ms"], 1)
     return res
 
 
@@ -556,8 +763,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-planes", type=int, default=320,
-                    help="z-planes of the volume the CPU baseline processes")
+    ap.add_argument("--cpu-planes", type=int, default=192,
+                    help="z-planes of the headline volume the CPU baseline processes")
+    ap.add_argument("--skip-small", action="store_true",
+                    help="leave out the 256^3 blob and 512^3 vessel workloads")
     ap.add_argument("--skip-plain", action="store_true",
                     help="profiling aid: leave out the headline workload (value is then null)")
     ap.add_argument("--skip-batched", action="store_true")
@@ -609,6 +818,8 @@ def main():
 
 
 def headline(args, ctx, stream):
+    import numpy as np
+    from meshtrex_amd.managers import read_device
     log("plain %d^3 f32: %d warmup + %d timed steps" % (args.size, args.warmup, args.steps))
     p = bench_plain(args, ctx, stream)
     ms = p["ms"]
@@ -630,14 +841,54 @@ def headline(args, ctx, stream):
         "cpu_baseline": None,
         "extra": p["extra"],
     }
-    log("plain: %.4f ms/step, %.1f Gvox/s" % (ms, line["value"]))
-    vol, mm, filt, ex = p.pop("_keep")
-    ex.cleanup()
-    filt.cleanup()
-    mm.cleanup()
+    log("plain: %.4f ms/step (median %.4f), %.1f Gvox/s" % (ms, p["extra"]["step_ms"]["median_ms"], line["value"]))
+    vol, path = p.pop("_keep")
+    path.close()
+    T = cpu_threads()
+    S = args.size
     if not args.no_cpu_baseline:
-        log("cpu baseline: %d z-planes" % args.cpu_planes)
-        line["cpu_baseline"] = cpu_baseline(vol.ptr, p["dims"], 0.0, min(args.cpu_planes, args.size))
+        P = min(args.cpu_planes, S)
+        log("cpu baseline: %d z-planes of the %d^3 volume, 1 and %d threads" % (P, S, T))
+        host = read_device(vol.ptr, np.float32, S * S * P)
+        c = cpu_both(host, (S, S, P), 0.0, T)
+        del host
+        line["cpu_baseline"] = {
+            "value": c["T"]["Gvoxels_per_s"], "unit": "Gvoxels/s", "cores": c["T"]["threads"], "kind": "port",
+            "sample": "z-slab %dx%dx%d (%d voxels) of the same %d^3 f32 volume, iso 0: oracle min-max + filter "
+                      "+ extraction with normals, split over %d threads by 8-plane block rows, in %.2f s "
+                      "(%d triangles)" % (S, S, P, S * S * P, S, c["T"]["threads"], c["T"]["total_s"],
+                                           c["T"]["triangles"]),
+            "one_thread": c["1"], "threads_T": c["T"], "cpu_model": cpu_model()}
+    line["extra"]["cpu"] = {"T": T, "cpu_model": cpu_model(), "os_cpu_count": os.cpu_count(),
+                            "T_rule": "the CPUs this process may use (sched_getaffinity), at most 16"}
+    if not args.skip_small:
+        flush = make_flush(stream)
+        small = {}
+        o, buf0 = bench_small(args, ctx, stream, "u8", (256, 256, 256), SEED - 2, 128.0, flush, "blobs 256^3 u8",
+                              "256^3 uint8, 16 Gaussian blobs + integer noise [-2,2], isovalue 128, 8^3 blocks; "
+                              "min-max + pyramid + filter + extraction with normals and meshlets")
+        small["blobs256_u8"] = o
+        o, buf1 = bench_small(args, ctx, stream, "u8vessels", (512, 512, 512), SEED - 1, 80.0, flush,
+                              "vessels 512^3 u8",
+                              "512^3 uint8, 40 polylines of 8 segments (length 20-80, radius 2-6) over a "
+                              "0-20 background, isovalue 80, 8^3 blocks; same stages")
+        small["vessels512_u8"] = o
+        small["reference_minmax_blobs256"] = bench_reference_minmax(ctx, stream, buf0, (256, 256, 256), flush)
+        if not args.no_cpu_baseline:
+            log("cpu: blobs 256^3 and vessels 512^3, 1 and %d threads" % T)
+            h0 = read_device(buf0.ptr, np.uint8, 256 ** 3)
+            small["blobs256_u8"]["cpu"] = cpu_both(h0, (256, 256, 256), 128.0, T)
+            del h0
+            h1 = read_device(buf1.ptr, np.uint8, 512 ** 3)
+            small["vessels512_u8"]["cpu"] = cpu_both(h1, (512, 512, 512), 80.0, T)
+            del h1
+            for k in ("blobs256_u8", "vessels512_u8"):
+                g = small[k]["Gvoxels_per_s"]
+                small[k]["speedup_vs_cpu_1_thread"] = round(g / small[k]["cpu"]["1"]["Gvoxels_per_s"], 1)
+                small[k]["speedup_vs_cpu_T_threads"] = round(g / small[k]["cpu"]["T"]["Gvoxels_per_s"], 1)
+        buf0.free()
+        buf1.free()
+        line["extra"]["small_volumes"] = small
     if not args.skip_ingest:
         log("ingest from host memory")
         line["extra"]["ingest"] = bench_ingest(args, ctx, vol.ptr, p["dims"], p["extra"]["stage_ms"]["minmax_leaf"])

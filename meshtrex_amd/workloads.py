@@ -6,7 +6,8 @@
 
 Volume recipes (shared with the CPU oracle through csrc/synth_recipes.h):
 uint8 blobs -- 16 Gaussian blobs, quantised to 0..255 plus integer noise in
-[-2, 2]; float32 SDF -- min over 64 spheres and 32 tori with radii scaled by
+[-2, 2]; uint8 vessels -- 40 polylines of 8 segments (length 20-80, radius
+2-6), 255 exp(-d^2 / 2r^2) over a uniform 0..20 background; float32 SDF -- min over 64 spheres and 32 tori with radii scaled by
 min(dim)/1024, plus N(0, 0.05^2); float32 blobs -- the blob field rescaled to
 [-1, 1].  Seeds are 0x6D747278 + k (k = 3 for the batched set, 4 for the
 differentiable set) mixed with the volume's index.
@@ -63,6 +64,10 @@ def generate(dims, kind, seed, stream=None):
         buf = DeviceBuffer(4 * n)
         rc = syn.mtx_synth_sdf_f32(buf.ptr, d3, seed, 64, 32, min(dims) / 1024.0, 0.05, stream)
         vt = "float32"
+    elif kind == "u8vessels":
+        buf = DeviceBuffer(n)
+        rc = syn.mtx_synth_vessels_u8(buf.ptr, d3, seed, 40, 8, stream)
+        vt = "uint8"
     elif kind == "f32blobs":
         buf = DeviceBuffer(4 * n)
         rc = syn.mtx_synth_blobs_f32(buf.ptr, d3, seed, 16, stream)

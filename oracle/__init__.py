@@ -52,6 +52,9 @@ def lib():
                                          ctypes.c_int, ctypes.c_double, ctypes.c_double]
         _lib.orc_extract_count.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint32,
                                            ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p]
+        _lib.orc_extract_count_rows.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint32,
+                                                ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p,
+                                                ctypes.c_void_p]
         _lib.orc_extract.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint32,
                                      ctypes.c_float] + [ctypes.c_void_p] * 5
         _lib.orc_case_indices.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint32,
@@ -161,6 +164,26 @@ def extract_count(vol, dims, B, iso):
     nt = ctypes.c_uint64()
     lib().orc_extract_count(_p(vol), _dt(vol), _dims(dims), B, iso, ctypes.byref(nv), ctypes.byref(nt))
     return nv.value, nt.value
+
+
+def extract_count_threaded(vol, dims, B, iso, threads=16):
+    """extract_count split over block rows (orc_extract_count_rows) on a
+    thread pool; ctypes releases the GIL, so the rows run in parallel."""
+    from concurrent.futures import ThreadPoolExecutor
+    vol = np.ascontiguousarray(vol)
+    rows = -(-dims[2] // B)
+    cuts = np.linspace(0, rows, min(threads, rows) + 1).astype(np.int64)
+    d = _dims(dims)
+
+    def part(k):
+        nv, nt = ctypes.c_uint64(), ctypes.c_uint64()
+        lib().orc_extract_count_rows(_p(vol), _dt(vol), d, B, iso, int(cuts[k]), int(cuts[k + 1]),
+                                     ctypes.byref(nv), ctypes.byref(nt))
+        return nv.value, nt.value
+
+    with ThreadPoolExecutor(len(cuts) - 1) as ex:
+        parts = list(ex.map(part, range(len(cuts) - 1)))
+    return sum(p[0] for p in parts), sum(p[1] for p in parts)
 
 
 def extract(vol, dims, B, iso, normals=True):

@@ -441,12 +441,14 @@ static void corner_normal(const vol_t* v, const int64_t p[3], double n[3])
  * in block-local x-fastest order, then axes x, y, z.  Calls fn for each. */
 typedef void (*vert_fn)(void* ctx, uint64_t idx, uint32_t x, uint32_t y, uint32_t z, int axis);
 
-static uint64_t for_each_vertex(const vol_t* v, uint32_t B, vert_fn fn, void* ctx)
+static uint64_t for_each_vertex_rows(const vol_t* v, uint32_t B, uint32_t bz0, uint32_t bz1, vert_fn fn,
+                                     void* ctx)
 {
     uint32_t G[3] = { ceil_div(v->d[0], B), ceil_div(v->d[1], B), ceil_div(v->d[2], B) };
     uint64_t n = 0;
     if (v->d[0] < 2 || v->d[1] < 2 || v->d[2] < 2) return 0;
-    for (uint32_t bz = 0; bz < G[2]; ++bz)
+    if (bz1 > G[2]) bz1 = G[2];
+    for (uint32_t bz = bz0; bz < bz1; ++bz)
         for (uint32_t by = 0; by < G[1]; ++by)
             for (uint32_t bx = 0; bx < G[0]; ++bx)
                 for (uint32_t lz = 0; lz < B; ++lz)
@@ -467,12 +469,14 @@ static uint64_t for_each_vertex(const vol_t* v, uint32_t B, vert_fn fn, void* ct
  * block-local x-fastest order). */
 typedef void (*cell_fn)(void* ctx, uint32_t x, uint32_t y, uint32_t z, unsigned c);
 
-static uint64_t for_each_cell(const vol_t* v, uint32_t B, cell_fn fn, void* ctx)
+static uint64_t for_each_cell_rows(const vol_t* v, uint32_t B, uint32_t bz0, uint32_t bz1, cell_fn fn,
+                                   void* ctx)
 {
     uint32_t G[3] = { ceil_div(v->d[0], B), ceil_div(v->d[1], B), ceil_div(v->d[2], B) };
     uint64_t ntri = 0;
     if (v->d[0] < 2 || v->d[1] < 2 || v->d[2] < 2) return 0;
-    for (uint32_t bz = 0; bz < G[2]; ++bz)
+    if (bz1 > G[2]) bz1 = G[2];
+    for (uint32_t bz = bz0; bz < bz1; ++bz)
         for (uint32_t by = 0; by < G[1]; ++by)
             for (uint32_t bx = 0; bx < G[0]; ++bx)
                 for (uint32_t lz = 0; lz < B; ++lz)
@@ -486,6 +490,27 @@ static uint64_t for_each_cell(const vol_t* v, uint32_t B, cell_fn fn, void* ctx)
                             ntri += nt;
                         }
     return ntri;
+}
+
+static uint64_t for_each_vertex(const vol_t* v, uint32_t B, vert_fn fn, void* ctx)
+{
+    return for_each_vertex_rows(v, B, 0, UINT32_MAX, fn, ctx);
+}
+
+static uint64_t for_each_cell(const vol_t* v, uint32_t B, cell_fn fn, void* ctx)
+{
+    return for_each_cell_rows(v, B, 0, UINT32_MAX, fn, ctx);
+}
+
+/* Vertex and triangle counts of the blocks in block rows bz0 <= bz < bz1.
+ * Every vertex belongs to the block of its owning corner and every cell to
+ * one block, so the mesh's counts are the sums over any split of the rows. */
+void orc_extract_count_rows(const void* vol, int dtype, const uint32_t dims[3], uint32_t B, float iso,
+                            uint32_t bz0, uint32_t bz1, uint64_t* nvert, uint64_t* ntri)
+{
+    vol_t v = { vol, dtype, { dims[0], dims[1], dims[2] }, iso };
+    *nvert = for_each_vertex_rows(&v, B, bz0, bz1, NULL, NULL);
+    *ntri = for_each_cell_rows(&v, B, bz0, bz1, NULL, NULL);
 }
 
 /* Vertex and triangle counts of the mesh. */

@@ -4,6 +4,10 @@
 //   ref_driver minmax <volume.raw> <pairs.out>   computeMinMaxFromFile (src/testMinMax.h:28-135)
 //   ref_driver count  <pairs.in> <iso>           computeActiveBlockCountCPU (src/testMinMax.h:220-242)
 //   ref_driver tables <tables.out>               MarchingCubes::triTable/edgeTable/numVerticesTable (src/mc_tables.h)
+//   ref_driver time   <volume.raw> <iso> <reps>  median wall time of computeMinMaxFromFile followed by
+//                                                computeActiveBlockCountCPU (the reference's CPU path as a
+//                                                whole, file read included; its console output discarded)
+#include <chrono>
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -40,6 +44,26 @@ int main(int argc, char** argv)
         std::fclose(f);
         uint32_t n = computeActiveBlockCountCPU(r, std::strtof(argv[3], nullptr));
         std::printf("ACTIVE %u\n", n);
+        return 0;
+    }
+    if (mode == "time" && argc == 5) {
+        const float iso = std::strtof(argv[3], nullptr);
+        const int reps = std::atoi(argv[4]);
+        if (reps < 1) return 2;
+        std::vector<double> secs;
+        uint32_t n = 0;
+        std::streambuf* out = std::cout.rdbuf();
+        std::cout.rdbuf(nullptr);
+        for (int k = 0; k < reps; ++k) {
+            auto t0 = std::chrono::steady_clock::now();
+            std::vector<MinMaxResult> r = computeMinMaxFromFile(argv[2]);
+            n = computeActiveBlockCountCPU(r, iso);
+            secs.push_back(std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+        }
+        std::cout.clear();
+        std::cout.rdbuf(out);
+        std::sort(secs.begin(), secs.end());
+        std::printf("SECONDS %.6f ACTIVE %u REPS %d\n", secs[secs.size() / 2], n, reps);
         return 0;
     }
     if (mode == "tables") {

@@ -13,10 +13,9 @@ import numpy as np
 import pytest
 
 import oracle
-from meshtrex_amd import (DeviceBuffer, MtxError, Volume, computeMinMaxMip, extractMeshletDescriptors,
-                          filterActiveBlocks, makeOptions, makePushConstants, read_device,
-                          readActiveBlockCount)
+from meshtrex_amd import DeviceBuffer, MtxError, Volume
 from meshtrex_amd.batch import Batch
+from parity_checks import check_units_against_plain
 
 pytestmark = pytest.mark.gpu
 
@@ -48,43 +47,6 @@ def volumes():
         buf = DeviceBuffer.from_numpy(data)
         out.append((data, dims, Volume(dims, "uint8" if kind == "u8" else "float32", buf), isos, buf))
     return out
-
-
-def plain(ctx, vol, dims, B, iso, edge_ids=True):
-    pc = makePushConstants(dims, B, iso)
-    mm = computeMinMaxMip(ctx, vol, pc)
-    f = filterActiveBlocks(ctx, mm, pc)
-    readActiveBlockCount(ctx, f)
-    ex = extractMeshletDescriptors(ctx, mm, f, pc, options=makeOptions(edge_ids=int(edge_ids), meshlets=0))
-    ex.read_counts()
-    res = {
-        "ids": f.read_ids(),
-        "verts": ex.read_vertices(),
-        "idx": ex.read_indices(),
-        "eid": ex.read_edge_ids() if edge_ids else None,
-    }
-    ex.cleanup()
-    f.cleanup()
-    mm.cleanup()
-    return res
-
-
-def check_units_against_plain(ctx, b, vols):
-    ranges = b.read_ranges()
-    u = 0
-    for data, dims, vol, isos, _ in vols:
-        for iso in isos:
-            r = ranges[u]
-            assert r["flags"] == 0
-            ref = plain(ctx, vol, dims, r["block_dim"], iso)
-            np.testing.assert_array_equal(b.unit_active_ids(u, ranges), ref["ids"])
-            v, i = b.unit_mesh(u, ranges)
-            np.testing.assert_array_equal(v, ref["verts"])
-            np.testing.assert_array_equal(i, ref["idx"])
-            assert r["meshlet_count"] == 0
-            np.testing.assert_array_equal(b.unit_edge_ids(u, ranges), ref["eid"])
-            u += 1
-    return ranges
 
 
 @pytest.mark.parametrize("B", [4, 8, 16])

@@ -13,33 +13,12 @@ import numpy as np
 import pytest
 
 import oracle
-from meshtrex_amd import (DeviceBuffer, Volume, computeMinMaxMip, extractMeshletDescriptors,
-                          filterActiveBlocks, makeOptions, makePushConstants, readActiveBlockCount)
+from meshtrex_amd import (computeMinMaxMip, extractMeshletDescriptors, filterActiveBlocks, makeOptions,
+                          makePushConstants, readActiveBlockCount)
+from parity_checks import check_extraction, check_meshlets, extract_both, rand_volume, run_minmax, vol_of
 
 pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
-
-
-def rand_volume(dims, dtype, seed):
-    rng = np.random.default_rng(seed)
-    n = int(np.prod(dims))
-    if dtype == np.uint8:
-        return rng.integers(0, 256, n, dtype=np.uint8)
-    v = rng.standard_normal(n).astype(np.float32)
-    v[rng.integers(0, n, 16)] = 0.0
-    v[rng.integers(0, n, 16)] = -0.0
-    return v
-
-
-def vol_of(data, dims, device=True):
-    t = "uint8" if data.dtype == np.uint8 else "float32"
-    return Volume(dims, t, DeviceBuffer.from_numpy(data) if device else data)
-
-
-def run_minmax(ctx, data, dims, B, iso=0.5, **opt):
-    pc = makePushConstants(dims, B, iso)
-    mm = computeMinMaxMip(ctx, vol_of(data, dims), pc, options=makeOptions(**opt))
-    return mm, pc
 
 
 # ---- block min/max -------------------------------------------------------------
@@ -140,89 +119,6 @@ def test_filter_generic_block_and_large_grid(ctx):
 
 
 # ---- extraction ---------------------------------------------------------------
-
-def extract_both(ctx, data, dims, B, iso, rule=0, normals=True):
-    mm, pc = run_minmax(ctx, data, dims, B, iso=iso, rule=rule)
-    f = filterActiveBlocks(ctx, mm, pc, options=makeOptions(rule=rule))
-    readActiveBlockCount(ctx, f)
-    opts = makeOptions(rule=rule, normals=int(normals), edge_ids=1, voxel_positions=1, case_indices=1)
-    ex = extractMeshletDescriptors(ctx, mm, f, pc, options=opts)
-    ex.read_counts()
-    return mm, f, ex, oracle.extract(data, dims, B, iso, normals=normals)
-
-
-def check_extraction(ctx, data, dims, B, iso, rule=0, normals=True):
-    mm, f, ex, ref = extract_both(ctx, data, dims, B, iso, rule, normals)
-    nv, nt = len(ref["edge_ids"]), len(ref["tris"])
-    assert (ex.vertexCount, ex.indexCount) == (nv, 3 * nt)
-    eid = ex.read_edge_ids()
-    np.testing.assert_array_equal(eid, ref["edge_ids"])
-    tris = ex.read_indices().reshape(-1, 3)
-    np.testing.assert_array_equal(tris, ref["tris"])
-    np.testing.assert_array_equal(oracle.canonical_triangles(tris, eid),
-                                  oracle.canonical_triangles(ref["tris"], ref["edge_ids"]))
-    if nv:
-        # the contract's record: (owning edge ID, t), t within 1e-6 of FP64
-        assert np.max(np.abs(ex.read_t().astype(np.float64) - ref["t"])) <= 1e-6
-        # voxel-space positions: one fp32 rounding of owner + t
-        vp = ex.read_voxel_positions().astype(np.float64)
-        axis = (eid % 3).astype(np.int64)
-        vox = eid // 3
-        X, Y = dims[0], dims[1]
-        owner = np.stack([vox % X, (vox // X) % Y, vox // (X * Y)], axis=1).astype(np.float64)
-        p64 = owner.copy()
-        p64[np.arange(nv), axis] += ref["t"]
-        assert np.all(np.abs(vp - p64) <= 2.0 ** -23 * np.maximum(np.abs(p64), 1.0) + 1e-6)
-        verts = ex.read_vertices()
-        assert np.max(np.abs(verts[:, :3] - ref["ndc"])) <= 2e-6
-        assert np.all(verts[:, 3] == 1.0)
-        if normals:
-            n = verts[:, 4:7].astype(np.float64)
-            cosang = np.clip(np.sum(n * ref["normal"], axis=1) / np.linalg.norm(n, axis=1), -1, 1)
-            assert np.max(np.arccos(cosang)) <= 1e-4
-    n_active = f.read_ids().size
-    cases = ex.read_case_indices(n_active, B)
-    np.testing.assert_array_equal(cases, oracle.case_indices(data, dims, B, iso, f.read_ids()))
-    check_meshlets(ex, data, dims, B, iso)
-    return ex
-
-
-def renderer_accepts(desc, mind, nverts):
-    """The reference renderer's acceptance rule, restated: a meshlet of at most
-    64 vertices and 126 primitives goes through whole
-    (src/shaders/render.task.glsl:61-63), and a triangle is dropped when an
-    index falls outside [vertexOffset, vertexOffset + vertexCount)
-    (:125-128).  Returns the number of triangles dropped."""
-    dropped = 0
-    for vo, io, vc, pc in desc.astype(np.int64):
-        assert vc <= 64 and pc <= 126
-        assert io % 3 == 0 and vo + vc <= nverts
-        t = mind[io // 3: io // 3 + pc].astype(np.int64)
-        dropped += int(np.sum(np.any((t < vo) | (t >= vo + vc), axis=1)))
-    return dropped
-
-
-def check_meshlets(ex, data, dims, B, iso):
-    """Renderer-layout meshlets against the oracle's greedy cut, bit for bit;
-    every triangle survives the renderer's rule; the meshlet mesh is the shared
-    mesh (map[meshlet_indices] == indices) and its vertices are copies."""
-    ref = oracle.meshlets(data, dims, B, iso)
-    desc = ex.read_meshlets()
-    assert ex.meshletCount == len(ref["desc"]) and ex.meshletVertexCount == ref["map"].size
-    np.testing.assert_array_equal(desc, ref["desc"])
-    mp = ex.read_meshlet_vertex_map()
-    np.testing.assert_array_equal(mp, ref["map"])
-    mind = ex.read_meshlet_indices().reshape(-1, 3)
-    np.testing.assert_array_equal(mind, ref["mind"])
-    np.testing.assert_array_equal(mp[mind], ex.read_indices().reshape(-1, 3))
-    assert renderer_accepts(desc, mind, mp.size) == 0
-    if mp.size:
-        np.testing.assert_array_equal(ex.read_meshlet_vertices(), ex.read_vertices()[mp])
-    # the meshlets cover the triangles in order, without gaps
-    if len(desc):
-        assert desc[0, 1] == 0 and np.all(desc[1:, 1] == desc[:-1, 1] + 3 * desc[:-1, 3])
-        assert int(desc[-1, 1]) + 3 * int(desc[-1, 3]) == ex.indexCount
-
 
 @pytest.mark.parametrize("B", [8, 4, 3, 16])
 def test_extract_blobs_u8(ctx, B):

@@ -22,7 +22,7 @@ DIMS = [(37, 45, 29), (338, 63, 17), (5, 7, 3), (64, 64, 64)]
 def gpu(kind, dims, seed):
     buf, _ = workloads.generate(dims, kind, seed)
     n = int(np.prod(dims))
-    out = read_device(buf.ptr, np.uint8 if kind == "u8" else np.float32, n)
+    out = read_device(buf.ptr, np.uint8 if kind.startswith("u8") else np.float32, n)
     buf.free()
     return out
 
@@ -45,3 +45,12 @@ def test_blobs_f32_and_sdf_match_oracle(dims, seed):
     g = gpu("f32", dims, seed)
     c = oracle.gen_sdf_f32(dims, seed, 64, 32, min(dims) / 1024.0, 0.05)
     assert np.max(np.abs(g - c) / np.maximum(np.abs(c), 1.0)) <= 1e-6
+
+
+@pytest.mark.parametrize("dims", DIMS)
+def test_vessels_u8_match_oracle(dims):
+    seed = oracle.SEED_BASE + 1
+    g = gpu("u8vessels", dims, seed)
+    c = oracle.gen_vessels_u8(dims, seed, 40, 8)
+    d = np.abs(g.astype(np.int32) - c.astype(np.int32))
+    assert d.max() <= 1 and np.mean(d != 0) <= 1e-4

@@ -246,6 +246,8 @@ def test_extract_counts_brute_force(B):
         nv, nt = oracle.extract_count(v, dims, B, iso)
         assert nv == sign_change_edges(v3, iso)
         assert nt == brute_ntri(v3, iso)
+        # split over block rows, uneven row counts per thread: the same sums
+        assert oracle.extract_count_threaded(v, dims, B, iso, threads=3) == (nv, nt)
 
 
 def python_mc(vol3, iso):
