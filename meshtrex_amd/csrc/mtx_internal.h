@@ -191,7 +191,7 @@ struct AdjointArgs {
     const float* dl_dpos;       // 3 per vertex
     float* dl_dvol;             // one per voxel
     float* dl_diso;             // one float
-    double* iso_partials;       // one double per adjoint wave
+    double* iso_partials;       // one double per adjoint workgroup
     uint32_t n_wg;
 };
 uint32_t adjoint_workgroups(uint64_t nb);

@@ -282,6 +282,72 @@ __global__ __launch_bounds__(256) void k_minmax_b8(
     }
 }
 
+// Volumes too small to fill the GPU with block columns: a workgroup per
+// (256-voxel x tile, y block, z block); its four waves take planes 0-2, 3-4,
+// 5-6 and 7-8 of the block row's nine and merge them through LDS, so a wave
+// waits for at most three planes in a row instead of nine.
+template <typename T, bool ALIGNED, bool MASK>
+__global__ __launch_bounds__(256) void k_minmax_b8_zsplit(
+    const T* __restrict__ vol, uint32_t X, uint32_t Y, uint32_t Z, uint32_t gx, uint32_t gy, uint32_t bzb,
+    uint32_t bze, uint32_t ntx, uint2* __restrict__ pairs, uint32_t* __restrict__ mask, uint32_t wpr,
+    ActParams act)
+{
+    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t tx = tile % ntx;
+    const uint32_t rest = tile / ntx;
+    const uint32_t by = rest % gy;
+    const uint32_t bz = bzb + rest / gy;
+    if (bz >= bze) return;  // the whole workgroup
+    const uint32_t lane = lane_id();
+    const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    __shared__ __align__(16) T s_stage[ALIGNED ? 1 : 4][ALIGNED ? 1 : 8 * 256];
+    __shared__ MMAcc s_acc[3][64];
+    T* stage = s_stage[ALIGNED ? 0u : w];
+    const uint32_t x0 = tx * 256u + lane * 4u;
+    const bool lane_ok = x0 < X;
+    const uint32_t xe = tx * 256u + 256u;
+    const bool xe_ok = xe < X;
+    const uint32_t y0 = by * 8u;
+    const uint64_t XY = (uint64_t)X * Y;
+    const uint32_t p0 = w == 0 ? 0u : 2u * w + 1u;  // 0, 3, 5, 7
+    const uint32_t p1 = w == 0 ? 3u : p0 + 2u;       // 3, 5, 7, 9
+    MMAcc acc[1];
+#pragma unroll 1
+    for (uint32_t p = p0; p < p1; ++p) {
+        const uint32_t z = min(bz * 8u + p, Z - 1u);
+        mm_plane<T, ALIGNED, 1>(acc, p != p0, vol, (uint64_t)z * XY, X, Y, y0, x0, lane_ok, xe, xe_ok, stage,
+                                z == Z - 1u);
+    }
+    if (w) s_acc[w - 1][lane] = acc[0];
+    __syncthreads();
+    if (w) return;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) mm_merge(acc[0], s_acc[k][lane]);
+    // x combine: pair (2k, 2k+1) plus lane 2k+2's first voxel
+    const uint32_t bx = tx * 32u + (lane >> 1);
+    uint32_t bmin = min(acc[0].amin, __shfl_xor(acc[0].amin, 1, 64));
+    uint32_t bmax = max(acc[0].amax, __shfl_xor(acc[0].amax, 1, 64));
+    const uint32_t src = min(lane + 2u, 63u);
+    uint32_t nmin = __shfl(acc[0].emin, (int)src, 64);
+    uint32_t nmax = __shfl(acc[0].emax, (int)src, 64);
+    if (lane >= 62u) { nmin = acc[0].xmin; nmax = acc[0].xmax; }
+    bmin = min(bmin, nmin);
+    bmax = max(bmax, nmax);
+    const bool writer = !(lane & 1u) && bx < gx;
+    const uint64_t id = bx + (uint64_t)gx * (by + (uint64_t)gy * bz);
+    if (writer) pairs[id] = make_uint2(bmin, bmax);
+    if (MASK) {
+        uint64_t bal = __ballot(writer && block_active(bmin, bmax, act));
+        uint64_t x = bal & 0x5555555555555555ull;
+        x = (x | (x >> 1)) & 0x3333333333333333ull;
+        x = (x | (x >> 2)) & 0x0F0F0F0F0F0F0F0Full;
+        x = (x | (x >> 4)) & 0x00FF00FF00FF00FFull;
+        x = (x | (x >> 8)) & 0x0000FFFF0000FFFFull;
+        x = (x | (x >> 16)) & 0x00000000FFFFFFFFull;
+        if (lane == 0) mask[((uint64_t)bz * gy + by) * wpr + tx] = (uint32_t)x;
+    }
+}
+
 // Compute units of the current device.
 static int device_cus()
 {
@@ -295,6 +361,7 @@ static int device_cus()
 struct MMPlan {
     uint32_t M, wx, ntxg, nygrp, zc, ncz;
     uint64_t nwg;
+    double cost;   // modelled seconds
 };
 
 // Model of a plan's duration: the larger of its HBM time -- the volume plus
@@ -338,9 +405,23 @@ static MMPlan plan_minmax(const Grid& g, uint32_t ntx, const int (&resident)[3],
         if (cost < best_cost) {
             best_cost = cost;
             best = p;
+            best.cost = cost;
         }
     }
     return best;
+}
+
+// The plane-split kernel's modelled duration: three planes of 1.5 batches of
+// rows per wave, one round per device-full of workgroups; the volume plus the
+// shared row and plane of each block read twice.
+static double zsplit_cost(const Grid& g, uint32_t ntx, int resident, double bytes)
+{
+    const uint64_t nwg = (uint64_t)ntx * g.gy * g.gz;
+    const uint64_t per_round = (uint64_t)device_cus() * (uint64_t)(resident > 0 ? resident : 1);
+    const double rounds = (double)((nwg + per_round - 1) / per_round);
+    const double t_hbm = bytes * (1.0 + 1.0 / 8.0 + 1.0 / 8.0) / kMmReadBps;
+    const double t_lat = rounds * 3.0 * 1.5 * kMmBatchSec;
+    return t_hbm > t_lat ? t_hbm : t_lat;
 }
 
 template <typename T, bool ALIGNED, bool MASK, int M>
@@ -361,7 +442,10 @@ static hipError_t launch_mm_b8_t(const VolDesc& v, const Grid& g, uint32_t bzb, 
     const bool aligned = (v.X % 4u) == 0 && (((uintptr_t)v.data) % (4 * sizeof(T))) == 0;
     // workgroups per CU of the aligned and the unaligned (LDS-staged) kernels
     static int resident[2][3] = { { 0, 0, 0 }, { 0, 0, 0 } };
+    static int resident_zs[2] = { 0, 0 };
     if (!resident[0][0]) {
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident_zs[0], k_minmax_b8_zsplit<T, true, MASK>, 256, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident_zs[1], k_minmax_b8_zsplit<T, false, MASK>, 256, 0);
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident[0][0], k_minmax_b8<T, true, MASK, 1>, 256, 0);
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident[0][1], k_minmax_b8<T, true, MASK, 2>, 256, 0);
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident[0][2], k_minmax_b8<T, true, MASK, 4>, 256, 0);
@@ -379,6 +463,21 @@ static hipError_t launch_mm_b8_t(const VolDesc& v, const Grid& g, uint32_t bzb, 
     if (p.nwg == 0 || p.nwg > 0x7FFFFFFFull) return hipErrorInvalidValue;
     ActParams a{};
     if (act) a = *act;
+    if (rows == 0 && zsplit_cost(gr, ntx, resident_zs[aligned ? 0 : 1], bytes) < p.cost) {
+        const uint64_t nwg = (uint64_t)ntx * g.gy * (bze - bzb);
+        if (nwg > 0x7FFFFFFFull) return hipErrorInvalidValue;
+        auto* pr = reinterpret_cast<uint2*>(pairs);
+        const uint32_t wpr = (g.gx + 31u) / 32u;
+        if (aligned)
+            hipLaunchKernelGGL((k_minmax_b8_zsplit<T, true, MASK>), dim3((uint32_t)nwg), dim3(256), 0, s,
+                               static_cast<const T*>(v.data), v.X, v.Y, v.Z, g.gx, g.gy, bzb, bze, ntx, pr, mask,
+                               wpr, a);
+        else
+            hipLaunchKernelGGL((k_minmax_b8_zsplit<T, false, MASK>), dim3((uint32_t)nwg), dim3(256), 0, s,
+                               static_cast<const T*>(v.data), v.X, v.Y, v.Z, g.gx, g.gy, bzb, bze, ntx, pr, mask,
+                               wpr, a);
+        return hipGetLastError();
+    }
 #define MTX_MM_CASE(MM)                                                                            \
     if (p.M == MM)                                                                                 \
         return aligned ? launch_mm_m<T, true, MASK, MM>(v, g, bzb, bze, p, ntx, pairs, mask, a, s) \
@@ -2824,139 +2923,203 @@ hipError_t launch_extract_scan(const ExtractArgs& a, TileSums ts, hipStream_t s)
 // p = owner + t * e_axis (voxel units), t = (iso - v0) / (v1 - v0) with the
 // topology of the forward pass frozen:
 //   dt/dv0 = (t - 1) / (v1 - v0),  dt/dv1 = -t / (v1 - v0),  dt/diso = 1 / (v1 - v0).
-// Every voxel belongs to one home block (the block holding it as a corner); a
-// wave per home block gathers, for each of its voxels, the contributions of
-// the voxel's owned edges (it is v0) and of the three edges owned by its -x,
-// -y, -z neighbours (it is v1), looking the vertices up in the forward pass's
-// corner tables and recomputing t from the volume.  Each voxel is written once
-// (zeros where no edge touches it): no atomics, a deterministic result.
-// dL/diso is summed per wave in a fixed order and the per-wave partials are
-// added in order by a second kernel.
+// Every voxel belongs to one home block (the block holding it as a corner).
+// A voxel's gradient gathers the active edges it ends: its own +x/+y/+z edges
+// (it is v0) and the edges of its -x/-y/-z neighbour voxels (it is v1), so only
+// the home blocks of active blocks and of their +x/+y/+z neighbours can be
+// nonzero.  Three launches:
+//   k_adjoint_zero  dL/dvolume = 0 everywhere, 16-byte stores;
+//   k_adjoint       a wave per active block gathers the gradient of every voxel
+//                   of the home blocks it answers for (its own, and each
+//                   inactive +x/+y/+z neighbour not claimed by an active block
+//                   nearer in the order -x, -y, -z), looking the vertices up in
+//                   the forward pass's corner tables and recomputing t from the
+//                   volume; each such voxel is written once, lanes on
+//                   consecutive x (no atomics, a deterministic result);
+//   k_adjoint_iso   the per-workgroup dL/diso partials added in a fixed order.
 // ---------------------------------------------------------------------------
 
+__global__ __launch_bounds__(256) void k_adjoint_zero(float* __restrict__ out, uint64_t n)
+{
+    const uint64_t i0 = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * 256u;
+    if (reinterpret_cast<uintptr_t>(out) % 16u == 0) {
+        float4* __restrict__ o4 = reinterpret_cast<float4*>(out);
+        const uint64_t n4 = n / 4;
+        for (uint64_t i = i0; i < n4; i += stride) o4[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        for (uint64_t i = 4 * n4 + i0; i < n; i += stride) out[i] = 0.0f;
+    } else {
+        for (uint64_t i = i0; i < n; i += stride) out[i] = 0.0f;
+    }
+}
+
+// The gradient of every voxel of home block b (its corner tables and those of
+// its -x, -y, -z neighbours give the vertices; the volume gives t).
 template <int B, typename T>
-__global__ __launch_bounds__(256) void k_adjoint(AdjointArgs a)
+__device__ __forceinline__ void adjoint_home(const AdjointArgs& a, float* s_val, uint32_t b, double& diso)
 {
     constexpr int E = B + 2;  // voxels [-1, B] per axis
     constexpr int NTILE = E * E * E;
     constexpr int NC = EB<B>::NC;
     constexpr int CPL = EB<B>::CPL;
     using Entry = typename EB<B>::Entry;
-    __shared__ float s_all[4][NTILE];
-    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
-    float* s_val = s_all[w];
+    const uint32_t lane = lane_id();
     const ExtractArgs& f = a.fwd;
     const T* __restrict__ vol = static_cast<const T*>(f.vol.data);
     const uint32_t X = f.vol.X, Y = f.vol.Y, Z = f.vol.Z;
     const float iso = f.iso;
     const Entry* __restrict__ table = static_cast<const Entry*>(f.corner_table);
-    const uint32_t gx = f.grid.gx, gy = f.grid.gy, gz = f.grid.gz;
-    const uint32_t nb = gx * gy * gz;
-    const WaveItems it = wave_items(nb, 4);
-    double diso = 0.0;
-    for (uint32_t b = it.first; b < it.end; b += it.stride) {
-        const uint32_t bx = b % gx, by = (b / gx) % gy, bz = b / (gx * gy);
-        const uint32_t ox = bx * B, oy = by * B, oz = bz * B;
-        const int32_t s = (int32_t)f.blk_map[b];
-        const int32_t nxm = bx > 0 ? (int32_t)f.blk_map[b - 1] : -1;
-        const int32_t nym = by > 0 ? (int32_t)f.blk_map[b - gx] : -1;
-        const int32_t nzm = bz > 0 ? (int32_t)f.blk_map[b - gx * gy] : -1;
-        const bool work = s >= 0 || nxm >= 0 || nym >= 0 || nzm >= 0;
-        if (work) {
-            for (uint32_t j = lane; j < (uint32_t)NTILE; j += 64u) {
-                const int32_t x = min(max((int32_t)(ox + j % E) - 1, 0), (int32_t)X - 1);
-                const int32_t y = min(max((int32_t)(oy + (j / E) % E) - 1, 0), (int32_t)Y - 1);
-                const int32_t z = min(max((int32_t)(oz + j / (E * E)) - 1, 0), (int32_t)Z - 1);
-                s_val[j] = vfloat(vol[(uint32_t)x + (uint64_t)X * ((uint32_t)y + (uint64_t)Y * (uint32_t)z)]);
-            }
-            wave_lds_sync();
-        }
-        const uint64_t vbs = s >= 0 ? f.blk_vbase[s] : 0;
-#pragma unroll
-        for (int j = 0; j < CPL; ++j) {
-            const uint32_t c = lane * CPL + j;
-            if (c >= (uint32_t)NC) continue;
-            const uint32_t lx = c % B, ly = (c / B) % B, lz = c / (B * B);
-            const uint32_t x = ox + lx, y = oy + ly, z = oz + lz;
-            if (!(x < X && y < Y && z < Z)) continue;
-            float g = 0.0f;
-            if (work) {
-                const uint32_t t0 = (lx + 1) + E * ((ly + 1) + E * (lz + 1));
-                const float vg = s_val[t0];
-                if (s >= 0) {
-                    const uint32_t e = table[(uint64_t)s * NC + c];
-#pragma unroll
-                    for (uint32_t ax = 0; ax < 3; ++ax) {
-                        if (!((e >> ax) & 1u)) continue;
-                        const uint64_t vi = vbs + (e >> 3) + __popc(e & 7u & ((1u << ax) - 1u));
-                        const float G = a.dl_dpos[3 * vi + ax];
-                        const uint32_t st = ax == 0 ? 1u : (ax == 1 ? (uint32_t)E : (uint32_t)(E * E));
-                        const float d = s_val[t0 + st] - vg;
-                        if (!(fabsf(d) > 1e-5f)) continue;  // t = 0.5: no dependence on the values
-                        const float t = (iso - vg) / d;
-                        g += G * (t - 1.0f) / d;
-                        diso += (double)(G / d);
-                    }
-                }
-#pragma unroll
-                for (uint32_t ax = 0; ax < 3; ++ax) {
-                    const uint32_t l = ax == 0 ? lx : (ax == 1 ? ly : lz);
-                    const uint32_t gcoord = ax == 0 ? x : (ax == 1 ? y : z);
-                    if (gcoord == 0) continue;  // no voxel at -1
-                    int32_t n;
-                    uint32_t oc;
-                    if (l == 0) {
-                        n = ax == 0 ? nxm : (ax == 1 ? nym : nzm);
-                        oc = ax == 0 ? (B - 1) + B * (ly + B * lz)
-                                     : (ax == 1 ? lx + B * ((B - 1) + B * lz) : lx + B * (ly + B * (B - 1)));
-                    } else {
-                        n = s;
-                        oc = c - (ax == 0 ? 1u : (ax == 1 ? (uint32_t)B : (uint32_t)(B * B)));
-                    }
-                    if (n < 0) continue;
-                    const uint32_t e = table[(uint64_t)n * NC + oc];
-                    if (!((e >> ax) & 1u)) continue;
-                    const uint64_t vi = f.blk_vbase[n] + (e >> 3) + __popc(e & 7u & ((1u << ax) - 1u));
-                    const float G = a.dl_dpos[3 * vi + ax];
-                    const uint32_t st = ax == 0 ? 1u : (ax == 1 ? (uint32_t)E : (uint32_t)(E * E));
-                    const float v0 = s_val[t0 - st];
-                    const float d = vg - v0;
-                    if (!(fabsf(d) > 1e-5f)) continue;
-                    const float t = (iso - v0) / d;
-                    g += -G * t / d;
-                }
-            }
-            a.dl_dvol[x + (uint64_t)X * (y + (uint64_t)Y * z)] = g;
-        }
-        if (work) wave_lds_sync();
+    const uint32_t gx = f.grid.gx, gy = f.grid.gy;
+    const uint32_t bx = b % gx, by = (b / gx) % gy, bz = b / (gx * gy);
+    const uint32_t ox = bx * B, oy = by * B, oz = bz * B;
+    const int32_t s = (int32_t)f.blk_map[b];
+    const int32_t nxm = bx > 0 ? (int32_t)f.blk_map[b - 1] : -1;
+    const int32_t nym = by > 0 ? (int32_t)f.blk_map[b - gx] : -1;
+    const int32_t nzm = bz > 0 ? (int32_t)f.blk_map[b - gx * gy] : -1;
+    for (uint32_t j = lane; j < (uint32_t)NTILE; j += 64u) {
+        const int32_t x = min(max((int32_t)(ox + j % E) - 1, 0), (int32_t)X - 1);
+        const int32_t y = min(max((int32_t)(oy + (j / E) % E) - 1, 0), (int32_t)Y - 1);
+        const int32_t z = min(max((int32_t)(oz + j / (E * E)) - 1, 0), (int32_t)Z - 1);
+        s_val[j] = vfloat(vol[(uint32_t)x + (uint64_t)X * ((uint32_t)y + (uint64_t)Y * (uint32_t)z)]);
     }
-    // fixed-order wave reduction, one partial per wave
+    wave_lds_sync();
+    const uint64_t vbs = s >= 0 ? f.blk_vbase[s] : 0;
 #pragma unroll
-    for (int d = 32; d > 0; d >>= 1) diso += __shfl_xor(diso, d, 64);
-    if (lane == 0) a.iso_partials[blockIdx.x * 4 + w] = diso;
+    for (int k = 0; k < CPL; ++k) {
+        const uint32_t c = lane + 64u * (uint32_t)k;   // consecutive lanes: consecutive x
+        if (c >= (uint32_t)NC) break;
+        const uint32_t lx = c % B, ly = (c / B) % B, lz = c / (B * B);
+        const uint32_t x = ox + lx, y = oy + ly, z = oz + lz;
+        if (!(x < X && y < Y && z < Z)) continue;
+        const uint32_t t0 = (lx + 1) + E * ((ly + 1) + E * (lz + 1));
+        const float vg = s_val[t0];
+        float g = 0.0f;
+        if (s >= 0) {
+            const uint32_t e = table[(uint64_t)s * NC + c];
+#pragma unroll
+            for (uint32_t ax = 0; ax < 3; ++ax) {
+                if (!((e >> ax) & 1u)) continue;
+                const uint64_t vi = vbs + (e >> 3) + __popc(e & 7u & ((1u << ax) - 1u));
+                const float G = a.dl_dpos[3 * vi + ax];
+                const uint32_t st = ax == 0 ? 1u : (ax == 1 ? (uint32_t)E : (uint32_t)(E * E));
+                const float d = s_val[t0 + st] - vg;
+                if (!(fabsf(d) > 1e-5f)) continue;  // t = 0.5: no dependence on the values
+                const float t = (iso - vg) / d;
+                g += G * (t - 1.0f) / d;
+                diso += (double)(G / d);
+            }
+        }
+#pragma unroll
+        for (uint32_t ax = 0; ax < 3; ++ax) {
+            const uint32_t l = ax == 0 ? lx : (ax == 1 ? ly : lz);
+            const uint32_t gcoord = ax == 0 ? x : (ax == 1 ? y : z);
+            if (gcoord == 0) continue;  // no voxel at -1
+            int32_t n;
+            uint32_t oc;
+            if (l == 0) {
+                n = ax == 0 ? nxm : (ax == 1 ? nym : nzm);
+                oc = ax == 0 ? (B - 1) + B * (ly + B * lz)
+                             : (ax == 1 ? lx + B * ((B - 1) + B * lz) : lx + B * (ly + B * (B - 1)));
+            } else {
+                n = s;
+                oc = c - (ax == 0 ? 1u : (ax == 1 ? (uint32_t)B : (uint32_t)(B * B)));
+            }
+            if (n < 0) continue;
+            const uint32_t e = table[(uint64_t)n * NC + oc];
+            if (!((e >> ax) & 1u)) continue;
+            const uint64_t vi = f.blk_vbase[n] + (e >> 3) + __popc(e & 7u & ((1u << ax) - 1u));
+            const float G = a.dl_dpos[3 * vi + ax];
+            const uint32_t st = ax == 0 ? 1u : (ax == 1 ? (uint32_t)E : (uint32_t)(E * E));
+            const float v0 = s_val[t0 - st];
+            const float d = vg - v0;
+            if (!(fabsf(d) > 1e-5f)) continue;
+            const float t = (iso - v0) / d;
+            g += -G * t / d;
+        }
+        a.dl_dvol[x + (uint64_t)X * (y + (uint64_t)Y * z)] = g;
+    }
+    wave_lds_sync();
 }
 
-__global__ __launch_bounds__(64) void k_adjoint_iso(const double* __restrict__ partials, uint32_t n,
-                                                    float* __restrict__ out)
+template <int B, typename T>
+__global__ __launch_bounds__(256) void k_adjoint(AdjointArgs a)
 {
-    // lane-strided sums, then a fixed shuffle tree: same order on every run
-    double s = 0.0;
-    for (uint32_t i = threadIdx.x; i < n; i += 64u) s += partials[i];
+    constexpr int E = B + 2;
+    __shared__ float s_all[4][E * E * E];
+    __shared__ double s_part[4];
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    float* s_val = s_all[w];
+    const ExtractArgs& f = a.fwd;
+    const uint32_t gx = f.grid.gx, gy = f.grid.gy, gz = f.grid.gz;
+    const uint32_t* __restrict__ map = f.blk_map;
+    const uint32_t na = active_items(f);
+    // one item per (active block, home block): its own, +x, +y, +z; four
+    // blocks in a row when 4 * na would not fit 32 bits
+    const bool split = na <= 0x3FFFFFFFu;
+    const WaveItems it = wave_items(split ? 4u * na : na, 4);
+    const uint32_t pz = gx * gy;
+    double diso = 0.0;
+    for (uint32_t q = it.first; q < it.end; q += it.stride) {
+        const uint32_t A = f.ids[split ? q >> 2 : q];
+        const uint32_t bx = A % gx, by = (A / gx) % gy, bz = A / (gx * gy);
+        auto off = [&](uint32_t b) { return (int32_t)map[b] < 0; };
+        const uint32_t k0 = split ? (q & 3u) : 0u, k1 = split ? k0 + 1u : 4u;
+        for (uint32_t k = k0; k < k1; ++k) {
+            // an inactive neighbour's home block goes to the first active
+            // block of its -x, -y, -z neighbours, in that order
+            uint32_t b = A;
+            bool mine = true;
+            if (k == 1) {
+                b = A + 1;
+                mine = bx + 1 < gx && off(b);
+            } else if (k == 2) {
+                b = A + gx;
+                mine = by + 1 < gy && off(b) && (bx == 0 || off(b - 1));
+            } else if (k == 3) {
+                b = A + pz;
+                mine = bz + 1 < gz && off(b) && (bx == 0 || off(b - 1)) && (by == 0 || off(b - gx));
+            }
+            if (mine) adjoint_home<B, T>(a, s_val, b, diso);
+        }
+    }
+    // fixed order: the wave's shuffle tree, then the workgroup's waves 0..3
 #pragma unroll
-    for (int d = 32; d > 0; d >>= 1) s += __shfl_xor(s, d, 64);
-    if (threadIdx.x == 0) *out = (float)s;
+    for (int d = 32; d > 0; d >>= 1) diso += __shfl_xor(diso, d, 64);
+    if (lane == 0) s_part[w] = diso;
+    __syncthreads();
+    if (threadIdx.x == 0) a.iso_partials[blockIdx.x] = ((s_part[0] + s_part[1]) + s_part[2]) + s_part[3];
+}
+
+__global__ __launch_bounds__(256) void k_adjoint_iso(const double* __restrict__ partials, uint32_t n,
+                                                     float* __restrict__ out)
+{
+    // thread-strided sums, then a fixed tree: the same order on every run
+    __shared__ double s[256];
+    double v = 0.0;
+    for (uint32_t i = threadIdx.x; i < n; i += 256u) v += partials[i];
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (uint32_t h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) s[threadIdx.x] += s[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = (float)s[0];
 }
 
 template <int B>
 static hipError_t launch_adjoint_b(const AdjointArgs& a, hipStream_t s)
 {
+    const uint64_t n = (uint64_t)a.fwd.vol.X * a.fwd.vol.Y * a.fwd.vol.Z;
+    uint64_t zg = (n / 4 + 255) / 256;
+    zg = zg < 1 ? 1 : (zg > 8192 ? 8192 : zg);
+    hipLaunchKernelGGL(k_adjoint_zero, dim3((uint32_t)zg), dim3(256), 0, s, a.dl_dvol, n);
     if (a.fwd.vol.dtype == kF32)
         hipLaunchKernelGGL((k_adjoint<B, float>), dim3(a.n_wg), dim3(256), 0, s, a);
     else
         hipLaunchKernelGGL((k_adjoint<B, uint8_t>), dim3(a.n_wg), dim3(256), 0, s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_adjoint_iso, dim3(1), dim3(64), 0, s, a.iso_partials, a.n_wg * 4, a.dl_diso);
+    hipLaunchKernelGGL(k_adjoint_iso, dim3(1), dim3(256), 0, s, a.iso_partials, a.n_wg, a.dl_diso);
     return hipGetLastError();
 }
 

@@ -32,9 +32,13 @@ FAST_DIMS = [(64, 48, 40), (61, 33, 17), (300, 20, 26), (256, 9, 8), (8, 8, 8), 
 @pytest.mark.parametrize("dims", FAST_DIMS)
 @pytest.mark.parametrize("dtype", [np.uint8, np.float32])
 def test_minmax_b8_vs_oracle(ctx, dims, dtype):
+    # rows 0: the planner's choice (small volumes: the plane-split kernel);
+    # 1, 2: the block-column kernel with that many y-blocks per wave
     data = rand_volume(dims, dtype, 1)
-    mm, _ = run_minmax(ctx, data, dims, 8)
-    np.testing.assert_array_equal(mm.read_pairs(0), oracle.minmax(data, dims, 8, overlap=1))
+    want = oracle.minmax(data, dims, 8, overlap=1)
+    for rows in (0, 1, 2):
+        mm, _ = run_minmax(ctx, data, dims, 8, minmax_rows=rows)
+        np.testing.assert_array_equal(mm.read_pairs(0), want, err_msg="rows %d" % rows)
 
 
 @pytest.mark.parametrize("B", [1, 2, 3, 4, 5, 7, 16])
@@ -98,12 +102,13 @@ def test_filter_f32_vs_oracle(ctx, rule, fuse):
     dims = (130, 41, 35)
     data = oracle.gen_sdf_f32(dims, 21, 6, 3, 0.2)
     for iso in ISOS_F32:
-        mm, pc = run_minmax(ctx, data, dims, 8, iso=iso, rule=rule, fuse_filter=fuse)
-        f = filterActiveBlocks(ctx, mm, pc, options=makeOptions(rule=rule))
-        readActiveBlockCount(ctx, f)
         want = oracle.filter_ids(oracle.minmax(data, dims, 8), oracle.F32, iso, rule)
-        assert f.activeBlockCount == want.size, iso
-        np.testing.assert_array_equal(f.read_ids(), want)
+        for rows in (0, 1):   # the fused mask from both min/max kernels
+            mm, pc = run_minmax(ctx, data, dims, 8, iso=iso, rule=rule, fuse_filter=fuse, minmax_rows=rows)
+            f = filterActiveBlocks(ctx, mm, pc, options=makeOptions(rule=rule))
+            readActiveBlockCount(ctx, f)
+            assert f.activeBlockCount == want.size, (iso, rows)
+            np.testing.assert_array_equal(f.read_ids(), want)
 
 
 def test_filter_generic_block_and_large_grid(ctx):
