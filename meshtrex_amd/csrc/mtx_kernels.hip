@@ -2954,14 +2954,19 @@ __global__ __launch_bounds__(256) void k_adjoint_zero(float* __restrict__ out, u
 }
 
 // The gradient of every voxel of home block b (its corner tables and those of
-// its -x, -y, -z neighbours give the vertices; the volume gives t).
+// its -x, -y, -z neighbours give the vertices; the volume gives t).  Loads are
+// issued in three rounds so that each round's latency is paid once: the tile
+// (direct to LDS for f32), the corner-table entries and the four vertex bases;
+// then every dL/dp the chunk's cells need; then the arithmetic.
 template <int B, typename T>
 __device__ __forceinline__ void adjoint_home(const AdjointArgs& a, float* s_val, uint32_t b, double& diso)
 {
     constexpr int E = B + 2;  // voxels [-1, B] per axis
     constexpr int NTILE = E * E * E;
+    constexpr int KT = (NTILE + 63) / 64;
     constexpr int NC = EB<B>::NC;
     constexpr int CPL = EB<B>::CPL;
+    constexpr int KC = CPL < 4 ? CPL : 4;  // cells per lane in one round of loads
     using Entry = typename EB<B>::Entry;
     const uint32_t lane = lane_id();
     const ExtractArgs& f = a.fwd;
@@ -2972,71 +2977,127 @@ __device__ __forceinline__ void adjoint_home(const AdjointArgs& a, float* s_val,
     const uint32_t gx = f.grid.gx, gy = f.grid.gy;
     const uint32_t bx = b % gx, by = (b / gx) % gy, bz = b / (gx * gy);
     const uint32_t ox = bx * B, oy = by * B, oz = bz * B;
-    const int32_t s = (int32_t)f.blk_map[b];
-    const int32_t nxm = bx > 0 ? (int32_t)f.blk_map[b - 1] : -1;
-    const int32_t nym = by > 0 ? (int32_t)f.blk_map[b - gx] : -1;
-    const int32_t nzm = bz > 0 ? (int32_t)f.blk_map[b - gx * gy] : -1;
-    for (uint32_t j = lane; j < (uint32_t)NTILE; j += 64u) {
+    // this block and its -x, -y, -z neighbours: active index (-1: inactive), vertex base
+    int32_t nb[4];
+    nb[0] = (int32_t)f.blk_map[b];
+    nb[1] = bx > 0 ? (int32_t)f.blk_map[b - 1] : -1;
+    nb[2] = by > 0 ? (int32_t)f.blk_map[b - gx] : -1;
+    nb[3] = bz > 0 ? (int32_t)f.blk_map[b - gx * gy] : -1;
+    uint64_t vb[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) vb[q] = nb[q] >= 0 ? f.blk_vbase[nb[q]] : 0;
+    // the tile [-1, B]^3, clamped to the volume: f32 straight into LDS, u8
+    // through registers 8 loads at a time
+    auto tile_src = [&](uint32_t j) {
         const int32_t x = min(max((int32_t)(ox + j % E) - 1, 0), (int32_t)X - 1);
         const int32_t y = min(max((int32_t)(oy + (j / E) % E) - 1, 0), (int32_t)Y - 1);
         const int32_t z = min(max((int32_t)(oz + j / (E * E)) - 1, 0), (int32_t)Z - 1);
-        s_val[j] = vfloat(vol[(uint32_t)x + (uint64_t)X * ((uint32_t)y + (uint64_t)Y * (uint32_t)z)]);
-    }
-    wave_lds_sync();
-    const uint64_t vbs = s >= 0 ? f.blk_vbase[s] : 0;
+        return vol + ((uint32_t)x + (uint64_t)X * ((uint32_t)y + (uint64_t)Y * (uint32_t)z));
+    };
+    if constexpr (sizeof(T) == 4) {
 #pragma unroll
-    for (int k = 0; k < CPL; ++k) {
-        const uint32_t c = lane + 64u * (uint32_t)k;   // consecutive lanes: consecutive x
-        if (c >= (uint32_t)NC) break;
-        const uint32_t lx = c % B, ly = (c / B) % B, lz = c / (B * B);
-        const uint32_t x = ox + lx, y = oy + ly, z = oz + lz;
-        if (!(x < X && y < Y && z < Z)) continue;
-        const uint32_t t0 = (lx + 1) + E * ((ly + 1) + E * (lz + 1));
-        const float vg = s_val[t0];
-        float g = 0.0f;
-        if (s >= 0) {
-            const uint32_t e = table[(uint64_t)s * NC + c];
+        for (int k = 0; k < KT; ++k) {
+            const uint32_t j = 64u * (uint32_t)k + lane;
+            if (j < (uint32_t)NTILE) lds_direct_load4(tile_src(j), s_val + 64 * k);
+        }
+    } else {
+#pragma unroll 1
+        for (int k0 = 0; k0 < KT; k0 += 8) {
+            float tv[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const uint32_t j = 64u * (uint32_t)(k0 + k) + lane;
+                tv[k] = j < (uint32_t)NTILE ? vfloat(*tile_src(j)) : 0.0f;
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const uint32_t j = 64u * (uint32_t)(k0 + k) + lane;
+                if (j < (uint32_t)NTILE) s_val[j] = tv[k];
+            }
+        }
+    }
+    bool tile_waited = false;
+#pragma unroll 1
+    for (int k0 = 0; k0 < CPL; k0 += KC) {
+        // round 2: corner-table entries -- the cell's own (it is v0 of its
+        // edges) and those of its -x, -y, -z neighbour voxels' owners (it is v1)
+        uint32_t eo[KC], en[KC][3];
+#pragma unroll
+        for (int k = 0; k < KC; ++k) {
+            const uint32_t c = lane + 64u * (uint32_t)(k0 + k);
+            const uint32_t lx = c % B, ly = (c / B) % B, lz = c / (B * B);
+            const bool ok = c < (uint32_t)NC && ox + lx < X && oy + ly < Y && oz + lz < Z;
+            eo[k] = ok && nb[0] >= 0 ? (uint32_t)table[(uint64_t)nb[0] * NC + c] : 0u;
 #pragma unroll
             for (uint32_t ax = 0; ax < 3; ++ax) {
-                if (!((e >> ax) & 1u)) continue;
-                const uint64_t vi = vbs + (e >> 3) + __popc(e & 7u & ((1u << ax) - 1u));
-                const float G = a.dl_dpos[3 * vi + ax];
-                const uint32_t st = ax == 0 ? 1u : (ax == 1 ? (uint32_t)E : (uint32_t)(E * E));
-                const float d = s_val[t0 + st] - vg;
-                if (!(fabsf(d) > 1e-5f)) continue;  // t = 0.5: no dependence on the values
-                const float t = (iso - vg) / d;
-                g += G * (t - 1.0f) / d;
-                diso += (double)(G / d);
+                const uint32_t l = ax == 0 ? lx : (ax == 1 ? ly : lz);
+                const uint32_t gcoord = ax == 0 ? ox + lx : (ax == 1 ? oy + ly : oz + lz);
+                const int32_t n = l == 0 ? nb[1 + ax] : nb[0];
+                const uint32_t oc = l == 0 ? (ax == 0 ? (B - 1) + B * (ly + B * lz)
+                                                      : (ax == 1 ? lx + B * ((B - 1) + B * lz)
+                                                                 : lx + B * (ly + B * (B - 1))))
+                                           : c - (ax == 0 ? 1u : (ax == 1 ? (uint32_t)B : (uint32_t)(B * B)));
+                en[k][ax] = ok && gcoord > 0 && n >= 0 ? (uint32_t)table[(uint64_t)n * NC + oc] : 0u;
             }
+        }
+        // round 3: the dL/dp of every edge found
+        float go[KC][3], gn[KC][3];
+#pragma unroll
+        for (int k = 0; k < KC; ++k) {
+            const uint32_t c = lane + 64u * (uint32_t)(k0 + k);
+            const uint32_t lx = c % B, ly = (c / B) % B, lz = c / (B * B);
+#pragma unroll
+            for (uint32_t ax = 0; ax < 3; ++ax) {
+                const uint32_t e = eo[k];
+                go[k][ax] = (e >> ax) & 1u
+                                ? a.dl_dpos[3 * (vb[0] + (e >> 3) + __popc(e & 7u & ((1u << ax) - 1u))) + ax]
+                                : 0.0f;
+                const uint32_t l = ax == 0 ? lx : (ax == 1 ? ly : lz);
+                const uint32_t en_ = en[k][ax];
+                const uint64_t vbn = l == 0 ? vb[1 + ax] : vb[0];
+                gn[k][ax] = (en_ >> ax) & 1u
+                                ? a.dl_dpos[3 * (vbn + (en_ >> 3) + __popc(en_ & 7u & ((1u << ax) - 1u))) + ax]
+                                : 0.0f;
+            }
+        }
+        if (!tile_waited) {
+            if constexpr (sizeof(T) == 4) __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the tile is in LDS
+            wave_lds_sync();
+            tile_waited = true;
         }
 #pragma unroll
-        for (uint32_t ax = 0; ax < 3; ++ax) {
-            const uint32_t l = ax == 0 ? lx : (ax == 1 ? ly : lz);
-            const uint32_t gcoord = ax == 0 ? x : (ax == 1 ? y : z);
-            if (gcoord == 0) continue;  // no voxel at -1
-            int32_t n;
-            uint32_t oc;
-            if (l == 0) {
-                n = ax == 0 ? nxm : (ax == 1 ? nym : nzm);
-                oc = ax == 0 ? (B - 1) + B * (ly + B * lz)
-                             : (ax == 1 ? lx + B * ((B - 1) + B * lz) : lx + B * (ly + B * (B - 1)));
-            } else {
-                n = s;
-                oc = c - (ax == 0 ? 1u : (ax == 1 ? (uint32_t)B : (uint32_t)(B * B)));
+        for (int k = 0; k < KC; ++k) {
+            const uint32_t c = lane + 64u * (uint32_t)(k0 + k);   // consecutive lanes: consecutive x
+            if (c >= (uint32_t)NC) break;
+            const uint32_t lx = c % B, ly = (c / B) % B, lz = c / (B * B);
+            const uint32_t x = ox + lx, y = oy + ly, z = oz + lz;
+            if (!(x < X && y < Y && z < Z)) continue;
+            const uint32_t t0 = (lx + 1) + E * ((ly + 1) + E * (lz + 1));
+            const float vg = s_val[t0];
+            float g = 0.0f;
+#pragma unroll
+            for (uint32_t ax = 0; ax < 3; ++ax) {
+                const uint32_t st = ax == 0 ? 1u : (ax == 1 ? (uint32_t)E : (uint32_t)(E * E));
+                if ((eo[k] >> ax) & 1u) {
+                    const float d = s_val[t0 + st] - vg;
+                    if (fabsf(d) > 1e-5f) {  // else t = 0.5: no dependence on the values
+                        const float G = go[k][ax];
+                        const float t = (iso - vg) / d;
+                        g += G * (t - 1.0f) / d;
+                        diso += (double)(G / d);
+                    }
+                }
+                if ((en[k][ax] >> ax) & 1u) {
+                    const float v0 = s_val[t0 - st];
+                    const float d = vg - v0;
+                    if (fabsf(d) > 1e-5f) {
+                        const float t = (iso - v0) / d;
+                        g += -gn[k][ax] * t / d;
+                    }
+                }
             }
-            if (n < 0) continue;
-            const uint32_t e = table[(uint64_t)n * NC + oc];
-            if (!((e >> ax) & 1u)) continue;
-            const uint64_t vi = f.blk_vbase[n] + (e >> 3) + __popc(e & 7u & ((1u << ax) - 1u));
-            const float G = a.dl_dpos[3 * vi + ax];
-            const uint32_t st = ax == 0 ? 1u : (ax == 1 ? (uint32_t)E : (uint32_t)(E * E));
-            const float v0 = s_val[t0 - st];
-            const float d = vg - v0;
-            if (!(fabsf(d) > 1e-5f)) continue;
-            const float t = (iso - v0) / d;
-            g += -G * t / d;
+            a.dl_dvol[x + (uint64_t)X * (y + (uint64_t)Y * z)] = g;
         }
-        a.dl_dvol[x + (uint64_t)X * (y + (uint64_t)Y * z)] = g;
     }
     wave_lds_sync();
 }
