@@ -8,6 +8,7 @@
 //   managers_demo <name_XxYxZ_type.raw> <isovalue> <block edge> <out.obj>
 #include <cstdio>
 #include <cstdlib>
+#include <vector>
 
 #include "meshtrex_amd/managers.hpp"
 
@@ -51,11 +52,35 @@ int main(int argc, char** argv)
         const uint32_t countViaBuffer = read_u32(filteringResult.activeBlockCountBuffer.buffer);
         const uint32_t firstId = filteringResult.activeBlockCount ? read_u32(filteringResult.compactedBlockIdBuffer.buffer)
                                                                   : 0xFFFFFFFFu;
+        // the renderer layout, under the renderer's rule (src/shaders/render.task.glsl:61-63,125-128):
+        // triangles with an index outside their meshlet's vertex range would be dropped
+        std::vector<mtx_meshlet> desc(extraction.meshletCount);
+        std::vector<uint32_t> mind(extraction.indexCount);
+        if (!desc.empty() && mtx_memcpy_d2h(desc.data(), extraction.meshletDescriptorBuffer.buffer,
+                                            desc.size() * sizeof(mtx_meshlet)) != MTX_OK)
+            throw std::runtime_error("readback failed");
+        if (!mind.empty() && mtx_memcpy_d2h(mind.data(), extraction.meshletIndexBuffer.buffer, 4 * mind.size()) != MTX_OK)
+            throw std::runtime_error("readback failed");
+        uint64_t dropped = 0, covered = 0, overfull = 0;
+        for (const mtx_meshlet& m : desc) {
+            if (m.vertexCount > 64 || m.primitiveCount > 126) ++overfull;
+            for (uint32_t t = 0; t < m.primitiveCount; ++t) {
+                bool ok = true;
+                for (uint32_t k = 0; k < 3; ++k) {
+                    const uint32_t i = mind[m.indexOffset + 3 * t + k];
+                    ok = ok && i >= m.vertexOffset && i < m.vertexOffset + m.vertexCount;
+                }
+                dropped += ok ? 0 : 1;
+                ++covered;
+            }
+        }
         std::printf("{\"activeBlockCount\": %u, \"vertexCount\": %u, \"indexCount\": %u, \"meshletCount\": %u, "
                     "\"levels\": %zu, \"activeBlockCountBuffer\": %u, \"firstActiveBlockId\": %u, "
                     "\"compactedBlockIdBuffer_size\": %zu, \"volumeImage_size\": %zu, \"minMaxImage_size\": %zu, "
                     "\"vertexCountBuffer\": %u, \"indexCountBuffer\": %u, \"meshletDescriptorCountBuffer\": %u, "
                     "\"vertexBuffer_size\": %zu, \"indexBuffer_size\": %zu, \"meshletDescriptorBuffer_size\": %zu, "
+                    "\"meshletVertexBuffer_size\": %zu, \"meshletIndexBuffer_size\": %zu, "
+                    "\"rendererTriangles\": %llu, \"rendererDropped\": %llu, \"overfullMeshlets\": %llu, "
                     "\"regions\": {",
                     filteringResult.activeBlockCount, extraction.vertexCount, extraction.indexCount,
                     extraction.meshletCount, minMaxOutput.minMaxMipViews.size(), countViaBuffer, firstId,
@@ -63,7 +88,9 @@ int main(int argc, char** argv)
                     minMaxOutput.minMaxImage.size, read_u32(extraction.vertexCountBuffer.buffer),
                     read_u32(extraction.indexCountBuffer.buffer),
                     read_u32(extraction.meshletDescriptorCountBuffer.buffer), extraction.vertexBuffer.size,
-                    extraction.indexBuffer.size, extraction.meshletDescriptorBuffer.size);
+                    extraction.indexBuffer.size, extraction.meshletDescriptorBuffer.size,
+                    extraction.meshletVertexBuffer.size, extraction.meshletIndexBuffer.size,
+                    (unsigned long long)covered, (unsigned long long)dropped, (unsigned long long)overfull);
         bool first = true;
         for (const auto& r : profiler.results()) {
             std::printf("%s\"%s\": %.4f", first ? "" : ", ", r.first.c_str(), r.second);

@@ -55,6 +55,11 @@ def test_cpp_managers_demo(tmp_path):
     assert res["vertexBuffer_size"] == 32 * res["vertexCount"]
     assert res["indexBuffer_size"] == 4 * res["indexCount"]
     assert res["meshletDescriptorBuffer_size"] == 16 * res["meshletCount"]
+    # the renderer layout through its members: every triangle covered once, none dropped
+    assert res["meshletIndexBuffer_size"] == 4 * res["indexCount"]
+    assert res["meshletVertexBuffer_size"] >= res["vertexBuffer_size"]
+    assert res["rendererTriangles"] == res["indexCount"] // 3
+    assert res["rendererDropped"] == 0 and res["overfullMeshlets"] == 0
     lines = obj.read_text().splitlines()
     vs = [l for l in lines if l.startswith("v ")]
     vns = [l for l in lines if l.startswith("vn ")]
