@@ -471,13 +471,45 @@ def bench_ingest(args, ctx, vol_ptr, dims, minmax_ms):
         times.append(time.perf_counter() - t0)
     d.free()
     copy_s = min(times)
-    res["pinned_copy_only"] = {"ms": round(copy_s * 1e3, 2), "GBps": round(nbytes / copy_s / 1e9, 2)}
+    res["pinned_copy_only"] = {"ms": round(copy_s * 1e3, 2), "GBps": round(nbytes / copy_s / 1e9, 2),
+                               "what": "one synchronous copy of the whole volume"}
+    # the same copies the slab pipeline makes (256 MB of whole 8-plane block
+    # rows each, plus the plane shared with the next slab), back to back on one
+    # stream with nothing else: what the pipeline's min/max has to hide behind
+    from meshtrex_amd import hiprt
+    X, Y, Z = dims
+    plane = X * Y * 4
+    rows = max(1, (256 << 20) // plane // 8)
+    cuts, copied, k = [], 0, 0
+    while k * rows < (Z + 7) // 8:
+        need = min(Z, (min((k + 1) * rows, (Z + 7) // 8)) * 8 + 1)
+        if need > copied:
+            cuts.append((copied, need))
+            copied = need
+        k += 1
+    d = DeviceBuffer(nbytes)
+    hip = hiprt.hip()
+    hip.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p]
+    s = hiprt.stream_create()
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        for a, b in cuts:
+            assert hip.hipMemcpyAsync(d.ptr + a * plane, pin.ctypes.data + a * plane, (b - a) * plane, 1, s) == 0
+        hiprt.stream_sync(s)
+        times.append(time.perf_counter() - t0)
+    hiprt.stream_destroy(s)
+    d.free()
+    slab_s = min(times)
+    res["pinned_slab_copies_only"] = {"ms": round(slab_s * 1e3, 2), "GBps": round(nbytes / slab_s / 1e9, 2),
+                                      "copies": len(cuts)}
     res["device_resident_minmax_ms"] = round(minmax_ms, 3)
-    serial = copy_s * 1e3 + minmax_ms
-    res["pinned_overlap"] = {"serial_copy_then_minmax_ms": round(serial, 2),
+    serial = slab_s * 1e3 + minmax_ms
+    res["pinned_overlap"] = {"serial_slab_copies_then_minmax_ms": round(serial, 2),
                              "slabbed_ms": res["pinned"]["ms"],
                              "hidden_fraction_of_minmax": round(max(0.0, min(1.0, (serial - res["pinned"]["ms"])
-                                                                             / minmax_ms)), 3)}
+                                                                             / minmax_ms)), 3),
+                             "note": "the min/max is under 1 % of the PCIe time: the pipeline can hide at most that"}
     return res
 
 

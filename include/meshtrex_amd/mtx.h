@@ -325,7 +325,12 @@ typedef struct mtx_batch_options {
     uint32_t graph;              /* 1: replay one captured hipGraph (default); 0: launch directly */
     uint32_t tune_iters;         /* timed runs per autotuner candidate (default 3) */
     uint32_t headroom_pct;       /* output capacity above the planning run's counts (default 0) */
-    uint32_t _reserved[8];
+    uint32_t schedule;           /* work assignment of the vertex and triangle passes: 0 = autotuned
+                                    (default), 1 = strided over the active list, 2 = contiguous
+                                    ranges of equal cost (vertices / triangles per block) */
+    uint32_t wg_per_cu;          /* persistent workgroups per CU of the extraction passes; 0 =
+                                    autotuned with the schedule (default) */
+    uint32_t _reserved[6];
 } mtx_batch_options;
 
 /* One unit's slice of the batch outputs, written on the device by every run. */
@@ -370,6 +375,11 @@ typedef struct mtx_batch_class_info {
     uint64_t triangles;          /* planning run */
     float tuned_ms[3];           /* autotuner: class pipeline at B = 4, 8, 16 (0 = not timed) */
     float tuned_rows_ms[3];      /* autotuner: min/max at 1, 2, 4 y-blocks per wave (0 = not timed) */
+    uint32_t schedule;           /* chosen work assignment (1 strided, 2 cost-balanced) */
+    uint32_t wg_per_cu;          /* chosen persistent workgroups per CU */
+    float tuned_sched_ms[6];     /* autotuner: class pipeline with (schedule, workgroups per CU) =
+                                    (1, 32), (1, 64), (1, 128), (2, 8), (2, 16), (2, 32)
+                                    (0 = not timed) */
 } mtx_batch_class_info;
 
 void mtx_batch_options_default(mtx_batch_options* o);

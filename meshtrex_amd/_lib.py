@@ -75,7 +75,7 @@ class CBatchVolume(ctypes.Structure):
 class BatchOptions(ctypes.Structure):
     _fields_ = [("block_dim", u32), ("minmax_rows", u32), ("rule", u32), ("normals", u32),
                 ("edge_ids", u32), ("graph", u32), ("tune_iters", u32), ("headroom_pct", u32),
-                ("_reserved", u32 * 8)]
+                ("schedule", u32), ("wg_per_cu", u32), ("_reserved", u32 * 6)]
 
 
 class BatchRange(ctypes.Structure):
@@ -94,7 +94,8 @@ class CBatchOutput(ctypes.Structure):
 class BatchClassInfo(ctypes.Structure):
     _fields_ = [("dtype", u32), ("block_dim", u32), ("minmax_rows", u32), ("num_volumes", u32),
                 ("num_units", u32), ("active_blocks", u32), ("blocks", u64), ("voxels", u64),
-                ("vertices", u64), ("triangles", u64), ("tuned_ms", f32 * 3), ("tuned_rows_ms", f32 * 3)]
+                ("vertices", u64), ("triangles", u64), ("tuned_ms", f32 * 3), ("tuned_rows_ms", f32 * 3),
+                ("schedule", u32), ("wg_per_cu", u32), ("tuned_sched_ms", f32 * 6)]
 
 
 # name -> (restype, argtypes)

@@ -24,7 +24,7 @@ RANGE_FIELDS = [f for f, _ in L.BatchRange._fields_]
 
 class Batch:
     def __init__(self, context, volumes, block_dim=0, minmax_rows=0, rule=L.MTX_RULE_CPU, normals=True,
-                 edge_ids=False, graph=True, tune_iters=3, headroom_pct=0):
+                 edge_ids=False, graph=True, tune_iters=3, headroom_pct=0, schedule=0, wg_per_cu=0):
         self.ctx = context
         self.volumes = list(volumes)
         vols = (L.CBatchVolume * len(self.volumes))()
@@ -45,6 +45,7 @@ class Batch:
         o.block_dim, o.minmax_rows, o.rule = block_dim, minmax_rows, rule
         o.normals, o.edge_ids, o.graph = int(bool(normals)), int(bool(edge_ids)), int(bool(graph))
         o.tune_iters, o.headroom_pct = tune_iters, headroom_pct
+        o.schedule, o.wg_per_cu = schedule, wg_per_cu
         h = ctypes.c_void_p()
         _check(context, L.lib().mtx_batch_create(context.handle, vols, len(self.volumes), ctypes.byref(o),
                                                  ctypes.byref(h)))
@@ -72,6 +73,7 @@ class Batch:
             d = {f: getattr(ci, f) for f, _ in L.BatchClassInfo._fields_}
             d["tuned_ms"] = list(ci.tuned_ms)
             d["tuned_rows_ms"] = list(ci.tuned_rows_ms)
+            d["tuned_sched_ms"] = list(ci.tuned_sched_ms)
             d["dtype"] = "float32" if ci.dtype == L.MTX_F32 else "uint8"
             out.append(d)
         return out

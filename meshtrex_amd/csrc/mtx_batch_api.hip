@@ -58,6 +58,8 @@ struct ClassPlan {
     uint32_t active = 0;
     float tuned_ms[3] = { 0, 0, 0 };
     float tuned_rows_ms[3] = { 0, 0, 0 };
+    uint32_t schedule = 1, wg_per_cu = 0;   // 0: the launchers' default
+    float tuned_sched_ms[6] = { 0, 0, 0, 0, 0, 0 };
 
     void release()
     {
@@ -260,6 +262,8 @@ hipError_t class_enqueue(const ClassPlan& c, hipStream_t s)
     a.units = c.d_units;
     a.item_unit = c.d_item_unit;
     a.unit_first = c.d_unit_first;
+    a.balanced = c.schedule == 2 ? 1u : 0u;
+    a.wg_per_cu = c.wg_per_cu;
     e = launch_extract_count(a, s);
     if (e == hipSuccess) e = launch_extract_scan(a, TileSums{ reinterpret_cast<uint64_t*>(c.d_ws + c.w.scan) }, s);
     if (e == hipSuccess) e = launch_extract_emit(a, s);
@@ -348,8 +352,13 @@ mtx_status time_min(mtx_context* ctx, hipStream_t s, uint32_t iters, F&& body, f
     return MTX_OK;
 }
 
+// Candidate (schedule, workgroups per CU) pairs of the extraction passes.
+constexpr uint32_t kSched[6][2] = { { 1, 32 }, { 1, 64 }, { 1, 128 }, { 2, 8 }, { 2, 16 }, { 2, 32 } };
+
 // Autotuner: block edge by the class's whole pipeline time, then the min/max
-// wave shape by the min/max time, each the fastest of the timed candidates.
+// wave shape by the min/max time, then the extraction passes' work assignment
+// and grid by the whole pipeline again, each the fastest of the timed
+// candidates (a fixed option skips its step).
 mtx_status class_tune(mtx_context* ctx, const std::vector<VolumeIn>& vols, const mtx_batch_options& o,
                       ClassPlan& c, hipStream_t s)
 {
@@ -384,7 +393,31 @@ mtx_status class_tune(mtx_context* ctx, const std::vector<VolumeIn>& vols, const
         }
         if (mtx_status st = class_replan_rows(ctx, c, bestM)) return st;
     }
-    return class_size(ctx, c, s);
+    if (mtx_status st = class_size(ctx, c, s)) return st;
+    // work assignment and grid of the extraction passes, by the whole pipeline
+    if (!o.schedule || !o.wg_per_cu) {
+        OwnOutputs own;
+        if (mtx_status st = class_own_outputs(ctx, c, own, o.edge_ids != 0)) return st;
+        float best = 1e30f;
+        uint32_t bs = o.schedule ? o.schedule : 1, bw = o.wg_per_cu;
+        for (int k = 0; k < 6; ++k) {
+            if (o.schedule && o.schedule != kSched[k][0]) continue;
+            if (o.wg_per_cu && o.wg_per_cu != kSched[k][1]) continue;
+            c.schedule = kSched[k][0];
+            c.wg_per_cu = kSched[k][1];
+            float ms = 0;
+            if (mtx_status st = time_min(ctx, s, iters, [&] { return class_enqueue(c, s); }, ms)) return st;
+            c.tuned_sched_ms[k] = ms;
+            if (ms < best) { best = ms; bs = c.schedule; bw = c.wg_per_cu; }
+        }
+        c.schedule = bs;
+        c.wg_per_cu = bw;
+        c.vertices = nullptr; c.indices = nullptr; c.meshlets = nullptr; c.edge_ids = nullptr;
+    } else {
+        c.schedule = o.schedule;
+        c.wg_per_cu = o.wg_per_cu;
+    }
+    return MTX_OK;
 }
 
 void batch_free(mtx_batch* b)
@@ -476,6 +509,8 @@ mtx_status mtx_batch_create(mtx_context* ctx, const mtx_batch_volume* volumes, u
     if (o.minmax_rows != 0 && o.minmax_rows != 1 && o.minmax_rows != 2 && o.minmax_rows != 4)
         return fail(ctx, MTX_EINVAL, "minmax_rows must be 0, 1, 2 or 4");
     if (o.rule > MTX_RULE_GPU) return fail(ctx, MTX_EINVAL, "unknown rule %u", o.rule);
+    if (o.schedule > 2) return fail(ctx, MTX_EINVAL, "schedule must be 0, 1 or 2");
+    if (o.wg_per_cu > 1024) return fail(ctx, MTX_EINVAL, "wg_per_cu must be at most 1024");
 
     mtx_batch* b = new (std::nothrow) mtx_batch;
     if (!b) return fail(ctx, MTX_ENOMEM, "batch");
@@ -643,6 +678,9 @@ mtx_status mtx_batch_class_info_get(const mtx_batch* b, uint32_t index, mtx_batc
         out->tuned_ms[k] = c.tuned_ms[k];
         out->tuned_rows_ms[k] = c.tuned_rows_ms[k];
     }
+    out->schedule = c.schedule;
+    out->wg_per_cu = c.wg_per_cu;
+    for (int k = 0; k < 6; ++k) out->tuned_sched_ms[k] = c.tuned_sched_ms[k];
     return MTX_OK;
 }
 

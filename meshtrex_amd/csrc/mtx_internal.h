@@ -172,6 +172,11 @@ struct ExtractArgs {
     const BatchUnit* units;
     const uint32_t* item_unit;
     const uint32_t* unit_first;
+    // work assignment of the vertex and triangle passes: 0 strided over the
+    // list, 1 contiguous ranges of equal cost (vertices / triangles + a fixed
+    // cost per block); persistent workgroups per CU (0: the measured default)
+    uint32_t balanced;
+    uint32_t wg_per_cu;
 };
 
 bool extract_supported_block(uint32_t B);

@@ -6,7 +6,10 @@
 #include "mc_cases.h"
 
 #include <cstdlib>
+#include <mutex>
 #include <type_traits>
+#include <utility>
+#include <vector>
 
 namespace mtx {
 
@@ -1176,12 +1179,58 @@ __device__ __forceinline__ WaveItems wave_items(uint32_t na, uint32_t wpg)
     return r;
 }
 
+// Cost-balanced assignment.  Item i of an emit pass costs kItemCost plus its
+// vertex (or triangle) count, so the cost of the items before i is
+// kItemCost * i + base[i], base being the scan's exclusive prefix.
+constexpr uint64_t kItemCost = 24;
+
+// The first i in [0, na] with kItemCost * i + base[i] >= target (na: none);
+// base[na] is never read.  Wave-uniform; each round probes 64 positions of the
+// remaining range, so a few rounds cover any list.
+__device__ __forceinline__ uint32_t cost_bound(const uint64_t* __restrict__ base, uint32_t na, uint64_t target)
+{
+    const uint32_t lane = lane_id();
+    uint32_t lo = 0, hi = na;
+    while (lo < hi) {
+        const uint32_t p = lo + (uint32_t)(((uint64_t)(hi - lo) * (lane + 1u)) / 65u);  // in [lo, hi)
+        const uint64_t bal = __ballot(kItemCost * p + base[p] >= target);
+        if (!bal) {
+            lo = __shfl(p, 63, 64) + 1u;
+        } else {
+            const int f = __builtin_ctzll(bal);
+            hi = __shfl(p, f, 64);
+            if (f > 0) lo = __shfl(p, f - 1, 64) + 1u;
+        }
+    }
+    return lo;
+}
+
 // Active blocks to process: the device count, clamped to the capacity of the
 // per-block arrays (a count beyond it means the list does not belong to this
 // extraction; the scan flags it).
 __device__ __forceinline__ uint32_t active_items(const ExtractArgs& a)
 {
     return min(*a.count, a.max_blocks);
+}
+
+// This wave's items of an emit pass: strided (wave_items), or with
+// a.balanced the contiguous items whose cost prefix lies in [C g / W,
+// C (g + 1) / W) for wave g of W (C the pass's whole cost, `total` its count
+// total), g numbered XCD by XCD so that neighbouring ranges share an L2.
+__device__ __forceinline__ WaveItems cost_items(const ExtractArgs& a, uint32_t na, uint32_t wpg,
+                                                const uint64_t* __restrict__ base, uint64_t total)
+{
+    if (!a.balanced) return wave_items(na, wpg);
+    const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint64_t W = (uint64_t)gridDim.x * wpg;
+    const uint64_t g = gridDim.x % 8u ? (uint64_t)blockIdx.x * wpg + w
+                                      : ((uint64_t)(blockIdx.x % 8u) * (gridDim.x / 8u) + blockIdx.x / 8u) * wpg + w;
+    const uint64_t C = kItemCost * (uint64_t)na + total;
+    WaveItems r;
+    r.first = cost_bound(base, na, C * g / W);
+    r.end = cost_bound(base, na, C * (g + 1) / W);
+    r.stride = 1;
+    return r;
 }
 
 template <typename V>
@@ -1949,7 +1998,7 @@ __global__ __launch_bounds__((64 * VertCfg<B, NORMALS, T>::WPG)) void k_extract_
     const uint32_t na = active_items(a);
     const Entry* __restrict__ table = static_cast<const Entry*>(a.corner_table);
     constexpr int LPL = (NTILE + 63) / 64;
-    const WaveItems it = wave_items(na, WPG);
+    const WaveItems it = cost_items(a, na, WPG, a.blk_vbase, a.totals[0]);
     for (uint32_t i = it.first; i < it.end; i += it.stride) {
         const uint32_t bnv = a.blk_counts[4 * (uint64_t)i];
         if (bnv == 0) continue;
@@ -2313,7 +2362,7 @@ __global__ __launch_bounds__((64 * TriCfg<B>::WPG)) void k_extract_triangles(Ext
         }
     };
 
-    const WaveItems it = wave_items(na, WPG);
+    const WaveItems it = cost_items(a, na, WPG, a.blk_tbase, a.totals[1]);
     TB cur, nxt;
     uint32_t i = it.first;
     if (i < it.end) {
@@ -2791,11 +2840,32 @@ constexpr uint32_t kWgPerCuSingleTriangles = 10;
 constexpr uint32_t kWgPerCuBatch = 64;
 constexpr uint32_t kWgPerCuMeshlet = 8;
 
+// Workgroups of `kernel` one CU holds, queried once per kernel (the first
+// launch of a batch is its planning run, outside graph capture).
+static int resident_wgs(const void* kernel, uint32_t threads)
+{
+    static std::mutex mu;
+    static std::vector<std::pair<const void*, int>> known;
+    std::lock_guard<std::mutex> lk(mu);
+    for (const auto& k : known)
+        if (k.first == kernel) return k.second;
+    int r = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&r, kernel, (int)threads, 0) != hipSuccess) r = 0;
+    known.emplace_back(kernel, r);
+    return r;
+}
+
 template <typename K>
 static hipError_t launch_persistent(K kernel, uint32_t threads, const ExtractArgs& a, hipStream_t s,
                                     uint32_t per_cu_single = kWgPerCuSingle)
 {
-    uint64_t n = (uint64_t)device_cus() * (a.units ? kWgPerCuBatch : per_cu_single);
+    uint64_t n = (uint64_t)device_cus() * (a.wg_per_cu ? a.wg_per_cu : (a.units ? kWgPerCuBatch : per_cu_single));
+    if (a.balanced) {
+        // equal-cost ranges assume every wave runs at once: exactly one round
+        // of the workgroups the CUs hold
+        const int r = resident_wgs(reinterpret_cast<const void*>(kernel), threads);
+        if (r > 0) n = (uint64_t)device_cus() * (uint64_t)r;
+    }
     const uint64_t need = ((uint64_t)a.max_blocks + threads / 64 - 1) / (threads / 64);
     if (need < n) n = need;
     n = n < 8 ? 8 : n / 8 * 8;

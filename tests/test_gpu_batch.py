@@ -49,9 +49,12 @@ def volumes():
     return out
 
 
-@pytest.mark.parametrize("B", [4, 8, 16])
-def test_batch_units_equal_single_volume_path(ctx, volumes, B):
-    with Batch(ctx, [(v[2], v[3]) for v in volumes], block_dim=B, minmax_rows=2, edge_ids=True) as b:
+@pytest.mark.parametrize("B,schedule,wg", [(4, 1, 64), (8, 1, 32), (16, 1, 64), (4, 2, 8), (8, 2, 16),
+                                            (16, 2, 8)])
+def test_batch_units_equal_single_volume_path(ctx, volumes, B, schedule, wg):
+    # strided and cost-balanced work assignment give the same outputs
+    with Batch(ctx, [(v[2], v[3]) for v in volumes], block_dim=B, minmax_rows=2, edge_ids=True,
+               schedule=schedule, wg_per_cu=wg) as b:
         b.run()
         ranges = check_units_against_plain(ctx, b, volumes)
         assert sum(r["vertex_count"] for r in ranges) > 0
@@ -83,6 +86,10 @@ def test_batch_autotuner_and_graph_replays(ctx, volumes):
             assert i["block_dim"] in (4, 8, 16) and i["minmax_rows"] in (1, 2, 4)
             assert all(t > 0 for t in i["tuned_ms"]) and all(t > 0 for t in i["tuned_rows_ms"])
             assert i["tuned_ms"][(4, 8, 16).index(i["block_dim"])] == min(i["tuned_ms"])
+            # the work assignment and grid: the fastest of the six timed candidates
+            cands = [(1, 32), (1, 64), (1, 128), (2, 8), (2, 16), (2, 32)]
+            assert all(t > 0 for t in i["tuned_sched_ms"])
+            assert i["tuned_sched_ms"][cands.index((i["schedule"], i["wg_per_cu"]))] == min(i["tuned_sched_ms"])
         b.run()
         first = [b.unit_mesh(u) for u in range(b.num_units)]
         for _ in range(3):
@@ -150,3 +157,5 @@ def test_batch_rejects_bad_input(ctx):
         Batch(ctx, [(dev, (1.0,))], block_dim=5)
     with pytest.raises(MtxError, match="MTX_EINVAL"):
         Batch(ctx, [(dev, ())])
+    with pytest.raises(MtxError, match="MTX_EINVAL"):
+        Batch(ctx, [(dev, (1.0,))], schedule=3)

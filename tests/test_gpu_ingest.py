@@ -1,10 +1,12 @@
 """Volume ingest: host volumes (pageable and pinned) upload in slabs of whole
 block planes, each slab's min/max running as soon as it is resident.  The
-result must equal the min/max of the same volume already on the device, bit
-for bit, for volumes of several slabs (256 MB each) with odd dimensions."""
+result must equal the min/max of the same volume already on the device, and
+the oracle's pairs and active list, bit for bit, for volumes of several slabs
+(256 MB each) with odd dimensions."""
 import numpy as np
 import pytest
 
+import oracle
 from meshtrex_amd import (DeviceBuffer, Volume, computeMinMaxMip, extractMeshletDescriptors,
                           filterActiveBlocks, loadVolume, makeOptions, makePushConstants, pinned_array,
                           read_device)
@@ -47,6 +49,11 @@ def test_slab_upload_equals_device_resident(ctx, dims, dtype, pinned):
     for a, b in zip(got, ref):
         np.testing.assert_array_equal(a, b)
     dev.free()
+    # and the oracle's pairs and active list of the same volume
+    pairs = oracle.minmax(src, dims, 8)
+    np.testing.assert_array_equal(got[0], pairs)
+    np.testing.assert_array_equal(got[2], oracle.filter_ids(pairs, oracle.U8 if dtype == np.uint8 else oracle.F32,
+                                                            iso))
 
 
 def test_pinned_load_volume_extracts_the_same(ctx, tmp_path):
