@@ -173,9 +173,25 @@ _lib = None
 _synth = None
 
 
+def _one_runtime():
+    """torch ships its own HIP and HSA runtimes, and its libraries ask for them
+    as libamdhip64.so / libhsa-runtime64.so; this library links the system
+    ones by their versioned names.  Two runtimes in one process cannot both
+    open the GPU, so the runtime is loaded under the names torch asks for
+    before this library: a later `import torch` then reuses it, and after an
+    earlier `import torch` these names resolve to torch's copies, which this
+    library then uses."""
+    for name in ("libhsa-runtime64.so", "libamdhip64.so"):
+        try:
+            ctypes.CDLL(name, mode=ctypes.RTLD_GLOBAL)
+        except OSError:
+            pass
+
+
 def _load(path, sigs):
     if not os.path.exists(path):
         raise ImportError("%s is not built; run `python -m meshtrex_amd.build`" % path)
+    _one_runtime()
     lib = ctypes.CDLL(path)
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)

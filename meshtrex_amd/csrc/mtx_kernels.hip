@@ -1385,11 +1385,13 @@ __device__ __forceinline__ void first_user(uint32_t u, uint32_t nb, uint32_t k, 
 }
 
 // Workgroup tables: active-edge masks of the 256 cases in owner-index bits,
-// and the masks of edges a cell adds (64 neighbour patterns x 10 classes).
+// the masks of edges a cell adds (64 neighbour patterns x 10 classes) and the
+// class of every k below B^2 + B.
 template <int B>
-__device__ __forceinline__ void meshlet_tables(uint16_t* s_emu, uint16_t* s_newtab)
+__device__ __forceinline__ void meshlet_tables(uint16_t* s_emu, uint16_t* s_newtab, uint8_t* s_cls)
 {
     for (uint32_t c = threadIdx.x; c < 256u; c += blockDim.x) s_emu[c] = (uint16_t)mtx_case_umask(c);
+    for (uint32_t k = threadIdx.x; k < (uint32_t)(B * B + B); k += blockDim.x) s_cls[k] = (uint8_t)back_class<B>(k);
     for (uint32_t nb = threadIdx.x; nb < 64u; nb += blockDim.x) {
         uint32_t back[12];
 #pragma unroll
@@ -1405,11 +1407,62 @@ __device__ __forceinline__ void meshlet_tables(uint16_t* s_emu, uint16_t* s_newt
     }
 }
 
+// First-user table: for each neighbour pattern nb and edge u of a cell, the
+// earlier cells of the block sharing the edge, as three 13-bit fields
+// (back distance, 9 bits | the edge's owner index there << 9), farthest
+// first; unused fields hold distance 0.  In a meshlet that started k cells
+// back the edge's first user is the first field with 0 < distance <= k (none:
+// the cell itself).
+template <int B>
+__device__ __forceinline__ void first_user_table(uint64_t* s_fu)
+{
+    for (uint32_t x = threadIdx.x; x < 64u * 12u; x += blockDim.x) {
+        const uint32_t nb = x / 12u, u = x % 12u;
+        uint32_t d[3] = { 0, 0, 0 }, uo[3] = { 0, 0, 0 };
+        uint32_t n = 0;
+        for_sharing_cells<B>(u, nb, [&](int delta, uint32_t o) {
+            if (delta < 0) {
+                d[n] = (uint32_t)(-delta);
+                uo[n] = o;
+                ++n;
+            }
+        });
+        // farthest first
+        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2 - i; ++j)
+                if (d[j] < d[j + 1]) {
+                    const uint32_t td = d[j], to = uo[j];
+                    d[j] = d[j + 1]; uo[j] = uo[j + 1];
+                    d[j + 1] = td; uo[j + 1] = to;
+                }
+        uint64_t e = 0;
+        for (int i = 0; i < 3; ++i) e |= (uint64_t)(d[i] | (uo[i] << 9)) << (13 * i);
+        s_fu[x] = e;
+    }
+}
+
+// The first user of edge u (back distance, owner index there) from its table entry.
+__device__ __forceinline__ void first_user_of(uint64_t e, uint32_t u, uint32_t k, uint32_t& back, uint32_t& u1)
+{
+    back = 0;
+    u1 = u;
+#pragma unroll
+    for (int f = 2; f >= 0; --f) {   // nearest first, so the farthest qualifying field is taken last
+        const uint32_t fld = (uint32_t)(e >> (13 * f)) & 0x1FFFu;
+        const uint32_t d = fld & 511u;
+        if (d != 0u && d <= k) {
+            back = d;
+            u1 = fld >> 9;
+        }
+    }
+}
+
 // Vertices a cell adds to a meshlet that started k cells before it.
 template <int B>
-__device__ __forceinline__ uint32_t added_edges(uint32_t emu, uint32_t nb, uint32_t k, const uint16_t* s_newtab)
+__device__ __forceinline__ uint32_t added_edges(uint32_t emu, uint32_t nb, uint32_t k, const uint16_t* s_newtab,
+                                                const uint8_t* s_cls)
 {
-    const uint32_t cls = k >= (uint32_t)(B * B + B) ? 9u : back_class<B>(k);
+    const uint32_t cls = k >= (uint32_t)(B * B + B) ? 9u : (uint32_t)s_cls[k];
     return emu & s_newtab[nb * kBackClasses + cls];
 }
 
@@ -1426,7 +1479,7 @@ __device__ __forceinline__ uint32_t added_edges(uint32_t emu, uint32_t nb, uint3
 // edges no earlier cell of the block uses << 21.
 template <int B, int CPL, typename Cell>
 __device__ __forceinline__ void meshlet_cut(Cell&& cell, const uint8_t* s_ntri, const uint16_t* s_emu,
-                                            const uint16_t* s_newtab, uint32_t& nm, uint32_t& ns, uint64_t& sbits)
+                                            const uint16_t* s_newtab, const uint8_t* s_cls, uint32_t& nm, uint32_t& ns, uint64_t& sbits)
 {
     constexpr uint32_t NC = B * B * B;
     constexpr bool KEEP = CPL <= 8;
@@ -1463,12 +1516,12 @@ __device__ __forceinline__ void meshlet_cut(Cell&& cell, const uint8_t* s_ntri, 
             const uint32_t e = info[j];
             p = (e >> 18) & 7u;
             if (k >= KFAR) v = e >> 21;
-            else if (p) v = __popc(added_edges<B>(e & 0xFFFu, (e >> 12) & 63u, k, s_newtab));
+            else if (p) v = __popc(added_edges<B>(e & 0xFFFu, (e >> 12) & 63u, k, s_newtab, s_cls));
         } else {
             uint32_t cs, nb;
             cell(j, cs, nb);
             p = s_ntri[cs];
-            if (p) v = __popc(added_edges<B>(s_emu[cs], nb, k, s_newtab));
+            if (p) v = __popc(added_edges<B>(s_emu[cs], nb, k, s_newtab, s_cls));
         }
     };
     for (;;) {
@@ -2539,9 +2592,10 @@ __global__ __launch_bounds__(256) void k_meshlet_cut(ExtractArgs a)
     __shared__ uint8_t s_ntri[256];
     __shared__ uint16_t s_emu[256];
     __shared__ uint16_t s_newtab[64 * kBackClasses];
+    __shared__ uint8_t s_cls[B * B + B];
     __shared__ uint32_t s_sb_all[4][NW];
     for (uint32_t c = threadIdx.x; c < 256u; c += blockDim.x) s_ntri[c] = (uint8_t)mtx_word_ntri(mtx_tri_words[c]);
-    meshlet_tables<B>(s_emu, s_newtab);
+    meshlet_tables<B>(s_emu, s_newtab, s_cls);
     __syncthreads();
     const uint32_t lane = lane_id();
     uint32_t* s_sb = s_sb_all[threadIdx.x >> 6];
@@ -2570,7 +2624,7 @@ __global__ __launch_bounds__(256) void k_meshlet_cut(ExtractArgs a)
                 cs = lc.cs(j);
                 nb = nbrs_of_cell<B>(lane * CPL + j, vlim);
             },
-            s_ntri, s_emu, s_newtab, nm, ns, sb);
+            s_ntri, s_emu, s_newtab, s_cls, nm, ns, sb);
         if constexpr (CPL == 8 && NW == 16) {
             // lane l's 8 cells are byte l of the block's 64
             reinterpret_cast<uint8_t*>(a.starts)[(uint64_t)i * 64 + lane] = (uint8_t)sb;
@@ -2598,8 +2652,8 @@ template <int B> struct EmitCfg {
     static constexpr bool KIN = B <= 8;
     static constexpr int MAXM = NC / 5 + 2;      // every meshlet but a block's last has >= 5 cells
     static constexpr int TL = MAXM > LIST ? MAXM : LIST;
-    static constexpr int WAVE_LDS = NC * 4 + (KIN ? 0 : NC * 2) + TL * 4 + 3 * LIST * 4;
-    static constexpr int WG_LDS = 256 + 512 + 64 * (int)kBackClasses * 2 + 256 * 5 * 2;
+    static constexpr int WAVE_LDS = NC * 4 + NC + (KIN ? 0 : NC * 2) + TL * 4 + 3 * LIST * 4;
+    static constexpr int WG_LDS = 256 + 512 + 64 * (int)kBackClasses * 2 + 256 * 5 * 2 + 64 * 12 * 8 + NC * 0;
     static constexpr int WPG = WAVE_LDS * 4 + WG_LDS <= 65536 ? 4 : (WAVE_LDS * 2 + WG_LDS <= 65536 ? 2 : 1);
 };
 
@@ -2615,7 +2669,7 @@ struct EmitBlock {
 };
 
 template <int B>
-__global__ __launch_bounds__(64 * EmitCfg<B>::WPG) void k_meshlet_emit(ExtractArgs a)
+__global__ __launch_bounds__(64 * EmitCfg<B>::WPG, B <= 8 ? 4 : 1) void k_meshlet_emit(ExtractArgs a)
 {
     using C = EmitCfg<B>;
     using EBk = EmitBlock<B>;
@@ -2627,12 +2681,15 @@ __global__ __launch_bounds__(64 * EmitCfg<B>::WPG) void k_meshlet_emit(ExtractAr
     __shared__ uint8_t s_ntri[256];
     __shared__ uint16_t s_emu[256];
     __shared__ uint16_t s_newtab[64 * kBackClasses];
+    __shared__ uint8_t s_cls[B * B + B];
     __shared__ uint16_t s_tu[256 * 5];          // owner indices of each decoded triangle's vertices
+    __shared__ uint64_t s_fu[64 * 12];          // first-user table
     // per wave: each cell's first meshlet vertex in the block (20 bits) | its
     // added edges << 20; the cells' distance back to their meshlet's start when
     // the list entry cannot hold it; the triangle list (then the descriptors'
     // inputs); the listed triangles' shared-mesh indices
     __shared__ uint32_t s_mc_all[WPG][NC];
+    __shared__ uint8_t s_nb_all[WPG][NC];      // each cell's neighbour bits
     __shared__ uint16_t s_mk_all[KIN ? 1 : WPG][KIN ? 1 : NC];
     __shared__ uint32_t s_tl_all[WPG][C::TL];
     __shared__ __align__(16) uint32_t s_idx_all[WPG][3 * LIST];
@@ -2645,10 +2702,12 @@ __global__ __launch_bounds__(64 * EmitCfg<B>::WPG) void k_meshlet_emit(ExtractAr
             for (uint32_t j = 0; j < 3u; ++j) tu |= mtx_edge_uidx(mtx_word_edge(wd, k, j)) << (4u * j);
         s_tu[x] = (uint16_t)tu;
     }
-    meshlet_tables<B>(s_emu, s_newtab);
+    meshlet_tables<B>(s_emu, s_newtab, s_cls);
+    first_user_table<B>(s_fu);
     __syncthreads();
     const uint32_t lane = lane_id(), w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     uint32_t* s_mc = s_mc_all[w];
+    uint8_t* s_nb = s_nb_all[w];
     uint16_t* s_mk = s_mk_all[KIN ? 0 : w];
     uint32_t* s_tl = s_tl_all[w];
     uint32_t* s_idx = s_idx_all[w];
@@ -2699,19 +2758,23 @@ __global__ __launch_bounds__(64 * EmitCfg<B>::WPG) void k_meshlet_emit(ExtractAr
             wave_excl_max(sbits ? lane * CPL + 63u - (uint32_t)__builtin_clzll(sbits) : 0u);
         // each cell's added edges and distance back to its meshlet's start,
         // its triangle count; the block's meshlet vertices and triangles before it
+        constexpr bool KEEPM = CPL <= 8;   // the added-edge masks stay in registers
         uint32_t S = S0, cnt_v = 0, npk = 0, nt = 0;
+        uint32_t mkeep[KEEPM ? CPL : 1];
         auto added = [&](int j, uint32_t& k) -> uint32_t {
             const uint32_t c = lane * CPL + j;
             if ((sbits >> j) & 1ull) S = c;
             k = c - S;
             const uint32_t cs = cur.lc.cs(j);
             if (c >= (uint32_t)NC || !s_ntri[cs]) return 0u;
-            return added_edges<B>(s_emu[cs], nbrs_of_cell<B>(c, vlim), k, s_newtab);
+            return added_edges<B>(s_emu[cs], nbrs_of_cell<B>(c, vlim), k, s_newtab, s_cls);
         };
 #pragma unroll
         for (int j = 0; j < CPL; ++j) {
             uint32_t k;
-            cnt_v += __popc(added(j, k));
+            const uint32_t m = added(j, k);
+            if constexpr (KEEPM) mkeep[j] = m;
+            cnt_v += __popc(m);
             const uint32_t n = s_ntri[cur.lc.cs(j)];
             nt += n;
             if (PACKN) npk |= n << (3 * j);
@@ -2730,10 +2793,17 @@ __global__ __launch_bounds__(64 * EmitCfg<B>::WPG) void k_meshlet_emit(ExtractAr
 #pragma unroll
         for (int j = 0; j < CPL; ++j) {
             const uint32_t c = lane * CPL + j;
-            uint32_t k;
-            const uint32_t m = added(j, k);
+            uint32_t k, m;
+            if constexpr (KEEPM) {
+                if ((sbits >> j) & 1ull) S = c;
+                k = c - S;
+                m = mkeep[j];
+            } else {
+                m = added(j, k);
+            }
             if (c < (uint32_t)NC) {
                 s_mc[c] = slot | (m << 20);
+                s_nb[c] = (uint8_t)nbrs_of_cell<B>(c, vlim);
                 if constexpr (!KIN) s_mk[c] = (uint16_t)k;
             }
             slot += __popc(m);
@@ -2773,7 +2843,7 @@ __global__ __launch_bounds__(64 * EmitCfg<B>::WPG) void k_meshlet_emit(ExtractAr
                 const uint32_t u = ((uint32_t)s_tu[(e >> CB) & 2047u] >> (4 * e3)) & 15u;
                 const uint32_t kk = KIN ? e >> (CB + 11) : (uint32_t)s_mk[c];
                 uint32_t back, u1;
-                first_user<B>(u, nbrs_of_cell<B>(c, vlim), kk, back, u1);
+                first_user_of(s_fu[(uint32_t)s_nb[c] * 12u + u], u, kk, back, u1);
                 const uint32_t mc = s_mc[c - back];
                 const uint32_t mv = msb + (mc & 0xFFFFFu) + __popc((mc >> 20) & ((1u << u1) - 1u));
                 a.mvmap[mv] = s_idx[v];

@@ -9,6 +9,8 @@ _hip = None
 def hip():
     global _hip
     if _hip is None:
+        from ._lib import _one_runtime
+        _one_runtime()
         for name in ("libamdhip64.so", os.path.join("/opt/rocm/lib", "libamdhip64.so")):
             try:
                 _hip = ctypes.CDLL(name)

@@ -108,3 +108,20 @@ def test_cpp_header_compiles_standalone(tmp_path):
     src.write_text('#include "meshtrex_amd/managers.hpp"\nint main(){ meshtrex::PushConstants pc; return (int)sizeof(pc) - 64; }\n')
     subprocess.check_call(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only",
                            "-I" + os.path.join(ROOT, "include"), str(src)])
+
+
+@pytest.mark.parametrize("torch_first", [False, True])
+def test_one_hip_runtime_whatever_the_import_order(torch_first):
+    """torch ships its own HIP/HSA runtimes; with either import order the
+    process must hold one copy of each (two cannot both open the GPU)."""
+    import subprocess
+    import sys
+    code = ("import sys; sys.path.insert(0, %r)\n" % ROOT +
+            ("import torch\n" if torch_first else "") +
+            "from meshtrex_amd import _lib as L\nL.lib()\n" +
+            ("" if torch_first else "import torch\n") +
+            "maps = {l.split()[-1] for l in open('/proc/self/maps') if 'amdhip64' in l or 'hsa-runtime' in l}\n"
+            "print(sum('libamdhip64' in m for m in maps), sum('libhsa-runtime64' in m for m in maps))\n")
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.split() == ["1", "1"], out.stdout
