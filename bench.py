@@ -473,20 +473,21 @@ def bench_ingest(args, ctx, vol_ptr, dims, minmax_ms):
     copy_s = min(times)
     res["pinned_copy_only"] = {"ms": round(copy_s * 1e3, 2), "GBps": round(nbytes / copy_s / 1e9, 2),
                                "what": "one synchronous copy of the whole volume"}
-    # the same copies the slab pipeline makes (256 MB of whole 8-plane block
-    # rows each, plus the plane shared with the next slab), back to back on one
-    # stream with nothing else: what the pipeline's min/max has to hide behind
+    # the same copies the slab pipeline makes (a head of whole 8-plane block
+    # rows plus the plane shared with the tail, then the tail: 1/50 of the
+    # block rows, at least one), back to back on one stream with nothing
+    # else: what the pipeline's min/max has to hide behind
     from meshtrex_amd import hiprt
     X, Y, Z = dims
     plane = X * Y * 4
-    rows = max(1, (256 << 20) // plane // 8)
-    cuts, copied, k = [], 0, 0
-    while k * rows < (Z + 7) // 8:
-        need = min(Z, (min((k + 1) * rows, (Z + 7) // 8)) * 8 + 1)
+    gz = (Z + 7) // 8
+    ends = [gz] if gz < 4 else [gz - (gz + 49) // 50, gz]
+    cuts, copied = [], 0
+    for bze in ends:
+        need = min(Z, bze * 8 + 1)
         if need > copied:
             cuts.append((copied, need))
             copied = need
-        k += 1
     d = DeviceBuffer(nbytes)
     hip = hiprt.hip()
     hip.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p]
@@ -509,6 +510,7 @@ def bench_ingest(args, ctx, vol_ptr, dims, minmax_ms):
                              "slabbed_ms": res["pinned"]["ms"],
                              "hidden_fraction_of_minmax": round(max(0.0, min(1.0, (serial - res["pinned"]["ms"])
                                                                              / minmax_ms)), 3),
+                             "slabbed_minus_copy_only_ms": round(res["pinned"]["ms"] - copy_s * 1e3, 3),
                              "note": "the min/max is under 1 % of the PCIe time: the pipeline can hide at most that"}
     return res
 

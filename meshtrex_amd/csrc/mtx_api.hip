@@ -65,12 +65,21 @@ mtx_options defaults()
     return o;
 }
 
-// Bytes per uploaded slab of a host volume: large enough for full-rate DMA,
-// small enough that min/max of the first slabs runs while the rest uploads.
-constexpr uint64_t kSlabBytes = 256ull << 20;
+// Block planes of the last uploaded slab of a host volume: min/max streams
+// the volume about 100x faster than PCIe delivers it (6 TB/s against
+// 30-60 GB/s), so the min/max of everything before the tail finishes while
+// the tail (1/50 of the block planes, at least one) uploads, and only the
+// tail's own min/max and one extra copy's start-up (~40 us) are left after
+// the last byte lands.  More slabs add a start-up each and hide nothing more.
+uint32_t tail_block_planes(uint32_t gz)
+{
+    if (gz < 4) return gz;   // one slab
+    const uint32_t t = (gz + 49) / 50;
+    return t;
+}
 
-// Host volume -> device in slabs of whole block planes on the context's copy
-// stream; after each slab (and the one plane it shares with the next) is
+// Host volume -> device in two slabs of whole block planes on the context's
+// copy stream; once a slab (and the one plane it shares with the next) is
 // resident, the compute stream runs the B = 8 min/max over its block planes.
 // Pinned host memory makes the copies asynchronous DMA; pageable memory is
 // staged by the runtime.
@@ -79,13 +88,12 @@ hipError_t upload_and_minmax(mtx_context* ctx, hipStream_t s, const void* host, 
                              uint32_t rows)
 {
     const uint64_t plane = (uint64_t)v.X * v.Y * esz;
-    uint64_t slab_planes = kSlabBytes / (plane ? plane : 1);
-    uint32_t slab_blocks = (uint32_t)(slab_planes / 8 ? slab_planes / 8 : 1);
-    if (slab_blocks > g.gz) slab_blocks = g.gz;
-    const uint32_t nslab = (g.gz + slab_blocks - 1) / slab_blocks;
+    const uint32_t tail = tail_block_planes(g.gz);
+    const uint32_t cuts[2] = { g.gz - tail, g.gz };   // block-plane ends of the slabs
+    const uint32_t first = cuts[0] ? 0u : 1u;         // gz < 4: the tail is the whole volume
     hipError_t e = hipSuccess;
     if (!ctx->copy) e = hipStreamCreateWithFlags(&ctx->copy, hipStreamNonBlocking);
-    while (e == hipSuccess && ctx->slab_events.size() < nslab) {
+    while (e == hipSuccess && ctx->slab_events.size() < 2) {
         hipEvent_t ev = nullptr;
         e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
         if (e == hipSuccess) ctx->slab_events.push_back(ev);
@@ -95,9 +103,9 @@ hipError_t upload_and_minmax(mtx_context* ctx, hipStream_t s, const void* host, 
     e = hipEventRecord(ctx->slab_events[0], s);
     if (e == hipSuccess) e = hipStreamWaitEvent(ctx->copy, ctx->slab_events[0], 0);
     uint64_t copied = 0;  // planes resident
-    for (uint32_t k = 0; e == hipSuccess && k < nslab; ++k) {
-        const uint32_t bzb = k * slab_blocks;
-        const uint32_t bze = bzb + slab_blocks < g.gz ? bzb + slab_blocks : g.gz;
+    uint32_t bzb = 0;
+    for (uint32_t k = first; e == hipSuccess && k < 2; ++k) {
+        const uint32_t bze = cuts[k];
         uint64_t need = (uint64_t)bze * 8 + 1;  // planes [0, need) for these blocks
         if (need > v.Z) need = v.Z;
         if (need > copied) {
@@ -108,6 +116,7 @@ hipError_t upload_and_minmax(mtx_context* ctx, hipStream_t s, const void* host, 
         if (e == hipSuccess) e = hipEventRecord(ctx->slab_events[k], ctx->copy);
         if (e == hipSuccess) e = hipStreamWaitEvent(s, ctx->slab_events[k], 0);
         if (e == hipSuccess) e = launch_minmax_b8(v, g, bzb, bze, pairs, mask, act, rows, s);
+        bzb = bze;
     }
     return e;
 }

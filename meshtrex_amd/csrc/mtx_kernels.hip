@@ -1699,13 +1699,21 @@ __global__ __launch_bounds__(256) void k_extract_count(ExtractArgs a)
 // operations on its four 9-bit rows (y, y+1) x (z, z+1).  A lane's 12 tile
 // coordinates are packed once; the loads take 32-bit offsets from the tile
 // origin.
-template <typename T, bool BATCH>
+//
+// CUT: the block's meshlets are cut from the cases right after they are
+// classified (the cut needs nothing else), instead of in a launch of its own.
+template <typename T, bool BATCH, bool CUT>
 __global__ __launch_bounds__(256) void k_extract_count_b8(ExtractArgs a)
 {
     constexpr int B = 8, E = 9, NTILE = 729, LPL = 12, NC = 512;
     __shared__ uint64_t s_bits_all[4][LPL];
     __shared__ uint8_t s_ntri[256];
+    // meshlet cut tables (CUT only)
+    __shared__ uint16_t s_emu[CUT ? 256 : 1];
+    __shared__ uint16_t s_newtab[CUT ? 64 * kBackClasses : 1];
+    __shared__ uint8_t s_cls[CUT ? B * B + B : 1];
     for (uint32_t c = threadIdx.x; c < 256u; c += blockDim.x) s_ntri[c] = (uint8_t)mtx_word_ntri(mtx_tri_words[c]);
+    if constexpr (CUT) meshlet_tables<B>(s_emu, s_newtab, s_cls);
     __syncthreads();
     const uint32_t lane = lane_id();
     uint64_t* s_bits = s_bits_all[threadIdx.x >> 6];
@@ -1803,7 +1811,22 @@ __global__ __launch_bounds__(256) void k_extract_count_b8(ExtractArgs a)
         }
         *reinterpret_cast<uint4*>(table + (uint64_t)i * NC + lane * 8u) = *reinterpret_cast<const uint4*>(ent);
         *reinterpret_cast<uint2*>(a.cases + (uint64_t)i * NC + lane * 8u) = make_uint2(cases0, cases1);
-        if (lane == 0) *reinterpret_cast<uint4*>(a.blk_counts + 4 * (uint64_t)i) = make_uint4(tv, tt, 0u, 0u);
+        uint32_t nm = 0, ns = 0;
+        if constexpr (CUT) {
+            // this lane's cells are lane * 8 + lx (lx = 0..7), the cells of its row
+            const uint32_t vx = X - 1u > ox ? min(X - 1u - ox, (uint32_t)B) : 0u;
+            const uint32_t vy = Y - 1u > oy ? min(Y - 1u - oy, (uint32_t)B) : 0u;
+            const uint32_t vz = Z - 1u > oz ? min(Z - 1u - oz, (uint32_t)B) : 0u;
+            uint64_t sb;
+            meshlet_cut<B, 8>(
+                [&](int j, uint32_t& cs, uint32_t& nb) {
+                    cs = ((j < 4 ? cases0 : cases1) >> (8 * (j & 3))) & 0xFFu;
+                    nb = cell_nbrs((uint32_t)j, ly, lz, vx, vy, vz);
+                },
+                s_ntri, s_emu, s_newtab, s_cls, nm, ns, sb);
+            reinterpret_cast<uint8_t*>(a.starts)[(uint64_t)i * 64 + lane] = (uint8_t)sb;
+        }
+        if (lane == 0) *reinterpret_cast<uint4*>(a.blk_counts + 4 * (uint64_t)i) = make_uint4(tv, tt, nm, ns);
         wave_lds_sync();
     }
 }
@@ -2874,6 +2897,337 @@ __global__ __launch_bounds__(64 * EmitCfg<B>::WPG, B <= 8 ? 4 : 1) void k_meshle
     }
 }
 
+// Triangle and meshlet pass in one launch (single volume, meshlets on).  Per
+// wave and active block the block's triangles are listed once, and every
+// triangle vertex is resolved twice: to the shared mesh (the first vertex at
+// its edge's owner corner, from the block's and its +x/+y/+z neighbours'
+// count-pass entries, as k_extract_triangles does) and to its meshlet (the
+// edge's first user in the meshlet and the edge's rank among that cell's
+// added edges, as k_meshlet_emit does).  One pass writes the index buffer,
+// the meshlet indices, the meshlet vertex map and the descriptors; the
+// separate passes listed the triangles twice and the emit read the index
+// buffer back.  The next block's inputs load during this one.
+template <int B> struct TriMeshCfg {
+    static constexpr int NC = B * B * B;
+    static constexpr int F = B + 1, NF = F * F * F, NFACE = NF - NC;
+    static constexpr int LIST = 256;             // triangles listed at a time
+    // list entry: cell index (CB bits) | decoded triangle (case * 5 + k) << CB
+    // [| B <= 8: the cell's distance back to its meshlet's first cell << CB + 11]
+    static constexpr int CB = B <= 8 ? 9 : 12;
+    static constexpr bool KIN = B <= 8;
+    static constexpr int MAXM = NC / 5 + 2;      // every meshlet but a block's last has >= 5 cells
+    static constexpr int TL = MAXM > LIST ? MAXM : LIST;
+    using TDec = typename TriCfg<B>::TDec;
+    static constexpr int DB = TriCfg<B>::DB;
+    static constexpr int WAVE_LDS = NF * 5 + NC * 5 + (KIN ? 0 : NC * 2) + TL * 4;
+    static constexpr int WG_LDS = 256 + 1280 * (int)sizeof(TDec) + 1280 * 2 + 4 * NFACE + 512 +
+                                  64 * (int)kBackClasses * 2 + (B * B + B) + 64 * 12 * 8;
+    static constexpr int WPG = WAVE_LDS * 4 + WG_LDS <= 65536 ? 4 : (WAVE_LDS * 2 + WG_LDS <= 65536 ? 2 : 1);
+    static constexpr bool FITS = WAVE_LDS * WPG + WG_LDS <= 65536;
+};
+
+template <int B>
+struct TriMeshBlock : TriBlock<B> {
+    uint32_t bnm, bns, msb;
+    uint64_t sbits;   // this lane's meshlet start bits
+};
+
+template <int B>
+__global__ __launch_bounds__(64 * TriMeshCfg<B>::WPG) void k_tri_meshlets(ExtractArgs a)
+{
+    using C = TriMeshCfg<B>;
+    using TB = TriMeshBlock<B>;
+    constexpr int WPG = C::WPG, F = C::F, NF = C::NF, NC = C::NC, CB = C::CB, DB = C::DB;
+    constexpr int CPL = EB<B>::CPL;
+    constexpr uint32_t LIST = C::LIST, NW = StartBits<B>::NW;
+    constexpr bool KIN = C::KIN;
+    constexpr bool PACKN = CPL * 3 <= 32;
+    constexpr int NFACE = C::NFACE, NPL = TriBlock<B>::NPL;
+    using TDec = typename C::TDec;
+    using Entry = typename EB<B>::Entry;
+    __shared__ uint8_t s_ntri[256];
+    __shared__ TDec s_tdec[256 * 5];            // owner-corner offsets and rank masks per decoded triangle
+    __shared__ uint16_t s_tu[256 * 5];          // owner indices of each decoded triangle's vertices
+    __shared__ uint32_t s_face[NFACE];
+    __shared__ uint16_t s_emu[256];
+    __shared__ uint16_t s_newtab[64 * kBackClasses];
+    __shared__ uint8_t s_cls[B * B + B];
+    __shared__ uint64_t s_fu[64 * 12];          // first-user table
+    // per wave: the first vertex and edge flags of every owner corner of
+    // [0, B]^3; each cell's first meshlet vertex in the block (20 bits) | its
+    // added edges << 20, and neighbour bits; the cells' distance back to their
+    // meshlet's start when the list entry cannot hold it; the triangle list
+    // (then the descriptors' inputs)
+    __shared__ uint32_t s_gidx_all[WPG][NF];
+    __shared__ uint8_t s_gfl_all[WPG][NF];
+    __shared__ uint32_t s_mc_all[WPG][NC];
+    __shared__ uint8_t s_nb_all[WPG][NC];
+    __shared__ uint16_t s_mk_all[KIN ? 1 : WPG][KIN ? 1 : NC];
+    __shared__ uint32_t s_tl_all[WPG][C::TL];
+    for (uint32_t c = threadIdx.x; c < 256u; c += blockDim.x) s_ntri[c] = (uint8_t)mtx_word_ntri(mtx_tri_words[c]);
+    for (uint32_t x = threadIdx.x; x < 256u * 5u; x += blockDim.x) {
+        const uint64_t wd = mtx_tri_words[x / 5u];
+        const uint32_t k = x % 5u;
+        TDec d = 0;
+        uint32_t tu = 0;
+        if (k < mtx_word_ntri(wd)) {
+            for (uint32_t j = 0; j < 3u; ++j) {
+                const uint32_t e = mtx_word_edge(wd, k, j);
+                const uint32_t o = (uint32_t)(kEdgeOwners >> (5u * e)) & 0x1Fu;
+                const uint32_t off = (o & 1u) + F * (((o >> 1) & 1u) + F * ((o >> 2) & 1u));
+                const uint32_t m = (1u << (o >> 3)) - 1u;
+                d |= (TDec)(off | (m << (DB - 2))) << (DB * j);
+                tu |= mtx_edge_uidx(e) << (4u * j);
+            }
+        }
+        s_tdec[x] = d;
+        s_tu[x] = (uint16_t)tu;
+    }
+    for (uint32_t f = threadIdx.x; f < (uint32_t)NFACE; f += blockDim.x) {
+        uint32_t qx, qy, qz;
+        face_corner<B>(f, qx, qy, qz);
+        const uint32_t sx = qx == (uint32_t)B, sy = qy == (uint32_t)B, sz = qz == (uint32_t)B;
+        s_face[f] = (qx + F * (qy + F * qz)) | ((sx | (sy << 1) | (sz << 2)) << 13) |
+                    (((qx - B * sx) + B * ((qy - B * sy) + B * (qz - B * sz))) << 16);
+    }
+    meshlet_tables<B>(s_emu, s_newtab, s_cls);
+    first_user_table<B>(s_fu);
+    __syncthreads();
+    const uint32_t lane = lane_id(), w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    uint32_t* s_gidx = s_gidx_all[w];
+    uint8_t* s_gfl = s_gfl_all[w];
+    uint32_t* s_mc = s_mc_all[w];
+    uint8_t* s_nb = s_nb_all[w];
+    uint16_t* s_mk = s_mk_all[KIN ? 0 : w];
+    uint32_t* s_tl = s_tl_all[w];
+    const uint32_t na = active_items(a);
+    const Entry* __restrict__ table = static_cast<const Entry*>(a.corner_table);
+    const uint32_t* __restrict__ vb32 = reinterpret_cast<const uint32_t*>(a.blk_vbase);
+    const uint32_t* __restrict__ mb32 = reinterpret_cast<const uint32_t*>(a.blk_mbase);
+    const uint32_t* __restrict__ sb32 = reinterpret_cast<const uint32_t*>(a.blk_sbase);
+    const uint32_t gx = a.grid.gx, gy = a.grid.gy, gz = a.grid.gz;
+    auto stage1 = [&](TB& P, uint32_t i) {
+        const uint4 cnt = *reinterpret_cast<const uint4*>(a.blk_counts + 4 * (uint64_t)i);
+        P.bnv = cnt.x;
+        P.bnt = cnt.y;
+        P.bnm = cnt.z;
+        P.bns = cnt.w;
+        P.vb = vb32[2 * (uint64_t)i];
+        P.tb = a.blk_tbase[i];
+        P.mb = mb32[2 * (uint64_t)i];
+        P.msb = sb32[2 * (uint64_t)i];
+        P.id = a.ids[i];
+        if constexpr (CPL == 8 && NW == 16)
+            P.sbits = reinterpret_cast<const uint8_t*>(a.starts)[(uint64_t)i * 64 + lane];
+        else
+            P.sbits = lane_start_bits<B, CPL>(a.starts + (uint64_t)i * NW, lane);
+        P.lc.load(table, a.cases, i, lane, true);
+    };
+    auto stage2 = [&](TB& P) {
+        const uint32_t bx = P.id % gx, by = (P.id / gx) % gy, bz = P.id / (gx * gy);
+        const uint32_t nx = bx + (lane & 1u), ny = by + ((lane >> 1) & 1u), nz = bz + ((lane >> 2) & 1u);
+        P.r = -1;
+        if (lane != 0 && lane < 7 && nx < gx && ny < gy && nz < gz)
+            P.r = (int32_t)a.blk_map[nx + gx * (ny + (uint64_t)gy * nz)];
+    };
+    auto stage3 = [&](TB& P) {
+        P.nbase = (lane < 8 && P.r >= 0) ? vb32[2 * (uint64_t)P.r] : 0u;
+#pragma unroll
+        for (int k = 0; k < TB::NFW; ++k) P.fentw[k] = 0xFFFFFFFFu;
+#pragma unroll
+        for (int k = 0; k < NPL; ++k) {
+            const uint32_t f = lane + 64u * k;
+            const uint32_t fd = s_face[f < (uint32_t)NFACE ? f : 0u];
+            const int32_t jn = __shfl(P.r, (int)((fd >> 13) & 7u), 64);
+            if (f < (uint32_t)NFACE && jn >= 0) P.set_face(k, (uint32_t)table[(uint64_t)jn * NC + (fd >> 16)]);
+        }
+    };
+
+    const WaveItems it = wave_items(na, WPG);
+    TB cur, nxt;
+    uint32_t i = it.first;
+    if (i < it.end) {
+        stage1(cur, i);
+        stage2(cur);
+        stage3(cur);
+    }
+    while (i < it.end) {
+        const uint32_t inx = i + it.stride;
+        const bool more = inx < it.end;
+        if (more) stage1(nxt, inx);
+        const uint32_t bnt = cur.bnt, bnm = cur.bnm, bns = cur.bns;
+        const uint64_t tb = cur.tb;
+        const uint32_t mb = cur.mb, msb = cur.msb;
+        bool work = bnt != 0;
+        if (work && (3 * (tb + bnt) > a.icap || (uint64_t)msb + bns > a.mvcap || (uint64_t)mb + bnm > a.mcap ||
+                     bnm > (uint32_t)C::MAXM)) {
+            if (lane == 0) atomicOr(reinterpret_cast<unsigned long long*>(&a.totals[2]), 1ull);
+            work = false;
+        }
+        uint32_t vlim = 0, S0 = 0, slot = 0, nt = 0, npk = 0, cnt_v = 0;
+        const uint64_t sbits = cur.sbits;
+        if (work) {
+            // own corners: vertex index = block base + prefix from the count pass
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) {
+                const uint32_t c = lane * CPL + j;
+                if (c < (uint32_t)NC) {
+                    const uint32_t q = c % B + F * ((c / B) % B + F * (c / (B * B)));
+                    s_gidx[q] = cur.vb + (cur.lc.ent(j) >> 3);
+                    s_gfl[q] = (uint8_t)(cur.lc.ent(j) & 7u);
+                }
+            }
+            // neighbour-owned face corners
+#pragma unroll
+            for (int k = 0; k < NPL; ++k) {
+                const uint32_t f = lane + 64u * k;
+                const uint32_t fd = s_face[f < (uint32_t)NFACE ? f : 0u];
+                const uint32_t nbv = (uint32_t)__shfl((int)cur.nbase, (int)((fd >> 13) & 7u), 64);
+                if (f < (uint32_t)NFACE) {
+                    const uint32_t q = fd & 0x1FFFu;
+                    const uint32_t e = cur.face(k);
+                    s_gidx[q] = e != TriBlock<B>::kNone ? nbv + (e >> 3) : 0u;
+                    s_gfl[q] = e != TriBlock<B>::kNone ? (uint8_t)(e & 7u) : (uint8_t)0xF0u;
+                }
+            }
+            // each cell's added edges and distance back to its meshlet's start,
+            // its triangle count
+            block_cells(a, cur.id, vlim);
+            S0 = wave_excl_max(sbits ? lane * CPL + 63u - (uint32_t)__builtin_clzll(sbits) : 0u);
+            uint32_t S = S0;
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) {
+                const uint32_t c = lane * CPL + j;
+                if ((sbits >> j) & 1ull) S = c;
+                const uint32_t cs = cur.lc.cs(j);
+                const uint32_t n = s_ntri[cs];
+                uint32_t m = 0;
+                if (c < (uint32_t)NC) {
+                    const uint32_t nb = nbrs_of_cell<B>(c, vlim);
+                    if (n) m = added_edges<B>(s_emu[cs], nb, c - S, s_newtab, s_cls);
+                    s_mc[c] = m << 20;   // the slot is added after the scan
+                    s_nb[c] = (uint8_t)nb;
+                    if constexpr (!KIN) s_mk[c] = (uint16_t)(c - S);
+                }
+                cnt_v += __popc(m);
+                nt += n;
+                if (PACKN) npk |= n << (3 * j);
+            }
+        }
+        if (more) stage2(nxt);
+        if (work) {
+            uint32_t ns, tt;
+            slot = wave_excl_scan<uint32_t>(cnt_v, ns);
+            const uint32_t et = wave_excl_scan<uint32_t>(nt, tt);
+            const uint32_t nm = wave_sum<uint32_t>((uint32_t)__popcll(sbits));
+            if (ns != bns || nm != bnm || tt != bnt) {
+                // the cut and the counts disagree with this block's data
+                if (lane == 0) atomicOr(reinterpret_cast<unsigned long long*>(&a.totals[2]), 4ull);
+            } else {
+#pragma unroll
+                for (int j = 0; j < CPL; ++j) {
+                    const uint32_t c = lane * CPL + j;
+                    if (c < (uint32_t)NC) {
+                        const uint32_t m = s_mc[c];
+                        s_mc[c] = m | slot;
+                        slot += __popc(m >> 20);
+                    }
+                }
+                uint32_t bad = 0;
+                for (uint32_t c0 = 0; c0 < tt; c0 += LIST) {
+                    const uint32_t n = min(LIST, tt - c0);
+                    // list entries in output order: cell by cell, the case's triangles k = 0..4
+                    uint32_t run = et - c0;
+                    uint32_t S = S0;
+                    // re-read per pass (opaque to the compiler): nothing derived
+                    // from them stays live across passes, which are rare
+                    LaneCorners<B> lc = cur.lc;
+#pragma unroll
+                    for (int k = 0; k < LaneCorners<B>::NCW; ++k) asm volatile("" : "+v"(lc.csw[k]));
+#pragma unroll
+                    for (int j = 0; j < CPL; ++j) {
+                        const uint32_t c = lane * CPL + j;
+                        if ((sbits >> j) & 1ull) S = c;
+                        if (c < (uint32_t)NC) {
+                            const uint32_t cs = lc.cs(j);
+                            const uint32_t nn = PACKN ? (npk >> (3 * j)) & 7u : (uint32_t)s_ntri[cs];
+                            uint32_t ent = c | ((cs * 5u) << CB);
+                            if constexpr (KIN) ent |= (c - S) << (CB + 11);
+#pragma unroll
+                            for (uint32_t k = 0; k < 5u; ++k)
+                                if (k < nn && run + k < LIST) s_tl[run + k] = ent + (k << CB);
+                            run += nn;
+                        }
+                    }
+                    wave_lds_sync();
+                    // one triangle per lane, its three vertices' lookups side by side
+                    for (uint32_t q = lane; q < n; q += 64u) {
+                        const uint32_t e = s_tl[q];
+                        const uint32_t c = e & ((1u << CB) - 1u);
+                        const uint32_t dec = (e >> CB) & 2047u;
+                        const TDec d = s_tdec[dec];
+                        const uint32_t tu = s_tu[dec];
+                        const uint32_t nbc = s_nb[c];
+                        const uint32_t kk = KIN ? e >> (CB + 11) : (uint32_t)s_mk[c];
+                        const uint32_t q0 = c % B + F * ((c / B) % B + F * (c / (B * B)));
+                        uint32_t gi[3], mv[3];
+#pragma unroll
+                        for (uint32_t e3 = 0; e3 < 3; ++e3) {
+                            // shared mesh: the owner corner's first vertex plus the edge's rank there
+                            const uint32_t f = (uint32_t)(d >> (DB * e3));
+                            const uint32_t qq = q0 + (f & ((1u << (DB - 2)) - 1u));
+                            const uint32_t gf = s_gfl[qq];
+                            bad |= gf;
+                            gi[e3] = __popc(gf & (f >> (DB - 2)) & 3u) + s_gidx[qq];
+                            // meshlet: the edge's first user in the meshlet and its rank there
+                            const uint32_t u = (tu >> (4 * e3)) & 15u;
+                            uint32_t back, u1;
+                            first_user_of(s_fu[nbc * 12u + u], u, kk, back, u1);
+                            const uint32_t mc = s_mc[c - back];
+                            mv[e3] = msb + (mc & 0xFFFFFu) + __popc((mc >> 20) & ((1u << u1) - 1u));
+                        }
+                        uint32_t* di = a.indices + 3 * (tb + c0 + q);
+                        uint32_t* dm = a.mindices + 3 * (tb + c0 + q);
+#pragma unroll
+                        for (uint32_t e3 = 0; e3 < 3; ++e3) {
+                            a.mvmap[mv[e3]] = gi[e3];
+                            __builtin_nontemporal_store(gi[e3], di + e3);
+                            __builtin_nontemporal_store(mv[e3], dm + e3);
+                        }
+                    }
+                    wave_lds_sync();
+                }
+                // an active edge whose owner block is not in the active list: the
+                // list does not belong to this volume/isovalue
+                if (bad & 0xF0u) atomicOr(reinterpret_cast<unsigned long long*>(&a.totals[2]), 2ull);
+                // descriptors: each meshlet's first meshlet vertex and first
+                // triangle within the block (16 bits each), staged at its index
+                uint32_t nmt;
+                uint32_t m = wave_excl_scan<uint32_t>((uint32_t)__popcll(sbits), nmt);
+                uint32_t tp = et;
+#pragma unroll
+                for (int j = 0; j < CPL; ++j) {
+                    const uint32_t c = lane * CPL + j;
+                    if ((sbits >> j) & 1ull) s_tl[m++] = (s_mc[c] & 0xFFFFu) | (tp << 16);
+                    tp += PACKN ? (npk >> (3 * j)) & 7u : (uint32_t)s_ntri[cur.lc.cs(j)];
+                }
+                wave_lds_sync();
+                const uint32_t tb_rel = (uint32_t)tb;
+                for (uint32_t k = lane; k < nm; k += 64u) {
+                    const uint32_t A = s_tl[k];
+                    const uint32_t Bn = k + 1 < nm ? s_tl[k + 1] : (bns | (bnt << 16));
+                    a.meshlets[(uint64_t)mb + k] = make_uint4(msb + (A & 0xFFFFu), 3u * (tb_rel + (A >> 16)),
+                                                              (Bn & 0xFFFFu) - (A & 0xFFFFu), (Bn >> 16) - (A >> 16));
+                }
+            }
+            wave_lds_sync();
+        }
+        if (more) stage3(nxt);
+        cur = nxt;
+        i = inx;
+    }
+}
+
 // Meshlet vertices: copies of the shared mesh's vertices through the meshlet
 // vertex map, two lanes per vertex (one 16-byte half each), so that a wave
 // reads its map entries and writes its copies as contiguous 1 KiB runs.  A map
@@ -2947,9 +3301,14 @@ template <int B, bool BATCH>
 static hipError_t launch_count_b(const ExtractArgs& a, hipStream_t s)
 {
     if (B == 8) {
+        if (!BATCH && a.starts) {   // with the meshlet cut
+            if (a.vol.dtype == kF32)
+                return launch_persistent(k_extract_count_b8<float, false, true>, 256, a, s, kWgPerCuSingleCount);
+            return launch_persistent(k_extract_count_b8<uint8_t, false, true>, 256, a, s, kWgPerCuSingleCount);
+        }
         if (a.vol.dtype == kF32)
-            return launch_persistent(k_extract_count_b8<float, BATCH>, 256, a, s, kWgPerCuSingleCount);
-        return launch_persistent(k_extract_count_b8<uint8_t, BATCH>, 256, a, s, kWgPerCuSingleCount);
+            return launch_persistent(k_extract_count_b8<float, BATCH, false>, 256, a, s, kWgPerCuSingleCount);
+        return launch_persistent(k_extract_count_b8<uint8_t, BATCH, false>, 256, a, s, kWgPerCuSingleCount);
     }
     if (a.vol.dtype == kF32) return launch_persistent(k_extract_count<B, float, BATCH>, 256, a, s);
     return launch_persistent(k_extract_count<B, uint8_t, BATCH>, 256, a, s);
@@ -2971,10 +3330,20 @@ static hipError_t launch_emit_b(const ExtractArgs& a, hipStream_t s)
                                           64 * VertCfg<B, false, uint8_t>::WPG, a, s, kWgPerCuSingleVertices);
     }
     if (e != hipSuccess) return e;
-    e = launch_persistent(k_extract_triangles<B, BATCH>, 64 * TriCfg<B>::WPG, a, s, kWgPerCuSingleTriangles);
-    if (e != hipSuccess || BATCH || !a.starts) return e;
-    // meshlets (not in the batched form)
-    e = launch_persistent(k_meshlet_emit<B>, 64 * EmitCfg<B>::WPG, a, s, kWgPerCuMeshlet);
+    bool fused = false;
+    if constexpr (!BATCH && TriMeshCfg<B>::FITS) {
+        if (a.starts) {
+            // triangles and meshlets in one pass
+            e = launch_persistent(k_tri_meshlets<B>, 64 * TriMeshCfg<B>::WPG, a, s, kWgPerCuSingleTriangles);
+            fused = true;
+        }
+    }
+    if (!fused) {
+        e = launch_persistent(k_extract_triangles<B, BATCH>, 64 * TriCfg<B>::WPG, a, s, kWgPerCuSingleTriangles);
+        if (e != hipSuccess || BATCH || !a.starts) return e;
+        // meshlets (not in the batched form)
+        e = launch_persistent(k_meshlet_emit<B>, 64 * EmitCfg<B>::WPG, a, s, kWgPerCuMeshlet);
+    }
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_meshlet_vertices, dim3((uint32_t)device_cus() * 8), dim3(256), 0, s,
                        reinterpret_cast<const float4*>(a.vertices), a.mvmap, reinterpret_cast<float4*>(a.mverts),
@@ -3034,7 +3403,7 @@ hipError_t launch_extract_count(const ExtractArgs& a, hipStream_t s)
 {
     if (a.units) { MTX_DISPATCH_BATCH_B(launch_count_b, a, s) }
     const hipError_t e = launch_count_single(a, s);
-    if (e != hipSuccess || !a.starts) return e;
+    if (e != hipSuccess || !a.starts || a.grid.B == 8) return e;   // B = 8: the count pass cuts
     return launch_cut(a, s);   // meshlet counts and start bits, from the cases
 }
 
