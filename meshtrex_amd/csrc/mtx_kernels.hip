@@ -2245,6 +2245,63 @@ __global__ __launch_bounds__((64 * VertCfg<B, NORMALS, T>::WPG)) void k_extract_
     }
 }
 
+// Three consecutive uint32 (4-byte aligned) as one nontemporal 12-byte store:
+// written once, read by the next stage or the caller, so kept out of the
+// caches that the next step's min/max streams through.
+typedef uint32_t mtx_u32x3 __attribute__((ext_vector_type(3), aligned(4)));
+__device__ __forceinline__ void store3_nt(uint32_t* p, uint32_t x, uint32_t y, uint32_t z)
+{
+    const mtx_u32x3 v = { x, y, z };
+    __builtin_nontemporal_store(v, reinterpret_cast<mtx_u32x3*>(p));
+}
+
+// Per-wave LDS table of the first vertex and the 3 edge flags at every owner
+// corner of [0, B]^3, for the triangle passes.  PK (every vertex index below
+// 2^28): one word per corner -- index in bits 0..27, flags in 28..30, bit 31
+// set when the corner's owner block is not active -- so a triangle vertex
+// resolves with one LDS read; otherwise a word of index and a byte of flags
+// (0xF0: owner not active).
+constexpr uint32_t kPackedIndexLimit = 1u << 28;
+
+template <bool PK>
+struct CornerLds {
+    uint32_t* idx;
+    uint8_t* fl;
+    __device__ __forceinline__ void set(uint32_t q, uint32_t index, uint32_t flags) const
+    {
+        if constexpr (PK) {
+            idx[q] = index | (flags << 28);
+        } else {
+            idx[q] = index;
+            fl[q] = (uint8_t)flags;
+        }
+    }
+    __device__ __forceinline__ void set_none(uint32_t q) const
+    {
+        if constexpr (PK) {
+            idx[q] = 0x80000000u;
+        } else {
+            idx[q] = 0u;
+            fl[q] = (uint8_t)0xF0u;
+        }
+    }
+    // index of the vertex on the edge with rank mask m (the owned edges of
+    // lower axis) at corner q; bad collects 0x80 (PK) or 0xF0 for a corner
+    // whose owner is not active
+    __device__ __forceinline__ uint32_t vertex(uint32_t q, uint32_t m, uint32_t& bad) const
+    {
+        if constexpr (PK) {
+            const uint32_t e = idx[q];
+            bad |= (e >> 24) & 0x80u;
+            return (e & (kPackedIndexLimit - 1u)) + __popc((e >> 28) & m & 3u);
+        } else {
+            const uint32_t gf = fl[q];
+            bad |= gf;
+            return __popc(gf & m & 3u) + idx[q];
+        }
+    }
+};
+
 template <int B> struct TriCfg {
     static constexpr int F = B + 1;             // owner corners [0, B]^3
     static constexpr int NF = F * F * F;
@@ -2334,7 +2391,7 @@ struct TriBlock {
 // are listed (cell, case, k) and resolved three LDS reads each by consecutive
 // lanes.  No voxel is read.  The next block's inputs are loaded during this
 // one, so no dependent global load waits at a block's start.
-template <int B, bool BATCH>
+template <int B, bool BATCH, bool PK>
 __global__ __launch_bounds__((64 * TriCfg<B>::WPG)) void k_extract_triangles(ExtractArgs a)
 {
     using C = TriCfg<B>;
@@ -2346,7 +2403,7 @@ __global__ __launch_bounds__((64 * TriCfg<B>::WPG)) void k_extract_triangles(Ext
     constexpr int NFACE = TB::NFACE, NPL = TB::NPL;
     using Entry = typename EB<B>::Entry;
     __shared__ uint32_t s_gidx_all[WPG][NF];
-    __shared__ uint8_t s_gfl_all[WPG][NF];
+    __shared__ uint8_t s_gfl_all[PK ? 1 : WPG][PK ? 1 : NF];
     __shared__ uint32_t s_tl_all[WPG][LIST];
     using TDec = typename C::TDec;
     constexpr int QB = C::QB, DB = C::DB;
@@ -2379,8 +2436,7 @@ __global__ __launch_bounds__((64 * TriCfg<B>::WPG)) void k_extract_triangles(Ext
     }
     __syncthreads();
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
-    uint32_t* s_gidx = s_gidx_all[w];
-    uint8_t* s_gfl = s_gfl_all[w];
+    const CornerLds<PK> cor{ s_gidx_all[w], s_gfl_all[PK ? 0 : w] };
     uint32_t* s_tl = s_tl_all[w];
     const uint32_t na = active_items(a);
     const Entry* __restrict__ table = static_cast<const Entry*>(a.corner_table);
@@ -2467,8 +2523,7 @@ __global__ __launch_bounds__((64 * TriCfg<B>::WPG)) void k_extract_triangles(Ext
                 const uint32_t c = lane * CPL + j;
                 if (c < (uint32_t)NC) {
                     const uint32_t q = c % B + F * ((c / B) % B + F * (c / (B * B)));
-                    s_gidx[q] = vrel + (cur.lc.ent(j) >> 3);
-                    s_gfl[q] = (uint8_t)(cur.lc.ent(j) & 7u);
+                    cor.set(q, vrel + (cur.lc.ent(j) >> 3), cur.lc.ent(j) & 7u);
                 }
                 const uint32_t n = s_ntri[cur.lc.cs(j)];
                 nt += n;
@@ -2483,8 +2538,8 @@ __global__ __launch_bounds__((64 * TriCfg<B>::WPG)) void k_extract_triangles(Ext
                 if (f < (uint32_t)NFACE) {
                     const uint32_t q = fd & 0x1FFFu;
                     const uint32_t e = cur.face(k);
-                    s_gidx[q] = e != TB::kNone ? nb + (e >> 3) : 0u;
-                    s_gfl[q] = e != TB::kNone ? (uint8_t)(e & 7u) : (uint8_t)0xF0u;
+                    if (e != TB::kNone) cor.set(q, nb + (e >> 3), e & 7u);
+                    else cor.set_none(q);
                 }
             }
         }
@@ -2528,14 +2583,10 @@ __global__ __launch_bounds__((64 * TriCfg<B>::WPG)) void k_extract_triangles(Ext
                     for (uint32_t e3 = 0; e3 < 3; ++e3) {
                         const uint32_t f = (uint32_t)(d >> (DB * e3));
                         const uint32_t qq = q0 + (f & ((1u << (DB - 2)) - 1u));
-                        const uint32_t gf = s_gfl[qq];
-                        bad |= gf;
-                        idx3[e3] = __popc(gf & (f >> (DB - 2)) & 3u) + s_gidx[qq];
+                        idx3[e3] = cor.vertex(qq, f >> (DB - 2), bad);
                     }
-                    uint32_t* di = a.indices + 3 * (tb + c0 + q);
-                    __builtin_nontemporal_store(idx3[0], di);
-                    __builtin_nontemporal_store(idx3[1], di + 1);
-                    __builtin_nontemporal_store(idx3[2], di + 2);
+                    // one 12-byte store per lane: the wave writes 768 contiguous bytes
+                    store3_nt(a.indices + 3 * (tb + c0 + q), idx3[0], idx3[1], idx3[2]);
                 }
                 wave_lds_sync();
             }
@@ -2919,11 +2970,11 @@ template <int B> struct TriMeshCfg {
     static constexpr int TL = MAXM > LIST ? MAXM : LIST;
     using TDec = typename TriCfg<B>::TDec;
     static constexpr int DB = TriCfg<B>::DB;
-    static constexpr int WAVE_LDS = NF * 5 + NC * 5 + (KIN ? 0 : NC * 2) + TL * 4;
-    static constexpr int WG_LDS = 256 + 1280 * (int)sizeof(TDec) + 1280 * 2 + 4 * NFACE + 512 +
-                                  64 * (int)kBackClasses * 2 + (B * B + B) + 64 * 12 * 8;
+    static constexpr int WAVE_LDS = NF * 5 + NC * 4 + (KIN ? 0 : NC * 2) + TL * 4;
+    static constexpr int WG_LDS = 256 + 1280 * 8 + 4 * NFACE + 512 + 64 * (int)kBackClasses * 2 + (B * B + B) +
+                                  64 * 12 * 8;
     static constexpr int WPG = WAVE_LDS * 4 + WG_LDS <= 65536 ? 4 : (WAVE_LDS * 2 + WG_LDS <= 65536 ? 2 : 1);
-    static constexpr bool FITS = WAVE_LDS * WPG + WG_LDS <= 65536;
+    static constexpr bool FITS = WAVE_LDS * WPG + WG_LDS <= 65536 && sizeof(TDec) == 4;
 };
 
 template <int B>
@@ -2932,7 +2983,7 @@ struct TriMeshBlock : TriBlock<B> {
     uint64_t sbits;   // this lane's meshlet start bits
 };
 
-template <int B>
+template <int B, bool PK>
 __global__ __launch_bounds__(64 * TriMeshCfg<B>::WPG) void k_tri_meshlets(ExtractArgs a)
 {
     using C = TriMeshCfg<B>;
@@ -2944,10 +2995,12 @@ __global__ __launch_bounds__(64 * TriMeshCfg<B>::WPG) void k_tri_meshlets(Extrac
     constexpr bool PACKN = CPL * 3 <= 32;
     constexpr int NFACE = C::NFACE, NPL = TriBlock<B>::NPL;
     using TDec = typename C::TDec;
+    static_assert(sizeof(TDec) == 4, "the decoded triangle shares a 64-bit word with its owner indices");
     using Entry = typename EB<B>::Entry;
     __shared__ uint8_t s_ntri[256];
-    __shared__ TDec s_tdec[256 * 5];            // owner-corner offsets and rank masks per decoded triangle
-    __shared__ uint16_t s_tu[256 * 5];          // owner indices of each decoded triangle's vertices
+    // per decoded triangle: owner-corner offsets and rank masks (low word),
+    // the owner indices of its vertices (bits 32..43)
+    __shared__ uint64_t s_tdu[256 * 5];
     __shared__ uint32_t s_face[NFACE];
     __shared__ uint16_t s_emu[256];
     __shared__ uint16_t s_newtab[64 * kBackClasses];
@@ -2959,9 +3012,8 @@ __global__ __launch_bounds__(64 * TriMeshCfg<B>::WPG) void k_tri_meshlets(Extrac
     // meshlet's start when the list entry cannot hold it; the triangle list
     // (then the descriptors' inputs)
     __shared__ uint32_t s_gidx_all[WPG][NF];
-    __shared__ uint8_t s_gfl_all[WPG][NF];
+    __shared__ uint8_t s_gfl_all[PK ? 1 : WPG][PK ? 1 : NF];
     __shared__ uint32_t s_mc_all[WPG][NC];
-    __shared__ uint8_t s_nb_all[WPG][NC];
     __shared__ uint16_t s_mk_all[KIN ? 1 : WPG][KIN ? 1 : NC];
     __shared__ uint32_t s_tl_all[WPG][C::TL];
     for (uint32_t c = threadIdx.x; c < 256u; c += blockDim.x) s_ntri[c] = (uint8_t)mtx_word_ntri(mtx_tri_words[c]);
@@ -2980,8 +3032,7 @@ __global__ __launch_bounds__(64 * TriMeshCfg<B>::WPG) void k_tri_meshlets(Extrac
                 tu |= mtx_edge_uidx(e) << (4u * j);
             }
         }
-        s_tdec[x] = d;
-        s_tu[x] = (uint16_t)tu;
+        s_tdu[x] = (uint64_t)d | ((uint64_t)tu << 32);
     }
     for (uint32_t f = threadIdx.x; f < (uint32_t)NFACE; f += blockDim.x) {
         uint32_t qx, qy, qz;
@@ -2994,10 +3045,8 @@ __global__ __launch_bounds__(64 * TriMeshCfg<B>::WPG) void k_tri_meshlets(Extrac
     first_user_table<B>(s_fu);
     __syncthreads();
     const uint32_t lane = lane_id(), w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    uint32_t* s_gidx = s_gidx_all[w];
-    uint8_t* s_gfl = s_gfl_all[w];
+    const CornerLds<PK> cor{ s_gidx_all[w], s_gfl_all[PK ? 0 : w] };
     uint32_t* s_mc = s_mc_all[w];
-    uint8_t* s_nb = s_nb_all[w];
     uint16_t* s_mk = s_mk_all[KIN ? 0 : w];
     uint32_t* s_tl = s_tl_all[w];
     const uint32_t na = active_items(a);
@@ -3073,8 +3122,7 @@ __global__ __launch_bounds__(64 * TriMeshCfg<B>::WPG) void k_tri_meshlets(Extrac
                 const uint32_t c = lane * CPL + j;
                 if (c < (uint32_t)NC) {
                     const uint32_t q = c % B + F * ((c / B) % B + F * (c / (B * B)));
-                    s_gidx[q] = cur.vb + (cur.lc.ent(j) >> 3);
-                    s_gfl[q] = (uint8_t)(cur.lc.ent(j) & 7u);
+                    cor.set(q, cur.vb + (cur.lc.ent(j) >> 3), cur.lc.ent(j) & 7u);
                 }
             }
             // neighbour-owned face corners
@@ -3086,8 +3134,8 @@ __global__ __launch_bounds__(64 * TriMeshCfg<B>::WPG) void k_tri_meshlets(Extrac
                 if (f < (uint32_t)NFACE) {
                     const uint32_t q = fd & 0x1FFFu;
                     const uint32_t e = cur.face(k);
-                    s_gidx[q] = e != TriBlock<B>::kNone ? nbv + (e >> 3) : 0u;
-                    s_gfl[q] = e != TriBlock<B>::kNone ? (uint8_t)(e & 7u) : (uint8_t)0xF0u;
+                    if (e != TriBlock<B>::kNone) cor.set(q, nbv + (e >> 3), e & 7u);
+                    else cor.set_none(q);
                 }
             }
             // each cell's added edges and distance back to its meshlet's start,
@@ -3106,7 +3154,6 @@ __global__ __launch_bounds__(64 * TriMeshCfg<B>::WPG) void k_tri_meshlets(Extrac
                     const uint32_t nb = nbrs_of_cell<B>(c, vlim);
                     if (n) m = added_edges<B>(s_emu[cs], nb, c - S, s_newtab, s_cls);
                     s_mc[c] = m << 20;   // the slot is added after the scan
-                    s_nb[c] = (uint8_t)nb;
                     if constexpr (!KIN) s_mk[c] = (uint16_t)(c - S);
                 }
                 cnt_v += __popc(m);
@@ -3165,9 +3212,10 @@ __global__ __launch_bounds__(64 * TriMeshCfg<B>::WPG) void k_tri_meshlets(Extrac
                         const uint32_t e = s_tl[q];
                         const uint32_t c = e & ((1u << CB) - 1u);
                         const uint32_t dec = (e >> CB) & 2047u;
-                        const TDec d = s_tdec[dec];
-                        const uint32_t tu = s_tu[dec];
-                        const uint32_t nbc = s_nb[c];
+                        const uint64_t du = s_tdu[dec];
+                        const TDec d = (TDec)(uint32_t)du;
+                        const uint32_t tu = (uint32_t)(du >> 32);
+                        const uint32_t nbc = nbrs_of_cell<B>(c, vlim);
                         const uint32_t kk = KIN ? e >> (CB + 11) : (uint32_t)s_mk[c];
                         const uint32_t q0 = c % B + F * ((c / B) % B + F * (c / (B * B)));
                         uint32_t gi[3], mv[3];
@@ -3176,9 +3224,7 @@ __global__ __launch_bounds__(64 * TriMeshCfg<B>::WPG) void k_tri_meshlets(Extrac
                             // shared mesh: the owner corner's first vertex plus the edge's rank there
                             const uint32_t f = (uint32_t)(d >> (DB * e3));
                             const uint32_t qq = q0 + (f & ((1u << (DB - 2)) - 1u));
-                            const uint32_t gf = s_gfl[qq];
-                            bad |= gf;
-                            gi[e3] = __popc(gf & (f >> (DB - 2)) & 3u) + s_gidx[qq];
+                            gi[e3] = cor.vertex(qq, f >> (DB - 2), bad);
                             // meshlet: the edge's first user in the meshlet and its rank there
                             const uint32_t u = (tu >> (4 * e3)) & 15u;
                             uint32_t back, u1;
@@ -3186,14 +3232,13 @@ __global__ __launch_bounds__(64 * TriMeshCfg<B>::WPG) void k_tri_meshlets(Extrac
                             const uint32_t mc = s_mc[c - back];
                             mv[e3] = msb + (mc & 0xFFFFFu) + __popc((mc >> 20) & ((1u << u1) - 1u));
                         }
-                        uint32_t* di = a.indices + 3 * (tb + c0 + q);
-                        uint32_t* dm = a.mindices + 3 * (tb + c0 + q);
 #pragma unroll
-                        for (uint32_t e3 = 0; e3 < 3; ++e3) {
-                            a.mvmap[mv[e3]] = gi[e3];
-                            __builtin_nontemporal_store(gi[e3], di + e3);
-                            __builtin_nontemporal_store(mv[e3], dm + e3);
-                        }
+                        for (uint32_t e3 = 0; e3 < 3; ++e3) a.mvmap[mv[e3]] = gi[e3];
+                        // a triangle's three entries as one 12-byte store per
+                        // lane (the wave writes 768 contiguous bytes; three
+                        // strided 4-byte stores rewrote each line three times)
+                        store3_nt(a.indices + 3 * (tb + c0 + q), gi[0], gi[1], gi[2]);
+                        store3_nt(a.mindices + 3 * (tb + c0 + q), mv[0], mv[1], mv[2]);
                     }
                     wave_lds_sync();
                 }
@@ -3279,10 +3324,15 @@ static int resident_wgs(const void* kernel, uint32_t threads)
     return r;
 }
 
+// per_cu_single = 0: one round of the workgroups a CU holds.
 template <typename K>
 static hipError_t launch_persistent(K kernel, uint32_t threads, const ExtractArgs& a, hipStream_t s,
                                     uint32_t per_cu_single = kWgPerCuSingle)
 {
+    if (per_cu_single == 0) {
+        const int r = resident_wgs(reinterpret_cast<const void*>(kernel), threads);
+        per_cu_single = r > 0 ? (uint32_t)r : kWgPerCuSingle;
+    }
     uint64_t n = (uint64_t)device_cus() * (a.wg_per_cu ? a.wg_per_cu : (a.units ? kWgPerCuBatch : per_cu_single));
     if (a.balanced) {
         // equal-cost ranges assume every wave runs at once: exactly one round
@@ -3334,12 +3384,20 @@ static hipError_t launch_emit_b(const ExtractArgs& a, hipStream_t s)
     if constexpr (!BATCH && TriMeshCfg<B>::FITS) {
         if (a.starts) {
             // triangles and meshlets in one pass
-            e = launch_persistent(k_tri_meshlets<B>, 64 * TriMeshCfg<B>::WPG, a, s, kWgPerCuSingleTriangles);
+            // one round of resident workgroups: each sets up about 20 KB of
+            // LDS tables, so fewer, longer-lived workgroups are faster
+            e = a.vcap < kPackedIndexLimit
+                    ? launch_persistent(k_tri_meshlets<B, true>, 64 * TriMeshCfg<B>::WPG, a, s, 0)
+                    : launch_persistent(k_tri_meshlets<B, false>, 64 * TriMeshCfg<B>::WPG, a, s, 0);
             fused = true;
         }
     }
     if (!fused) {
-        e = launch_persistent(k_extract_triangles<B, BATCH>, 64 * TriCfg<B>::WPG, a, s, kWgPerCuSingleTriangles);
+        e = a.vcap < kPackedIndexLimit
+                ? launch_persistent(k_extract_triangles<B, BATCH, true>, 64 * TriCfg<B>::WPG, a, s,
+                                    kWgPerCuSingleTriangles)
+                : launch_persistent(k_extract_triangles<B, BATCH, false>, 64 * TriCfg<B>::WPG, a, s,
+                                    kWgPerCuSingleTriangles);
         if (e != hipSuccess || BATCH || !a.starts) return e;
         // meshlets (not in the batched form)
         e = launch_persistent(k_meshlet_emit<B>, 64 * EmitCfg<B>::WPG, a, s, kWgPerCuMeshlet);
