@@ -434,12 +434,15 @@ def bench_reference_minmax(ctx, stream, buf, dims, flush):
 
 
 def bench_ingest(args, ctx, vol_ptr, dims, minmax_ms):
-    """Host volume -> min/max: slabs uploaded on a copy stream, each slab's
-    min/max run as soon as it is resident; pageable and pinned host memory,
-    beside a bare pinned copy and the device-resident min/max."""
+    """Host volume -> min/max (two slabs, the head's min/max running while the
+    tail uploads), from pageable and pinned host memory, beside a bare pinned
+    copy and the device-resident min/max; and a copy of the whole volume with
+    back-to-back min/max launches over the resident volume on another stream,
+    which shows what the overlap costs the copy."""
     import numpy as np
     from meshtrex_amd import _lib as L
-    from meshtrex_amd.managers import (MinMaxOutput, Volume, _check, makeOptions, makePushConstants,
+    from meshtrex_amd import hiprt
+    from meshtrex_amd.managers import (DeviceBuffer, MinMaxOutput, Volume, _check, makeOptions, makePushConstants,
                                        pinned_array, read_device)
     nvox = dims[0] * dims[1] * dims[2]
     nbytes = 4 * nvox
@@ -454,7 +457,7 @@ def bench_ingest(args, ctx, vol_ptr, dims, minmax_ms):
         v = Volume(dims, "float32", arr)._c()
         mm = MinMaxOutput(ctx)
         times = []
-        for _ in range(3):
+        for _ in range(4):
             t0 = time.perf_counter()
             _check(ctx, lib.mtx_compute_min_max(h, ctypes.byref(v), ctypes.byref(pc), ctypes.byref(o), None,
                                                 ctypes.byref(mm.c)))
@@ -462,56 +465,57 @@ def bench_ingest(args, ctx, vol_ptr, dims, minmax_ms):
         mm.cleanup()
         best = min(times[1:])   # the first call also allocates the device copy
         res[name] = {"ms": round(best * 1e3, 2), "GBps": round(nbytes / best / 1e9, 2)}
-    from meshtrex_amd.managers import DeviceBuffer
     d = DeviceBuffer(nbytes)
     times = []
     for _ in range(3):
         t0 = time.perf_counter()
         assert lib.mtx_memcpy_h2d(d.ptr, pin.ctypes.data, nbytes) == 0
         times.append(time.perf_counter() - t0)
-    d.free()
     copy_s = min(times)
     res["pinned_copy_only"] = {"ms": round(copy_s * 1e3, 2), "GBps": round(nbytes / copy_s / 1e9, 2),
                                "what": "one synchronous copy of the whole volume"}
-    # the same copies the slab pipeline makes (a head of whole 8-plane block
-    # rows plus the plane shared with the tail, then the tail: 1/50 of the
-    # block rows, at least one), back to back on one stream with nothing
-    # else: what the pipeline's min/max has to hide behind
-    from meshtrex_amd import hiprt
-    X, Y, Z = dims
-    plane = X * Y * 4
-    gz = (Z + 7) // 8
-    ends = [gz] if gz < 4 else [gz - (gz + 49) // 50, gz]
-    cuts, copied = [], 0
-    for bze in ends:
-        need = min(Z, bze * 8 + 1)
-        if need > copied:
-            cuts.append((copied, need))
-            copied = need
-    d = DeviceBuffer(nbytes)
+    res["device_resident_minmax_ms"] = round(minmax_ms, 3)
+    res["pinned_minus_copy_only_ms"] = round(res["pinned"]["ms"] - copy_s * 1e3, 3)
+    # the copy (events on its stream) alone, then with min/max launches over
+    # the resident volume back to back on a second stream during it
     hip = hiprt.hip()
     hip.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p]
-    s = hiprt.stream_create()
-    times = []
-    for _ in range(3):
-        t0 = time.perf_counter()
-        for a, b in cuts:
-            assert hip.hipMemcpyAsync(d.ptr + a * plane, pin.ctypes.data + a * plane, (b - a) * plane, 1, s) == 0
-        hiprt.stream_sync(s)
-        times.append(time.perf_counter() - t0)
-    hiprt.stream_destroy(s)
+    sc, sk = hiprt.stream_create(), hiprt.stream_create()
+    e0, e1 = hiprt.Event(), hiprt.Event()
+    cv = L.CVolume()
+    cv.data, cv.dims[:], cv.on_device, cv.dtype = vol_ptr, dims, 1, L.MTX_F32
+    mm = MinMaxOutput(ctx)
+    _check(ctx, lib.mtx_compute_min_max(h, ctypes.byref(cv), ctypes.byref(pc), ctypes.byref(o), sk, ctypes.byref(mm.c)))
+    hiprt.stream_sync(sk)
+    k0, k1 = hiprt.Event(), hiprt.Event()
+    out = {}
+    for label, nk in (("alone", 0), ("with_minmax", 60)):
+        best = None
+        for _ in range(2):
+            e0.record(sc)
+            assert hip.hipMemcpyAsync(d.ptr, pin.ctypes.data, nbytes, 1, sc) == 0
+            e1.record(sc)
+            k0.record(sk)
+            for _ in range(nk):
+                _check(ctx, lib.mtx_compute_min_max(h, ctypes.byref(cv), ctypes.byref(pc), ctypes.byref(o), sk,
+                                                    ctypes.byref(mm.c)))
+            k1.record(sk)
+            hiprt.stream_sync(sc)
+            hiprt.stream_sync(sk)
+            ms = e1.ms_since(e0)
+            best = ms if best is None else min(best, ms)
+            kms = k1.ms_since(k0)
+        out[label] = {"copy_ms": round(best, 3), "minmax_launches": nk, "minmax_ms_total": round(kms, 3)}
+    a, w = out["alone"], out["with_minmax"]
+    # how much longer the copy took per ms of min/max running beside it
+    out["copy_slowdown_ms_per_minmax_ms"] = round((w["copy_ms"] - a["copy_ms"]) / max(w["minmax_ms_total"], 1e-9), 3)
+    out["note"] = ("copy_slowdown_ms_per_minmax_ms near 0: a full-rate min/max beside the copy does not slow it, "
+                   "so the upload's min/max can run under the copy of the rest")
+    res["copy_under_minmax"] = out
+    mm.cleanup()
+    hiprt.stream_destroy(sc)
+    hiprt.stream_destroy(sk)
     d.free()
-    slab_s = min(times)
-    res["pinned_slab_copies_only"] = {"ms": round(slab_s * 1e3, 2), "GBps": round(nbytes / slab_s / 1e9, 2),
-                                      "copies": len(cuts)}
-    res["device_resident_minmax_ms"] = round(minmax_ms, 3)
-    serial = slab_s * 1e3 + minmax_ms
-    res["pinned_overlap"] = {"serial_slab_copies_then_minmax_ms": round(serial, 2),
-                             "slabbed_ms": res["pinned"]["ms"],
-                             "hidden_fraction_of_minmax": round(max(0.0, min(1.0, (serial - res["pinned"]["ms"])
-                                                                             / minmax_ms)), 3),
-                             "slabbed_minus_copy_only_ms": round(res["pinned"]["ms"] - copy_s * 1e3, 3),
-                             "note": "the min/max is under 1 % of the PCIe time: the pipeline can hide at most that"}
     return res
 
 

@@ -1,13 +1,15 @@
-"""Volume ingest: host volumes (pageable and pinned) upload in two slabs of
+"""Volume ingest: a host volume (pageable or pinned) uploads in two slabs of
 whole block planes (a head, and a tail of 1/50 of the block planes, at least
 one; volumes of fewer than 4 block planes in one), each slab's min/max
-running as soon as it is resident.  The result must equal the min/max of the
-same volume already on the device, and the oracle's pairs and active list,
-bit for bit, for volumes with odd dimensions of both kinds."""
+running as soon as it is resident -- with blocking copies for a synchronous
+call, on the context's copy stream for a stream-ordered one.  The result must
+equal the min/max of the same volume already on the device, and the oracle's
+pairs and active list, bit for bit, for volumes with odd dimensions."""
 import numpy as np
 import pytest
 
 import oracle
+from meshtrex_amd import hiprt
 from meshtrex_amd import (DeviceBuffer, Volume, computeMinMaxMip, extractMeshletDescriptors,
                           filterActiveBlocks, loadVolume, makeOptions, makePushConstants, pinned_array,
                           read_device)
@@ -15,10 +17,12 @@ from meshtrex_amd import (DeviceBuffer, Volume, computeMinMaxMip, extractMeshlet
 pytestmark = pytest.mark.gpu
 
 
-def minmax_of(ctx, vol, dims, iso):
+def minmax_of(ctx, vol, dims, iso, stream=None):
     pc = makePushConstants(dims, 8, iso)
-    mm = computeMinMaxMip(ctx, vol, pc)
-    f = filterActiveBlocks(ctx, mm, pc)
+    mm = computeMinMaxMip(ctx, vol, pc, stream=stream)
+    f = filterActiveBlocks(ctx, mm, pc, stream=stream)
+    if stream is not None:
+        hiprt.stream_sync(stream)
     pairs = mm.read_pairs(0)
     mask = read_device(mm.c.active_mask, np.uint32, mm.c.mask_words_per_row * mm.c.grid[1] * mm.c.grid[2])
     ids = f.read_ids()
@@ -28,9 +32,9 @@ def minmax_of(ctx, vol, dims, iso):
 
 
 @pytest.mark.parametrize("dims,dtype", [((421, 403, 399), np.float32), ((701, 651, 653), np.uint8),
-                                        ((77, 53, 20), np.float32)])
-@pytest.mark.parametrize("pinned", [False, True])
-def test_slab_upload_equals_device_resident(ctx, dims, dtype, pinned):
+                                        ((77, 53, 20), np.float32), ((45, 38, 16), np.uint8)])
+@pytest.mark.parametrize("pinned,on_stream", [(False, False), (True, False), (True, True)])
+def test_slab_upload_equals_device_resident(ctx, dims, dtype, pinned, on_stream):
     n = int(np.prod(dims))
     rng = np.random.default_rng(11)
     if dtype == np.uint8:
@@ -46,7 +50,10 @@ def test_slab_upload_equals_device_resident(ctx, dims, dtype, pinned):
         host[:] = src
     dev = DeviceBuffer.from_numpy(src)
     ref = minmax_of(ctx, Volume(dims, tname, dev), dims, iso)
-    got = minmax_of(ctx, Volume(dims, tname, host), dims, iso)
+    stream = hiprt.stream_create() if on_stream else None
+    got = minmax_of(ctx, Volume(dims, tname, host), dims, iso, stream)
+    if stream is not None:
+        hiprt.stream_destroy(stream)
     for a, b in zip(got, ref):
         np.testing.assert_array_equal(a, b)
     dev.free()
