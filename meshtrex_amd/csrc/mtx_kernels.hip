@@ -2004,7 +2004,8 @@ template <int B, bool NORMALS, typename T> struct VertCfg {
     static constexpr int TILE = ND * 64 * 4;    // LDS voxels incl. the spare lanes' segments
     static constexpr unsigned SEGB = 4 * sizeof(T);   // bytes per direct load and lane
     static constexpr int LIST = 512;            // vertices listed at a time
-    static constexpr int WAVE_LDS = TILE * (int)sizeof(T) + LIST * 2 + 16;
+    // + the 64 vertices of a round (32 bytes each), staged for coalesced stores
+    static constexpr int WAVE_LDS = TILE * (int)sizeof(T) + LIST * 2 + 16 + 64 * 32;
     static constexpr int WPG = WAVE_LDS * 4 <= 65536 ? 4 : (WAVE_LDS * 2 <= 65536 ? 2 : 1);
 };
 
@@ -2068,9 +2069,11 @@ __global__ __launch_bounds__((64 * VertCfg<B, NORMALS, T>::WPG)) void k_extract_
     using Entry = typename EB<B>::Entry;
     __shared__ __align__(16) T s_val_all[WPG][C::TILE];
     __shared__ uint16_t s_vl_all[WPG][LIST];
+    __shared__ __align__(16) float4 s_vo_all[WPG][128];
     const uint32_t lane = lane_id(), w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     T* s_val = s_val_all[w];
     uint16_t* s_vl = s_vl_all[w];
+    float4* s_vo = s_vo_all[w];
     const uint32_t na = active_items(a);
     const Entry* __restrict__ table = static_cast<const Entry*>(a.corner_table);
     constexpr int LPL = (NTILE + 63) / 64;
@@ -2196,8 +2199,14 @@ __global__ __launch_bounds__((64 * VertCfg<B, NORMALS, T>::WPG)) void k_extract_
             }
             wave_lds_sync();
             const uint32_t n = min(LIST, bnv - c0);
-            for (uint32_t v = lane; v < n; v += 64u) {
-                const uint32_t e = s_vl[v];
+            // rounds of 64 vertices, every lane taking part: a lane computes
+            // vertex r0 + lane, the round's 2 KB are staged in LDS and written
+            // as two contiguous 1 KB runs (a 32-byte vertex per lane would
+            // take two stores 32 bytes apart, each line written half at a time)
+            for (uint32_t r0 = 0; r0 < n; r0 += 64u) {
+                const uint32_t v = r0 + lane;
+                const bool live = v < n;
+                const uint32_t e = s_vl[live ? v : r0];
                 const uint32_t c = e & 0xFFFu, ax = e >> 12;
                 const uint32_t lx = c % B, ly = (c / B) % B, lz = c / (B * B);
                 const uint32_t x = ox + lx, y = oy + ly, z = oz + lz;
@@ -2227,17 +2236,28 @@ __global__ __launch_bounds__((64 * VertCfg<B, NORMALS, T>::WPG)) void k_extract_
                     const float r = rsqrtf(l2);
                     nrm = make_float4(m0 * r, m1 * r, m2 * r, 0.0f);
                 }
-                const uint64_t gv = vb + c0 + v;
-                float4* dst = reinterpret_cast<float4*>(a.vertices) + 2 * gv;
+                s_vo[2 * lane] = pos;
+                s_vo[2 * lane + 1] = nrm;
+                wave_lds_sync();
+                const uint32_t nh = 2 * min(64u, n - r0);   // 16-byte halves of this round
+                float4* dst = reinterpret_cast<float4*>(a.vertices) + 2 * (vb + c0 + r0);
+                const float4 h0 = s_vo[lane], h1 = s_vo[64 + lane];
                 // nontemporal: written once, read by the caller later
-                __builtin_nontemporal_store(mtx_f4v{pos.x, pos.y, pos.z, pos.w}, reinterpret_cast<mtx_f4v*>(dst));
-                __builtin_nontemporal_store(mtx_f4v{nrm.x, nrm.y, nrm.z, nrm.w}, reinterpret_cast<mtx_f4v*>(dst + 1));
-                if (a.edge_ids) a.edge_ids[gv] = 3ull * (x + (uint64_t)X * (y + (uint64_t)Y * z)) + ax;
-                if (a.edge_t) a.edge_t[gv] = t;
-                if (a.vox_pos) {
-                    a.vox_pos[3 * gv + 0] = px;
-                    a.vox_pos[3 * gv + 1] = py;
-                    a.vox_pos[3 * gv + 2] = pz;
+                if (lane < nh)
+                    __builtin_nontemporal_store(mtx_f4v{h0.x, h0.y, h0.z, h0.w}, reinterpret_cast<mtx_f4v*>(dst + lane));
+                if (64 + lane < nh)
+                    __builtin_nontemporal_store(mtx_f4v{h1.x, h1.y, h1.z, h1.w},
+                                                reinterpret_cast<mtx_f4v*>(dst + 64 + lane));
+                wave_lds_sync();
+                if (live) {
+                    const uint64_t gv = vb + c0 + v;
+                    if (a.edge_ids) a.edge_ids[gv] = 3ull * (x + (uint64_t)X * (y + (uint64_t)Y * z)) + ax;
+                    if (a.edge_t) a.edge_t[gv] = t;
+                    if (a.vox_pos) {
+                        a.vox_pos[3 * gv + 0] = px;
+                        a.vox_pos[3 * gv + 1] = py;
+                        a.vox_pos[3 * gv + 2] = pz;
+                    }
                 }
             }
             wave_lds_sync();
