@@ -297,6 +297,12 @@ void mtx_context_destroy(mtx_context* ctx)
         hipStreamSynchronize(ctx->copy);
         hipStreamDestroy(ctx->copy);
     }
+    if (ctx->side) {
+        hipStreamSynchronize(ctx->side);
+        hipStreamDestroy(ctx->side);
+    }
+    for (hipEvent_t e : ctx->side_ev)
+        if (e) hipEventDestroy(e);
     for (hipEvent_t e : ctx->slab_events) hipEventDestroy(e);
     delete ctx;
 }
@@ -542,6 +548,21 @@ void mtx_filtering_release(mtx_context* ctx, mtx_filtering_output* out)
 
 // ---- extraction ------------------------------------------------------------------
 
+// The context's side stream and events for the vertex pass (created on first
+// use), set in `a`.
+static hipError_t side_stream(mtx_context* ctx, ExtractArgs& a)
+{
+    hipError_t e = hipSuccess;
+    if (!ctx->side) e = hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking);
+    for (int k = 0; k < 2 && e == hipSuccess; ++k)
+        if (!ctx->side_ev[k]) e = hipEventCreateWithFlags(&ctx->side_ev[k], hipEventDisableTiming);
+    if (e != hipSuccess) return e;
+    a.side = ctx->side;
+    a.side_ev[0] = ctx->side_ev[0];
+    a.side_ev[1] = ctx->side_ev[1];
+    return hipSuccess;
+}
+
 static ExtractArgs make_extract_args(const mtx_minmax_output* mm, const mtx_filtering_output* filt,
                                      const mtx_push_constants* pc, mtx_extraction_output* out,
                                      const WsLayout& w)
@@ -632,6 +653,7 @@ mtx_status mtx_extract(mtx_context* ctx, const mtx_minmax_output* mm,
         TileSums ts{ reinterpret_cast<uint64_t*>(static_cast<char*>(out->workspace) + w.scan) };
         hipError_t e = launch_extract_count(a, bd.s);
         if (e == hipSuccess) e = launch_extract_scan(a, ts, bd.s);
+        if (e == hipSuccess) e = side_stream(ctx, a);
         if (e == hipSuccess) e = launch_extract_emit(a, bd.s);
         if (e != hipSuccess) return fail(ctx, MTX_EDEVICE, "extract launch: %s", hipGetErrorString(e));
         if (mtx_status st = mark_done(bd, &out->done)) return st;
@@ -694,7 +716,8 @@ mtx_status mtx_extract(mtx_context* ctx, const mtx_minmax_output* mm,
     }
     a = make_extract_args(mm, filt, pc, out, w);
     a.normals = o.normals ? 1 : 0;
-    e = launch_extract_emit(a, bd.s);
+    e = side_stream(ctx, a);
+    if (e == hipSuccess) e = launch_extract_emit(a, bd.s);
     if (e != hipSuccess) return fail(ctx, MTX_EDEVICE, "extract emit: %s", hipGetErrorString(e));
     if (mtx_status st = mark_done(bd, &out->done)) return st;
     out->vertexCount = (uint32_t)tot[0];

@@ -16,6 +16,8 @@ struct mtx_context {
     int device = 0;
     hipStream_t own = nullptr;
     hipStream_t copy = nullptr;           // host-to-device uploads (created on first use)
+    hipStream_t side = nullptr;           // extraction's vertex pass beside its triangle pass
+    hipEvent_t side_ev[2] = { nullptr, nullptr };
     std::vector<hipEvent_t> slab_events;  // one per uploaded slab in flight
     void* scratch = nullptr;              // this device's address of the static scratch array
     std::string last_error;

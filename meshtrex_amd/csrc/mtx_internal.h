@@ -177,6 +177,10 @@ struct ExtractArgs {
     // cost per block); persistent workgroups per CU (0: the measured default)
     uint32_t balanced;
     uint32_t wg_per_cu;
+    // single volume: a second stream and two events of the context; the
+    // vertex pass runs on it beside the triangle pass (nullptr: one stream)
+    hipStream_t side;
+    hipEvent_t side_ev[2];
 };
 
 bool extract_supported_block(uint32_t B);
