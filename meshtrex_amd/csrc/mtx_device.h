@@ -54,37 +54,74 @@ __device__ __forceinline__ bool block_active(uint32_t lo, uint32_t hi, const Act
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
 
+// Wave-wide scans on DPP lane moves (GFX9 controls, which CDNA keeps): shifts
+// by 1, 2, 4, 8 lanes inside each row of 16, then the row broadcasts of lane
+// 15 (into rows 1 and 3) and lane 31 (into rows 2 and 3).  A lane whose
+// source is outside its row, or whose row the row mask leaves out, reads 0,
+// the identity of the sums and unsigned maxima below.  The moves run on the
+// VALU: no LDS permute and no per-step lane masks held in scalar registers.
+constexpr int kDppRowShr1 = 0x111, kDppRowShr2 = 0x112, kDppRowShr4 = 0x114, kDppRowShr8 = 0x118;
+constexpr int kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143;
+
+template <int CTRL, int ROWS, typename V>
+__device__ __forceinline__ V dpp_or_zero(V v)
+{
+    if constexpr (sizeof(V) == 8) {
+        const uint64_t u = (uint64_t)v;
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, ROWS, 0xF, false);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, ROWS, 0xF, false);
+        return (V)(((uint64_t)hi << 32) | lo);
+    } else {
+        return (V)(uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, CTRL, ROWS, 0xF, false);
+    }
+}
+
+template <typename V, typename Op>
+__device__ __forceinline__ V wave_incl_op(V v, Op op)
+{
+    v = op(v, dpp_or_zero<kDppRowShr1, 0xF>(v));
+    v = op(v, dpp_or_zero<kDppRowShr2, 0xF>(v));
+    v = op(v, dpp_or_zero<kDppRowShr4, 0xF>(v));
+    v = op(v, dpp_or_zero<kDppRowShr8, 0xF>(v));
+    v = op(v, dpp_or_zero<kDppRowBcast15, 0xA>(v));
+    v = op(v, dpp_or_zero<kDppRowBcast31, 0xC>(v));
+    return v;
+}
+
+// Value of lane 63 (a scalar register).
+template <typename V>
+__device__ __forceinline__ V wave_last(V v)
+{
+    if constexpr (sizeof(V) == 8) {
+        const uint64_t u = (uint64_t)v;
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, 63);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), 63);
+        return (V)(((uint64_t)hi << 32) | lo);
+    } else {
+        return (V)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, 63);
+    }
+}
+
+// Inclusive prefix sum over the wave's lanes (unsigned 32- or 64-bit; all lanes active).
 template <typename V>
 __device__ __forceinline__ V wave_incl_scan(V v)
 {
-    const uint32_t lane = lane_id();
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        V t = __shfl_up(v, d, 64);
-        if (lane >= (uint32_t)d) v += t;
-    }
-    return v;
+    return wave_incl_op(v, [](V x, V y) { return (V)(x + y); });
 }
 
 // Maximum of v over the lanes before this one (0 for lane 0).
 __device__ __forceinline__ uint32_t wave_excl_max(uint32_t v)
 {
-    const uint32_t lane = lane_id();
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t t = __shfl_up(v, d, 64);
-        if (lane >= (uint32_t)d) v = max(v, t);
-    }
-    const uint32_t p = __shfl_up(v, 1, 64);
-    return lane == 0 ? 0u : p;
+    const uint32_t m = wave_incl_op(v, [](uint32_t x, uint32_t y) { return max(x, y); });
+    const uint32_t p = __shfl_up(m, 1, 64);
+    return lane_id() == 0 ? 0u : p;
 }
 
+// Sum over the wave, the same in every lane.
 template <typename V>
 __device__ __forceinline__ V wave_sum(V v)
 {
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
-    return v;
+    return wave_last(wave_incl_scan<V>(v));
 }
 
 // XCD-aware remap: the dispatcher deals consecutive workgroup IDs round-robin
