@@ -37,6 +37,26 @@ __device__ __forceinline__ T* as_global(T* p)
     return (T*)(G*)(uintptr_t)p;
 }
 
+// The argument block of a kernel whose only parameter is a T, read again from
+// the kernel-argument segment at each use (scalar loads through the constant
+// cache).  A long loop that keeps a dozen 64-bit argument pointers live
+// otherwise holds them in scalar registers; past the scalar register budget
+// they spill to vector-register lanes and every use costs two lane reads on
+// the vector ALU.  The segment pointer comes from the dispatch (the only
+// parameter sits at its offset 0), never from the parameter's address, which
+// would make the compiler copy the block to private memory; the empty asm
+// makes each use's address opaque, so the loads stay where they are used.
+template <typename T>
+struct ArgRef {
+    using P = __attribute__((address_space(4))) const T*;
+    __device__ __forceinline__ P operator->() const
+    {
+        P q = (P)__builtin_amdgcn_kernarg_segment_ptr();
+        asm volatile("" : "+s"(q));
+        return q;
+    }
+};
+
 __device__ __forceinline__ float vfloat(float f) { return f; }
 __device__ __forceinline__ float vfloat(uint8_t v) { return (float)v; }
 

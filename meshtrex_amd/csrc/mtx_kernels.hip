@@ -3070,37 +3070,39 @@ __global__ __launch_bounds__(64 * TriMeshCfg<B>::WPG) void k_tri_meshlets(Extrac
     uint16_t* s_mk = s_mk_all[KIN ? 0 : w];
     uint32_t* s_tl = s_tl_all[w];
     const uint32_t na = active_items(a);
-    const Entry* __restrict__ table = static_cast<const Entry*>(a.corner_table);
-    const uint32_t* __restrict__ vb32 = reinterpret_cast<const uint32_t*>(a.blk_vbase);
-    const uint32_t* __restrict__ mb32 = reinterpret_cast<const uint32_t*>(a.blk_mbase);
-    const uint32_t* __restrict__ sb32 = reinterpret_cast<const uint32_t*>(a.blk_sbase);
+    // argument pointers re-read where used (the block loop's scalar registers
+    // are taken by the two blocks in flight)
+    const ArgRef<ExtractArgs> AR;
+    auto table_ = [&]() { return static_cast<const Entry*>(AR->corner_table); };
+    auto vb32_ = [&]() { return reinterpret_cast<const uint32_t*>(AR->blk_vbase); };
     const uint32_t gx = a.grid.gx, gy = a.grid.gy, gz = a.grid.gz;
     auto stage1 = [&](TB& P, uint32_t i) {
-        const uint4 cnt = *reinterpret_cast<const uint4*>(a.blk_counts + 4 * (uint64_t)i);
+        const uint4 cnt = *reinterpret_cast<const uint4*>(AR->blk_counts + 4 * (uint64_t)i);
         P.bnv = cnt.x;
         P.bnt = cnt.y;
         P.bnm = cnt.z;
         P.bns = cnt.w;
-        P.vb = vb32[2 * (uint64_t)i];
-        P.tb = a.blk_tbase[i];
-        P.mb = mb32[2 * (uint64_t)i];
-        P.msb = sb32[2 * (uint64_t)i];
-        P.id = a.ids[i];
+        P.vb = vb32_()[2 * (uint64_t)i];
+        P.tb = AR->blk_tbase[i];
+        P.mb = reinterpret_cast<const uint32_t*>(AR->blk_mbase)[2 * (uint64_t)i];
+        P.msb = reinterpret_cast<const uint32_t*>(AR->blk_sbase)[2 * (uint64_t)i];
+        P.id = AR->ids[i];
         if constexpr (CPL == 8 && NW == 16)
-            P.sbits = reinterpret_cast<const uint8_t*>(a.starts)[(uint64_t)i * 64 + lane];
+            P.sbits = reinterpret_cast<const uint8_t*>(AR->starts)[(uint64_t)i * 64 + lane];
         else
-            P.sbits = lane_start_bits<B, CPL>(a.starts + (uint64_t)i * NW, lane);
-        P.lc.load(table, a.cases, i, lane, true);
+            P.sbits = lane_start_bits<B, CPL>(AR->starts + (uint64_t)i * NW, lane);
+        P.lc.load(table_(), AR->cases, i, lane, true);
     };
     auto stage2 = [&](TB& P) {
         const uint32_t bx = P.id % gx, by = (P.id / gx) % gy, bz = P.id / (gx * gy);
         const uint32_t nx = bx + (lane & 1u), ny = by + ((lane >> 1) & 1u), nz = bz + ((lane >> 2) & 1u);
         P.r = -1;
         if (lane != 0 && lane < 7 && nx < gx && ny < gy && nz < gz)
-            P.r = (int32_t)a.blk_map[nx + gx * (ny + (uint64_t)gy * nz)];
+            P.r = (int32_t)AR->blk_map[nx + gx * (ny + (uint64_t)gy * nz)];
     };
     auto stage3 = [&](TB& P) {
-        P.nbase = (lane < 8 && P.r >= 0) ? vb32[2 * (uint64_t)P.r] : 0u;
+        P.nbase = (lane < 8 && P.r >= 0) ? vb32_()[2 * (uint64_t)P.r] : 0u;
+        const Entry* __restrict__ table = table_();
 #pragma unroll
         for (int k = 0; k < TB::NFW; ++k) P.fentw[k] = 0xFFFFFFFFu;
 #pragma unroll
@@ -3128,9 +3130,9 @@ __global__ __launch_bounds__(64 * TriMeshCfg<B>::WPG) void k_tri_meshlets(Extrac
         const uint64_t tb = cur.tb;
         const uint32_t mb = cur.mb, msb = cur.msb;
         bool work = bnt != 0;
-        if (work && (3 * (tb + bnt) > a.icap || (uint64_t)msb + bns > a.mvcap || (uint64_t)mb + bnm > a.mcap ||
+        if (work && (3 * (tb + bnt) > AR->icap || (uint64_t)msb + bns > AR->mvcap || (uint64_t)mb + bnm > AR->mcap ||
                      bnm > (uint32_t)C::MAXM)) {
-            if (lane == 0) atomicOr(reinterpret_cast<unsigned long long*>(&a.totals[2]), 1ull);
+            if (lane == 0) atomicOr(reinterpret_cast<unsigned long long*>(&AR->totals[2]), 1ull);
             work = false;
         }
         uint32_t vlim = 0, S0 = 0, slot = 0, nt = 0, npk = 0, cnt_v = 0;
@@ -3189,7 +3191,7 @@ __global__ __launch_bounds__(64 * TriMeshCfg<B>::WPG) void k_tri_meshlets(Extrac
             const uint32_t nm = wave_sum<uint32_t>((uint32_t)__popcll(sbits));
             if (ns != bns || nm != bnm || tt != bnt) {
                 // the cut and the counts disagree with this block's data
-                if (lane == 0) atomicOr(reinterpret_cast<unsigned long long*>(&a.totals[2]), 4ull);
+                if (lane == 0) atomicOr(reinterpret_cast<unsigned long long*>(&AR->totals[2]), 4ull);
             } else {
 #pragma unroll
                 for (int j = 0; j < CPL; ++j) {
@@ -3253,18 +3255,18 @@ __global__ __launch_bounds__(64 * TriMeshCfg<B>::WPG) void k_tri_meshlets(Extrac
                             mv[e3] = msb + (mc & 0xFFFFFu) + __popc((mc >> 20) & ((1u << u1) - 1u));
                         }
 #pragma unroll
-                        for (uint32_t e3 = 0; e3 < 3; ++e3) a.mvmap[mv[e3]] = gi[e3];
+                        for (uint32_t e3 = 0; e3 < 3; ++e3) AR->mvmap[mv[e3]] = gi[e3];
                         // a triangle's three entries as one 12-byte store per
                         // lane (the wave writes 768 contiguous bytes; three
                         // strided 4-byte stores rewrote each line three times)
-                        store3_nt(a.indices + 3 * (tb + c0 + q), gi[0], gi[1], gi[2]);
-                        store3_nt(a.mindices + 3 * (tb + c0 + q), mv[0], mv[1], mv[2]);
+                        store3_nt(AR->indices + 3 * (tb + c0 + q), gi[0], gi[1], gi[2]);
+                        store3_nt(AR->mindices + 3 * (tb + c0 + q), mv[0], mv[1], mv[2]);
                     }
                     wave_lds_sync();
                 }
                 // an active edge whose owner block is not in the active list: the
                 // list does not belong to this volume/isovalue
-                if (bad & 0xF0u) atomicOr(reinterpret_cast<unsigned long long*>(&a.totals[2]), 2ull);
+                if (bad & 0xF0u) atomicOr(reinterpret_cast<unsigned long long*>(&AR->totals[2]), 2ull);
                 // descriptors: each meshlet's first meshlet vertex and first
                 // triangle within the block (16 bits each), staged at its index
                 uint32_t nmt;
@@ -3281,7 +3283,7 @@ __global__ __launch_bounds__(64 * TriMeshCfg<B>::WPG) void k_tri_meshlets(Extrac
                 for (uint32_t k = lane; k < nm; k += 64u) {
                     const uint32_t A = s_tl[k];
                     const uint32_t Bn = k + 1 < nm ? s_tl[k + 1] : (bns | (bnt << 16));
-                    a.meshlets[(uint64_t)mb + k] = make_uint4(msb + (A & 0xFFFFu), 3u * (tb_rel + (A >> 16)),
+                    AR->meshlets[(uint64_t)mb + k] = make_uint4(msb + (A & 0xFFFFu), 3u * (tb_rel + (A >> 16)),
                                                               (Bn & 0xFFFFu) - (A & 0xFFFFu), (Bn >> 16) - (A >> 16));
                 }
             }
