@@ -1571,6 +1571,82 @@ __device__ __forceinline__ void meshlet_cut(Cell&& cell, const uint8_t* s_ntri, 
     }
 }
 
+// The same greedy cut over a list of the block's occupied cells (cells with
+// triangles; an empty cell adds nothing to a meshlet, so no cut falls on
+// one): the lanes write their cells' entries in cell order to s_ent (NC words
+// of LDS, this wave's), then each step scans 64 entries per pass instead of
+// every cell of the block.  Entry: edge bits (12) | neighbour bits << 12 |
+// triangles << 18 | cell << 21.
+template <int B, int CPL, typename Cell>
+__device__ __forceinline__ void meshlet_cut_list(Cell&& cell, const uint8_t* s_ntri, const uint16_t* s_emu,
+                                                 const uint16_t* s_newtab, const uint8_t* s_cls, uint32_t* s_ent,
+                                                 uint32_t& nm, uint32_t& ns, uint64_t& sbits)
+{
+    constexpr uint32_t NC = B * B * B;
+    static_assert(NC <= 2048 && CPL <= 32, "the cell index shares the entry word");
+    constexpr uint32_t KFAR = (uint32_t)(B * B + B);
+    const uint32_t lane = lane_id();
+    nm = 0;
+    ns = 0;
+    sbits = 0;
+    uint32_t info[CPL];
+    uint32_t occ = 0;
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) {
+        uint32_t cs, nb;
+        cell(j, cs, nb);
+        const uint32_t p = lane * CPL + j < NC ? (uint32_t)s_ntri[cs] : 0u;
+        info[j] = p ? (uint32_t)s_emu[cs] | (nb << 12) | (p << 18) | ((lane * CPL + j) << 21) : 0u;
+        occ |= (p ? 1u : 0u) << j;
+    }
+    uint32_t T;
+    uint32_t q = wave_excl_scan<uint32_t>((uint32_t)__popc(occ), T);
+    if (T == 0) return;
+#pragma unroll
+    for (int j = 0; j < CPL; ++j)
+        if ((occ >> j) & 1u) s_ent[q++] = info[j];
+    wave_lds_sync();
+    if (lane == 0) sbits = 1;
+    uint32_t S = 0, e0 = 0;   // the meshlet's first cell and its first entry
+    for (;;) {
+        uint32_t cv = 0, cp = 0;   // vertices and triangles of the meshlet's earlier passes
+        bool cut = false;
+        for (uint32_t q0 = e0 & ~63u; q0 < T; q0 += 64u) {
+            const uint32_t e = q0 + lane;
+            uint32_t v = 0, p = 0, c = 0;
+            if (e >= e0 && e < T) {
+                const uint32_t x = s_ent[e];
+                c = x >> 21;
+                p = (x >> 18) & 7u;
+                const uint32_t k = c - S;
+                const uint32_t cls = k >= KFAR ? kBackClasses - 1u : (uint32_t)s_cls[k];
+                v = __popc(x & 0xFFFu & s_newtab[((x >> 12) & 63u) * kBackClasses + cls]);
+            }
+            const uint32_t rv = cv + wave_incl_scan<uint32_t>(v), rp = cp + wave_incl_scan<uint32_t>(p);
+            const uint64_t bal = __ballot(rv > kMeshletVerts || rp > kMeshletTris);
+            if (bal) {
+                // the first entry past a limit starts the next meshlet
+                const int src = __builtin_ctzll(bal);
+                const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)c, src);
+                ns += (uint32_t)__builtin_amdgcn_readlane((int)(rv - v), src);
+                if (lane == b / CPL) sbits |= 1ull << (b % CPL);
+                S = b;
+                e0 = q0 + (uint32_t)src;
+                cut = true;
+                break;
+            }
+            cv = wave_last(rv);
+            cp = wave_last(rp);
+        }
+        ++nm;
+        if (!cut) {
+            ns += cv;
+            break;
+        }
+    }
+    wave_lds_sync();
+}
+
 // Words of a block's meshlet start bits (one bit per cell).
 template <int B> struct StartBits {
     static constexpr uint32_t NW = (B * B * B + 31) / 32;
@@ -1712,11 +1788,13 @@ __global__ __launch_bounds__(256) void k_extract_count_b8(ExtractArgs a)
     __shared__ uint16_t s_emu[CUT ? 256 : 1];
     __shared__ uint16_t s_newtab[CUT ? 64 * kBackClasses : 1];
     __shared__ uint8_t s_cls[CUT ? B * B + B : 1];
+    __shared__ uint32_t s_ent_all[CUT ? 4 : 1][CUT ? NC : 1];   // each wave's list of occupied cells
     for (uint32_t c = threadIdx.x; c < 256u; c += blockDim.x) s_ntri[c] = (uint8_t)mtx_word_ntri(mtx_tri_words[c]);
     if constexpr (CUT) meshlet_tables<B>(s_emu, s_newtab, s_cls);
     __syncthreads();
     const uint32_t lane = lane_id();
     uint64_t* s_bits = s_bits_all[threadIdx.x >> 6];
+    uint32_t* s_ent = s_ent_all[CUT ? threadIdx.x >> 6 : 0];
     // tile coordinates of the voxels this lane loads, j = lane + 64 k:
     // (j % 9) | (j / 9 % 9) << 4 | (j / 81) << 8, and bit 12 past the tile
     uint32_t tcoord[LPL];
@@ -1818,12 +1896,12 @@ __global__ __launch_bounds__(256) void k_extract_count_b8(ExtractArgs a)
             const uint32_t vy = Y - 1u > oy ? min(Y - 1u - oy, (uint32_t)B) : 0u;
             const uint32_t vz = Z - 1u > oz ? min(Z - 1u - oz, (uint32_t)B) : 0u;
             uint64_t sb;
-            meshlet_cut<B, 8>(
+            meshlet_cut_list<B, 8>(
                 [&](int j, uint32_t& cs, uint32_t& nb) {
                     cs = ((j < 4 ? cases0 : cases1) >> (8 * (j & 3))) & 0xFFu;
                     nb = cell_nbrs((uint32_t)j, ly, lz, vx, vy, vz);
                 },
-                s_ntri, s_emu, s_newtab, s_cls, nm, ns, sb);
+                s_ntri, s_emu, s_newtab, s_cls, s_ent, nm, ns, sb);
             reinterpret_cast<uint8_t*>(a.starts)[(uint64_t)i * 64 + lane] = (uint8_t)sb;
         }
         if (lane == 0) *reinterpret_cast<uint4*>(a.blk_counts + 4 * (uint64_t)i) = make_uint4(tv, tt, nm, ns);
