@@ -2156,14 +2156,35 @@ __global__ __launch_bounds__((64 * VertCfg<B, NORMALS, T>::WPG)) void k_extract_
     const Entry* __restrict__ table = static_cast<const Entry*>(a.corner_table);
     constexpr int LPL = (NTILE + 63) / 64;
     const WaveItems it = cost_items(a, na, WPG, a.blk_vbase, a.totals[0]);
-    for (uint32_t i = it.first; i < it.end; i += it.stride) {
-        const uint32_t bnv = a.blk_counts[4 * (uint64_t)i];
-        if (bnv == 0) continue;
-        const uint64_t vb = a.blk_vbase[i];
-        if (vb + bnv > a.vcap) {
+    // the next block's count, bases, ID and corner entries load while this
+    // block's tile loads and its vertices are computed: one memory round trip
+    // per block on the critical path (the tile) instead of three
+    struct Pre {
+        uint32_t bnv, id;
+        uint64_t vb;
+        LaneCorners<B> lc;
+    };
+    auto fetch = [&](Pre& P, uint32_t i) {
+        P.bnv = a.blk_counts[4 * (uint64_t)i];
+        P.vb = a.blk_vbase[i];
+        P.id = a.ids[i];
+        P.lc.load(table, nullptr, i, lane, false);
+    };
+    uint32_t i = it.first;
+    Pre cur;
+    if (i < it.end) fetch(cur, i);
+    while (i < it.end) {
+        const uint32_t inx = i + it.stride;
+        Pre nxt = cur;
+        if (inx < it.end) fetch(nxt, inx);
+        const uint32_t bnv = cur.bnv;
+        const uint64_t vb = cur.vb;
+        bool work = bnv != 0;
+        if (work && vb + bnv > a.vcap) {
             if (lane == 0) atomicOr(reinterpret_cast<unsigned long long*>(&a.totals[2]), 1ull);
-            continue;
+            work = false;
         }
+        if (work) {
         uint32_t unit;
         const ItemVol V = item_vol<BATCH>(a, i, unit);
         const T* __restrict__ vol = static_cast<const T*>(V.data);
@@ -2171,12 +2192,11 @@ __global__ __launch_bounds__((64 * VertCfg<B, NORMALS, T>::WPG)) void k_extract_
         const float iso = V.iso;
         const float inv2x = 2.0f / (float)X, inv2y = 2.0f / (float)Y, inv2z = 2.0f / (float)Z;
         const bool off32 = (uint64_t)X * Y * E < 0x80000000ull;
-        const uint32_t id = a.ids[i];
+        const uint32_t id = cur.id;
         const uint32_t bx = id % V.gx, by = (id / V.gx) % V.gy;
         const uint32_t bz = id / (V.gx * V.gy);
         const uint32_t ox = bx * B, oy = by * B, oz = bz * B;
-        LaneCorners<B> lc;
-        lc.load(table, nullptr, i, lane, false);
+        const LaneCorners<B>& lc = cur.lc;
         // LDS voxel (tx, ty, tz) of the tile from (ox - A, oy - A, oz - A) sits at tx + RP * (ty + E * tz)
         const bool direct = off32 && ox >= (uint32_t)A && oy >= (uint32_t)A && oz >= (uint32_t)A &&
                             ox - A + RP <= X && oy + B + A < Y && oz + B + A < Z;
@@ -2340,6 +2360,9 @@ __global__ __launch_bounds__((64 * VertCfg<B, NORMALS, T>::WPG)) void k_extract_
             }
             wave_lds_sync();
         }
+        }
+        cur = nxt;
+        i = inx;
     }
 }
 
