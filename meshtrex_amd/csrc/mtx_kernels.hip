@@ -3091,7 +3091,10 @@ template <int B> struct TriMeshCfg {
     static constexpr int TL = MAXM > LIST ? MAXM : LIST;
     using TDec = typename TriCfg<B>::TDec;
     static constexpr int DB = TriCfg<B>::DB;
-    static constexpr int WAVE_LDS = NF * 5 + NC * 4 + (KIN ? 0 : NC * 2) + TL * 4;
+    // B = 8: the block's occupied cells are listed once and every per-cell
+    // step after the corner tables runs over the list, 64 cells per pass
+    static constexpr bool LISTED = B == 8;
+    static constexpr int WAVE_LDS = NF * 5 + NC * 4 + (KIN ? 0 : NC * 2) + TL * 4 + (LISTED ? NC * 4 : 0);
     static constexpr int WG_LDS = 256 + 1280 * 8 + 4 * NFACE + 512 + 64 * (int)kBackClasses * 2 + (B * B + B) +
                                   64 * 12 * 8;
     static constexpr int WPG = WAVE_LDS * 4 + WG_LDS <= 65536 ? 4 : (WAVE_LDS * 2 + WG_LDS <= 65536 ? 2 : 1);
@@ -3112,7 +3115,7 @@ __global__ __launch_bounds__(64 * TriMeshCfg<B>::WPG) void k_tri_meshlets(Extrac
     constexpr int WPG = C::WPG, F = C::F, NF = C::NF, NC = C::NC, CB = C::CB, DB = C::DB;
     constexpr int CPL = EB<B>::CPL;
     constexpr uint32_t LIST = C::LIST, NW = StartBits<B>::NW;
-    constexpr bool KIN = C::KIN;
+    constexpr bool KIN = C::KIN, LISTED = C::LISTED;
     constexpr bool PACKN = CPL * 3 <= 32;
     constexpr int NFACE = C::NFACE, NPL = TriBlock<B>::NPL;
     using TDec = typename C::TDec;
@@ -3134,9 +3137,12 @@ __global__ __launch_bounds__(64 * TriMeshCfg<B>::WPG) void k_tri_meshlets(Extrac
     // (then the descriptors' inputs)
     __shared__ uint32_t s_gidx_all[WPG][NF];
     __shared__ uint8_t s_gfl_all[PK ? 1 : WPG][PK ? 1 : NF];
-    __shared__ uint32_t s_mc_all[WPG][NC];
+    __shared__ __align__(16) uint32_t s_mc_all[WPG][NC];
     __shared__ uint16_t s_mk_all[KIN ? 1 : WPG][KIN ? 1 : NC];
     __shared__ uint32_t s_tl_all[WPG][C::TL];
+    // B = 8: the occupied cells in cell order: cell | case << 9 | distance back
+    // to its meshlet's start << 17 | triangles << 26 | starts a meshlet << 29
+    __shared__ uint32_t s_ls_all[LISTED ? WPG : 1][LISTED ? NC : 1];
     for (uint32_t c = threadIdx.x; c < 256u; c += blockDim.x) s_ntri[c] = (uint8_t)mtx_word_ntri(mtx_tri_words[c]);
     for (uint32_t x = threadIdx.x; x < 256u * 5u; x += blockDim.x) {
         const uint64_t wd = mtx_tri_words[x / 5u];
@@ -3170,6 +3176,7 @@ __global__ __launch_bounds__(64 * TriMeshCfg<B>::WPG) void k_tri_meshlets(Extrac
     uint32_t* s_mc = s_mc_all[w];
     uint16_t* s_mk = s_mk_all[KIN ? 0 : w];
     uint32_t* s_tl = s_tl_all[w];
+    uint32_t* s_ls = s_ls_all[LISTED ? w : 0];
     const uint32_t na = active_items(a);
     // argument pointers re-read where used (the block loop's scalar registers
     // are taken by the two blocks in flight)
@@ -3236,7 +3243,7 @@ __global__ __launch_bounds__(64 * TriMeshCfg<B>::WPG) void k_tri_meshlets(Extrac
             if (lane == 0) atomicOr(reinterpret_cast<unsigned long long*>(&AR->totals[2]), 1ull);
             work = false;
         }
-        uint32_t vlim = 0, S0 = 0, slot = 0, nt = 0, npk = 0, cnt_v = 0;
+        uint32_t vlim = 0, S0 = 0, slot = 0, nt = 0, npk = 0, cnt_v = 0, T = 0;
         const uint64_t sbits = cur.sbits;
         if (work) {
             // own corners: vertex index = block base + prefix from the count pass
@@ -3266,6 +3273,29 @@ __global__ __launch_bounds__(64 * TriMeshCfg<B>::WPG) void k_tri_meshlets(Extrac
             block_cells(a, cur.id, vlim);
             S0 = wave_excl_max(sbits ? lane * CPL + 63u - (uint32_t)__builtin_clzll(sbits) : 0u);
             uint32_t S = S0;
+            if constexpr (LISTED) {
+                static_assert(CPL == 8 && NC == 512, "the list entry holds a 9-bit cell index");
+                // every cell's meshlet-vertex word starts at 0 (slot 0, no
+                // added edges); the slot pass rewrites the occupied cells
+                *reinterpret_cast<uint4*>(s_mc + lane * 8u) = make_uint4(0u, 0u, 0u, 0u);
+                *reinterpret_cast<uint4*>(s_mc + lane * 8u + 4u) = make_uint4(0u, 0u, 0u, 0u);
+                uint32_t ent[CPL], occ = 0;
+#pragma unroll
+                for (int j = 0; j < CPL; ++j) {
+                    const uint32_t c = lane * CPL + j;
+                    const uint32_t st = (uint32_t)(sbits >> j) & 1u;
+                    if (st) S = c;
+                    const uint32_t cs = cur.lc.cs(j);
+                    const uint32_t n = s_ntri[cs];
+                    ent[j] = c | (cs << 9) | ((c - S) << 17) | (n << 26) | (st << 29);
+                    occ |= (n ? 1u : 0u) << j;
+                    nt += n;
+                }
+                uint32_t q = wave_excl_scan<uint32_t>((uint32_t)__popc(occ), T);
+#pragma unroll
+                for (int j = 0; j < CPL; ++j)
+                    if ((occ >> j) & 1u) s_ls[q++] = ent[j];
+            } else {
 #pragma unroll
             for (int j = 0; j < CPL; ++j) {
                 const uint32_t c = lane * CPL + j;
@@ -3283,29 +3313,71 @@ __global__ __launch_bounds__(64 * TriMeshCfg<B>::WPG) void k_tri_meshlets(Extrac
                 nt += n;
                 if (PACKN) npk |= n << (3 * j);
             }
+            }
         }
         if (more) stage2(nxt);
         if (work) {
-            uint32_t ns, tt;
-            slot = wave_excl_scan<uint32_t>(cnt_v, ns);
-            const uint32_t et = wave_excl_scan<uint32_t>(nt, tt);
+            uint32_t ns, tt, et = 0;
+            if constexpr (LISTED) {
+                wave_lds_sync();
+                tt = wave_sum<uint32_t>(nt);
+                // meshlet-vertex slots: the occupied cells' added edges, in cell order
+                uint32_t run = 0;
+                for (uint32_t e0 = 0; e0 < T; e0 += 64u) {
+                    const uint32_t e = e0 + lane;
+                    const uint32_t x = e < T ? s_ls[e] : 0u;
+                    const uint32_t c = x & 511u;
+                    uint32_t m = 0;
+                    if (e < T)
+                        m = added_edges<B>(s_emu[(x >> 9) & 255u], nbrs_of_cell<B>(c, vlim), (x >> 17) & 511u,
+                                           s_newtab, s_cls);
+                    uint32_t tot;
+                    const uint32_t sl = run + wave_excl_scan<uint32_t>((uint32_t)__popc(m), tot);
+                    if (e < T) s_mc[c] = sl | (m << 20);
+                    run += tot;
+                }
+                ns = run;
+            } else {
+                slot = wave_excl_scan<uint32_t>(cnt_v, ns);
+                et = wave_excl_scan<uint32_t>(nt, tt);
+            }
             const uint32_t nm = wave_sum<uint32_t>((uint32_t)__popcll(sbits));
             if (ns != bns || nm != bnm || tt != bnt) {
                 // the cut and the counts disagree with this block's data
                 if (lane == 0) atomicOr(reinterpret_cast<unsigned long long*>(&AR->totals[2]), 4ull);
             } else {
+                if constexpr (!LISTED) {
 #pragma unroll
-                for (int j = 0; j < CPL; ++j) {
-                    const uint32_t c = lane * CPL + j;
-                    if (c < (uint32_t)NC) {
-                        const uint32_t m = s_mc[c];
-                        s_mc[c] = m | slot;
-                        slot += __popc(m >> 20);
+                    for (int j = 0; j < CPL; ++j) {
+                        const uint32_t c = lane * CPL + j;
+                        if (c < (uint32_t)NC) {
+                            const uint32_t m = s_mc[c];
+                            s_mc[c] = m | slot;
+                            slot += __popc(m >> 20);
+                        }
                     }
                 }
                 uint32_t bad = 0;
                 for (uint32_t c0 = 0; c0 < tt; c0 += LIST) {
                     const uint32_t n = min(LIST, tt - c0);
+                    if constexpr (LISTED) {
+                        // list entries in output order: cell by cell, the case's triangles k = 0..4
+                        wave_lds_sync();
+                        uint32_t trun = 0;
+                        for (uint32_t e0 = 0; e0 < T; e0 += 64u) {
+                            const uint32_t e = e0 + lane;
+                            const uint32_t x = e < T ? s_ls[e] : 0u;
+                            const uint32_t nn = (x >> 26) & 7u;
+                            uint32_t tot;
+                            const uint32_t r = trun + wave_excl_scan<uint32_t>(nn, tot) - c0;
+                            const uint32_t ent =
+                                (x & 511u) | ((((x >> 9) & 255u) * 5u) << CB) | (((x >> 17) & 511u) << (CB + 11));
+#pragma unroll
+                            for (uint32_t k = 0; k < 5u; ++k)
+                                if (k < nn && r + k < LIST) s_tl[r + k] = ent + (k << CB);
+                            trun += tot;
+                        }
+                    } else {
                     // list entries in output order: cell by cell, the case's triangles k = 0..4
                     uint32_t run = et - c0;
                     uint32_t S = S0;
@@ -3328,6 +3400,7 @@ __global__ __launch_bounds__(64 * TriMeshCfg<B>::WPG) void k_tri_meshlets(Extrac
                                 if (k < nn && run + k < LIST) s_tl[run + k] = ent + (k << CB);
                             run += nn;
                         }
+                    }
                     }
                     wave_lds_sync();
                     // one triangle per lane, its three vertices' lookups side by side
@@ -3370,6 +3443,26 @@ __global__ __launch_bounds__(64 * TriMeshCfg<B>::WPG) void k_tri_meshlets(Extrac
                 if (bad & 0xF0u) atomicOr(reinterpret_cast<unsigned long long*>(&AR->totals[2]), 2ull);
                 // descriptors: each meshlet's first meshlet vertex and first
                 // triangle within the block (16 bits each), staged at its index
+                if constexpr (LISTED) {
+                    // cell 0 starts the first meshlet (offsets 0 and 0 when it is
+                    // empty); every other start is an occupied cell
+                    uint32_t srun = 0, trun = 0;
+                    if ((s_ls[0] & 511u) != 0u) {
+                        if (lane == 0) s_tl[0] = 0u;
+                        srun = 1;
+                    }
+                    for (uint32_t e0 = 0; e0 < T; e0 += 64u) {
+                        const uint32_t e = e0 + lane;
+                        const uint32_t x = e < T ? s_ls[e] : 0u;
+                        const uint32_t sf = (x >> 29) & 1u;
+                        uint32_t tot, stot;
+                        const uint32_t tp = trun + wave_excl_scan<uint32_t>((x >> 26) & 7u, tot);
+                        const uint32_t mi = srun + wave_excl_scan<uint32_t>(sf, stot);
+                        if (sf && mi < (uint32_t)C::TL) s_tl[mi] = (s_mc[x & 511u] & 0xFFFFu) | (tp << 16);
+                        trun += tot;
+                        srun += stot;
+                    }
+                } else {
                 uint32_t nmt;
                 uint32_t m = wave_excl_scan<uint32_t>((uint32_t)__popcll(sbits), nmt);
                 uint32_t tp = et;
@@ -3378,6 +3471,7 @@ __global__ __launch_bounds__(64 * TriMeshCfg<B>::WPG) void k_tri_meshlets(Extrac
                     const uint32_t c = lane * CPL + j;
                     if ((sbits >> j) & 1ull) s_tl[m++] = (s_mc[c] & 0xFFFFu) | (tp << 16);
                     tp += PACKN ? (npk >> (3 * j)) & 7u : (uint32_t)s_ntri[cur.lc.cs(j)];
+                }
                 }
                 wave_lds_sync();
                 const uint32_t tb_rel = (uint32_t)tb;
