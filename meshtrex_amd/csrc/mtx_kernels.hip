@@ -1250,6 +1250,28 @@ struct ItemVol {
     const uint32_t* map;
 };
 
+// The volume of a batch unit (`unit` loaded ahead by the caller) or the launch's own.
+template <bool BATCH>
+__device__ __forceinline__ ItemVol item_vol_of(const ExtractArgs& a, uint32_t unit)
+{
+    ItemVol r;
+    if constexpr (BATCH) {
+        const BatchUnit& U = a.units[unit];
+        r.data = as_global(U.data);
+        r.X = U.X; r.Y = U.Y; r.Z = U.Z;
+        r.gx = U.gx; r.gy = U.gy; r.gz = U.gz;
+        r.iso = U.act.iso;
+        r.map = as_global(U.map);
+    } else {
+        r.data = a.vol.data;
+        r.X = a.vol.X; r.Y = a.vol.Y; r.Z = a.vol.Z;
+        r.gx = a.grid.gx; r.gy = a.grid.gy; r.gz = a.grid.gz;
+        r.iso = a.iso;
+        r.map = a.blk_map;
+    }
+    return r;
+}
+
 template <bool BATCH>
 __device__ __forceinline__ ItemVol item_vol(const ExtractArgs& a, uint32_t i, uint32_t& unit)
 {
@@ -2156,11 +2178,11 @@ __global__ __launch_bounds__((64 * VertCfg<B, NORMALS, T>::WPG)) void k_extract_
     const Entry* __restrict__ table = static_cast<const Entry*>(a.corner_table);
     constexpr int LPL = (NTILE + 63) / 64;
     const WaveItems it = cost_items(a, na, WPG, a.blk_vbase, a.totals[0]);
-    // the next block's count, bases, ID and corner entries load while this
-    // block's tile loads and its vertices are computed: one memory round trip
-    // per block on the critical path (the tile) instead of three
+    // the next block's count, bases, ID, batch unit and corner entries load
+    // while this block's tile loads and its vertices are computed: one memory
+    // round trip per block on the critical path (the tile) instead of three
     struct Pre {
-        uint32_t bnv, id;
+        uint32_t bnv, id, unit;
         uint64_t vb;
         LaneCorners<B> lc;
     };
@@ -2168,6 +2190,7 @@ __global__ __launch_bounds__((64 * VertCfg<B, NORMALS, T>::WPG)) void k_extract_
         P.bnv = a.blk_counts[4 * (uint64_t)i];
         P.vb = a.blk_vbase[i];
         P.id = a.ids[i];
+        P.unit = BATCH ? a.item_unit[i] : 0u;
         P.lc.load(table, nullptr, i, lane, false);
     };
     uint32_t i = it.first;
@@ -2185,8 +2208,7 @@ __global__ __launch_bounds__((64 * VertCfg<B, NORMALS, T>::WPG)) void k_extract_
             work = false;
         }
         if (work) {
-        uint32_t unit;
-        const ItemVol V = item_vol<BATCH>(a, i, unit);
+        const ItemVol V = item_vol_of<BATCH>(a, __builtin_amdgcn_readfirstlane(cur.unit));
         const T* __restrict__ vol = static_cast<const T*>(V.data);
         const uint32_t X = V.X, Y = V.Y, Z = V.Z;
         const float iso = V.iso;
