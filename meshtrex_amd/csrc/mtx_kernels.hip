@@ -2103,9 +2103,13 @@ template <int B, bool NORMALS, typename T> struct VertCfg {
     static constexpr int ND = (NSEG + 63) / 64;
     static constexpr int TILE = ND * 64 * 4;    // LDS voxels incl. the spare lanes' segments
     static constexpr unsigned SEGB = 4 * sizeof(T);   // bytes per direct load and lane
-    static constexpr int LIST = 512;            // vertices listed at a time
-    // + the 64 vertices of a round (32 bytes each), staged for coalesced stores
-    static constexpr int WAVE_LDS = TILE * (int)sizeof(T) + LIST * 2 + 16 + 64 * 32;
+    // vertices listed at a time, and half a round of vertices (32 bytes
+    // each) staged for coalesced stores: with the f32 tile that is 30 KB per
+    // workgroup, so 5 workgroups fit a CU (the kernel is held to the 102
+    // VGPRs of 5 waves per SIMD); 512 entries and whole rounds took 36.9 KB,
+    // 4 waves per SIMD, and 23 us more at 1024^3
+    static constexpr int LIST = 256;
+    static constexpr int WAVE_LDS = TILE * (int)sizeof(T) + LIST * 2 + 16 + 32 * 32;
     static constexpr int WPG = WAVE_LDS * 4 <= 65536 ? 4 : (WAVE_LDS * 2 <= 65536 ? 2 : 1);
 };
 
@@ -2159,7 +2163,8 @@ struct LaneCorners {
 // the block's vertices (corner, axis) in prefix order and let the 64 lanes
 // take consecutive entries: balanced work and coalesced writes.
 template <int B, typename T, bool NORMALS, bool BATCH>
-__global__ __launch_bounds__((64 * VertCfg<B, NORMALS, T>::WPG)) void k_extract_vertices(ExtractArgs a)
+__global__ __launch_bounds__((64 * VertCfg<B, NORMALS, T>::WPG)) __attribute__((amdgpu_waves_per_eu(5, 8)))
+void k_extract_vertices(ExtractArgs a)
 {
     using C = VertCfg<B, NORMALS, T>;
     constexpr int A = C::A, E = C::E, NTILE = C::NTILE, WPG = C::WPG;
@@ -2169,7 +2174,7 @@ __global__ __launch_bounds__((64 * VertCfg<B, NORMALS, T>::WPG)) void k_extract_
     using Entry = typename EB<B>::Entry;
     __shared__ __align__(16) T s_val_all[WPG][C::TILE];
     __shared__ uint16_t s_vl_all[WPG][LIST];
-    __shared__ __align__(16) float4 s_vo_all[WPG][128];
+    __shared__ __align__(16) float4 s_vo_all[WPG][64];
     const uint32_t lane = lane_id(), w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     T* s_val = s_val_all[w];
     uint16_t* s_vl = s_vl_all[w];
@@ -2320,9 +2325,10 @@ __global__ __launch_bounds__((64 * VertCfg<B, NORMALS, T>::WPG)) void k_extract_
             wave_lds_sync();
             const uint32_t n = min(LIST, bnv - c0);
             // rounds of 64 vertices, every lane taking part: a lane computes
-            // vertex r0 + lane, the round's 2 KB are staged in LDS and written
-            // as two contiguous 1 KB runs (a 32-byte vertex per lane would
-            // take two stores 32 bytes apart, each line written half at a time)
+            // vertex r0 + lane; each half of the round (lanes 0-31, then
+            // 32-63) is staged in LDS and written as one contiguous 1 KB run
+            // (a 32-byte vertex per lane would take two stores 32 bytes apart,
+            // each line written half at a time)
             for (uint32_t r0 = 0; r0 < n; r0 += 64u) {
                 const uint32_t v = r0 + lane;
                 const bool live = v < n;
@@ -2356,15 +2362,24 @@ __global__ __launch_bounds__((64 * VertCfg<B, NORMALS, T>::WPG)) void k_extract_
                     const float r = rsqrtf(l2);
                     nrm = make_float4(m0 * r, m1 * r, m2 * r, 0.0f);
                 }
-                s_vo[2 * lane] = pos;
-                s_vo[2 * lane + 1] = nrm;
-                wave_lds_sync();
                 const uint32_t nh = 2 * min(64u, n - r0);   // 16-byte halves of this round
                 float4* dst = reinterpret_cast<float4*>(a.vertices) + 2 * (vb + c0 + r0);
-                const float4 h0 = s_vo[lane], h1 = s_vo[64 + lane];
                 // nontemporal: written once, read by the caller later
+                if (lane < 32u) {
+                    s_vo[2 * lane] = pos;
+                    s_vo[2 * lane + 1] = nrm;
+                }
+                wave_lds_sync();
+                const float4 h0 = s_vo[lane];
                 if (lane < nh)
                     __builtin_nontemporal_store(mtx_f4v{h0.x, h0.y, h0.z, h0.w}, reinterpret_cast<mtx_f4v*>(dst + lane));
+                wave_lds_sync();
+                if (lane >= 32u) {
+                    s_vo[2 * (lane - 32u)] = pos;
+                    s_vo[2 * (lane - 32u) + 1] = nrm;
+                }
+                wave_lds_sync();
+                const float4 h1 = s_vo[lane];
                 if (64 + lane < nh)
                     __builtin_nontemporal_store(mtx_f4v{h1.x, h1.y, h1.z, h1.w},
                                                 reinterpret_cast<mtx_f4v*>(dst + 64 + lane));
