@@ -1479,6 +1479,46 @@ __device__ __forceinline__ void first_user_of(uint64_t e, uint32_t u, uint32_t k
     }
 }
 
+// The first-user table by distance class: for each neighbour pattern nb,
+// edge u and class cls of the distance k back to the meshlet's start (the
+// classes of k between consecutive sharing distances, as s_cls gives them),
+// one byte: the index i + 1 of the first user's distance back_dist(i) (0:
+// the cell itself) | the edge's owner index there << 4.  s_bd[i + 1] holds
+// back_dist(i), s_bd[0] = 0.  The same answer as first_user_of for every k of
+// the class (the qualifying distances are the same), in one byte read.
+template <int B>
+__device__ __forceinline__ void first_user_bytes(uint8_t* s_fu8, uint8_t* s_bd)
+{
+    static_assert(B >= 3, "the nine sharing distances are distinct");
+    if (threadIdx.x < 16u)
+        s_bd[threadIdx.x] = (uint8_t)(threadIdx.x >= 1u && threadIdx.x <= 9u ? back_dist<B>((int)threadIdx.x - 1) : 0u);
+    for (uint32_t x = threadIdx.x; x < 64u * 12u; x += blockDim.x) {
+        const uint32_t nb = x / 12u, u = x % 12u;
+        uint32_t d[3] = { 0, 0, 0 }, uo[3] = { 0, 0, 0 };
+        uint32_t n = 0;
+        for_sharing_cells<B>(u, nb, [&](int delta, uint32_t o) {
+            if (delta < 0) {
+                d[n] = (uint32_t)(-delta);
+                uo[n] = o;
+                ++n;
+            }
+        });
+        for (uint32_t cls = 0; cls < kBackClasses; ++cls) {
+            const uint32_t k = back_class_rep<B>(cls);
+            uint32_t back = 0, u1 = u;
+            for (int f = 0; f < 3; ++f)
+                if (d[f] != 0u && d[f] <= k && d[f] > back) {
+                    back = d[f];
+                    u1 = uo[f];
+                }
+            uint32_t idx = 0;
+            for (int i = 0; i < 9; ++i)
+                if (back != 0u && back_dist<B>(i) == back) idx = (uint32_t)i + 1u;
+            s_fu8[x * kBackClasses + cls] = (uint8_t)(idx | (u1 << 4));
+        }
+    }
+}
+
 // Vertices a cell adds to a meshlet that started k cells before it.
 template <int B>
 __device__ __forceinline__ uint32_t added_edges(uint32_t emu, uint32_t nb, uint32_t k, const uint16_t* s_newtab,
@@ -2110,6 +2150,7 @@ template <int B, bool NORMALS, typename T> struct VertCfg {
     // 4 waves per SIMD, and 23 us more at 1024^3
     static constexpr int LIST = 256;
     static constexpr int WAVE_LDS = TILE * (int)sizeof(T) + LIST * 2 + 16 + 32 * 32;
+    static constexpr int MIN_WAVES = B == 8 ? 5 : 1;   // per SIMD; larger tiles stop at fewer
     static constexpr int WPG = WAVE_LDS * 4 <= 65536 ? 4 : (WAVE_LDS * 2 <= 65536 ? 2 : 1);
 };
 
@@ -2163,8 +2204,8 @@ struct LaneCorners {
 // the block's vertices (corner, axis) in prefix order and let the 64 lanes
 // take consecutive entries: balanced work and coalesced writes.
 template <int B, typename T, bool NORMALS, bool BATCH>
-__global__ __launch_bounds__((64 * VertCfg<B, NORMALS, T>::WPG)) __attribute__((amdgpu_waves_per_eu(5, 8)))
-void k_extract_vertices(ExtractArgs a)
+__global__ __launch_bounds__((64 * VertCfg<B, NORMALS, T>::WPG))
+__attribute__((amdgpu_waves_per_eu(VertCfg<B, NORMALS, T>::MIN_WAVES, 8))) void k_extract_vertices(ExtractArgs a)
 {
     using C = VertCfg<B, NORMALS, T>;
     constexpr int A = C::A, E = C::E, NTILE = C::NTILE, WPG = C::WPG;
@@ -3133,7 +3174,7 @@ template <int B> struct TriMeshCfg {
     static constexpr bool LISTED = B == 8;
     static constexpr int WAVE_LDS = NF * 5 + NC * 4 + (KIN ? 0 : NC * 2) + TL * 4 + (LISTED ? NC * 4 : 0);
     static constexpr int WG_LDS = 256 + 1280 * 8 + 4 * NFACE + 512 + 64 * (int)kBackClasses * 2 + (B * B + B) +
-                                  64 * 12 * 8;
+                                  (LISTED ? 64 * 12 * (int)kBackClasses + 16 : 64 * 12 * 8);
     static constexpr int WPG = WAVE_LDS * 4 + WG_LDS <= 65536 ? 4 : (WAVE_LDS * 2 + WG_LDS <= 65536 ? 2 : 1);
     static constexpr bool FITS = WAVE_LDS * WPG + WG_LDS <= 65536 && sizeof(TDec) == 4;
 };
@@ -3153,6 +3194,8 @@ __global__ __launch_bounds__(64 * TriMeshCfg<B>::WPG) void k_tri_meshlets(Extrac
     constexpr int CPL = EB<B>::CPL;
     constexpr uint32_t LIST = C::LIST, NW = StartBits<B>::NW;
     constexpr bool KIN = C::KIN, LISTED = C::LISTED;
+    // B = 8 with packed corners: the first-user table by distance class
+    constexpr bool FU8 = LISTED && PK;
     constexpr bool PACKN = CPL * 3 <= 32;
     constexpr int NFACE = C::NFACE, NPL = TriBlock<B>::NPL;
     using TDec = typename C::TDec;
@@ -3166,7 +3209,9 @@ __global__ __launch_bounds__(64 * TriMeshCfg<B>::WPG) void k_tri_meshlets(Extrac
     __shared__ uint16_t s_emu[256];
     __shared__ uint16_t s_newtab[64 * kBackClasses];
     __shared__ uint8_t s_cls[B * B + B];
-    __shared__ uint64_t s_fu[64 * 12];          // first-user table
+    __shared__ uint64_t s_fu[FU8 ? 1 : 64 * 12];                  // first-user table
+    __shared__ uint8_t s_fu8[FU8 ? 64 * 12 * kBackClasses : 1];   // ... by distance class
+    __shared__ uint8_t s_bd[FU8 ? 16 : 1];
     // per wave: the first vertex and edge flags of every owner corner of
     // [0, B]^3; each cell's first meshlet vertex in the block (20 bits) | its
     // added edges << 20, and neighbour bits; the cells' distance back to their
@@ -3206,7 +3251,8 @@ __global__ __launch_bounds__(64 * TriMeshCfg<B>::WPG) void k_tri_meshlets(Extrac
                     (((qx - B * sx) + B * ((qy - B * sy) + B * (qz - B * sz))) << 16);
     }
     meshlet_tables<B>(s_emu, s_newtab, s_cls);
-    first_user_table<B>(s_fu);
+    if constexpr (FU8) first_user_bytes<B>(s_fu8, s_bd);
+    else first_user_table<B>(s_fu);
     __syncthreads();
     const uint32_t lane = lane_id(), w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const CornerLds<PK> cor{ s_gidx_all[w], s_gfl_all[PK ? 0 : w] };
@@ -3450,6 +3496,8 @@ __global__ __launch_bounds__(64 * TriMeshCfg<B>::WPG) void k_tri_meshlets(Extrac
                         const uint32_t tu = (uint32_t)(du >> 32);
                         const uint32_t nbc = nbrs_of_cell<B>(c, vlim);
                         const uint32_t kk = KIN ? e >> (CB + 11) : (uint32_t)s_mk[c];
+                        // the distance class of kk (FU8)
+                        const uint32_t fcls = kk >= (uint32_t)(B * B + B) ? kBackClasses - 1u : (uint32_t)s_cls[kk];
                         const uint32_t q0 = c % B + F * ((c / B) % B + F * (c / (B * B)));
                         uint32_t gi[3], mv[3];
 #pragma unroll
@@ -3461,7 +3509,13 @@ __global__ __launch_bounds__(64 * TriMeshCfg<B>::WPG) void k_tri_meshlets(Extrac
                             // meshlet: the edge's first user in the meshlet and its rank there
                             const uint32_t u = (tu >> (4 * e3)) & 15u;
                             uint32_t back, u1;
-                            first_user_of(s_fu[nbc * 12u + u], u, kk, back, u1);
+                            if constexpr (FU8) {
+                                const uint32_t fe = s_fu8[(nbc * 12u + u) * kBackClasses + fcls];
+                                back = s_bd[fe & 15u];
+                                u1 = fe >> 4;
+                            } else {
+                                first_user_of(s_fu[nbc * 12u + u], u, kk, back, u1);
+                            }
                             const uint32_t mc = s_mc[c - back];
                             mv[e3] = msb + (mc & 0xFFFFFu) + __popc((mc >> 20) & ((1u << u1) - 1u));
                         }
