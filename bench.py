@@ -197,9 +197,13 @@ class Path:
 
 def time_path(path, steps, warmup, flush=None):
     """hipEvent times of `steps` steps after `warmup`: the bracket over all of
-    them, and each step's and its min/max launch's own events.  With `flush`,
-    a 512 MiB device write runs before each step, outside the step's events
-    (and inside the bracket, which then is not a per-step figure)."""
+    them, each step's time (from its own start event to the next step's, the
+    last to the closing event) and its min/max launch's time (events before
+    and after that launch).  An event recorded inside a step is a marker the
+    GPU stops at (a few microseconds each on MI355X), so a step carries just
+    these two.  With `flush`, a 512 MiB device write runs before each step,
+    outside the step's events (and inside the bracket, which then is not a
+    per-step figure)."""
     import statistics
     from meshtrex_amd import hiprt
     s = path.stream
@@ -208,18 +212,22 @@ def time_path(path, steps, warmup, flush=None):
             flush()
         path.step()
     hiprt.stream_sync(s)
-    evs = [[hiprt.Event() for _ in range(3)] for _ in range(steps)]
+    evs = [[hiprt.Event() for _ in range(2)] for _ in range(steps)]
+    ends = [hiprt.Event() for _ in range(steps)] if flush else None
     start, stop = hiprt.Event(), hiprt.Event()
     w0 = time.perf_counter()
     start.record(s)
     for k in range(steps):
         if flush:
             flush()
-        path.step((evs[k][0], evs[k][1], None, None, evs[k][2]))
+        path.step((evs[k][0], evs[k][1], None, None, ends[k] if flush else None))
     stop.record(s)
     stop.sync()
     wall = time.perf_counter() - w0
-    per = [e[2].ms_since(e[0]) for e in evs]
+    if flush:
+        per = [ends[k].ms_since(evs[k][0]) for k in range(steps)]
+    else:
+        per = [(evs[k + 1][0] if k + 1 < steps else stop).ms_since(evs[k][0]) for k in range(steps)]
     mm = [e[1].ms_since(e[0]) for e in evs]
     return {"bracket_ms": stop.ms_since(start) / steps, "median_ms": statistics.median(per),
             "mean_ms": statistics.fmean(per), "min_ms": min(per), "max_ms": max(per),
