@@ -1487,33 +1487,37 @@ __device__ __forceinline__ void first_user_of(uint64_t e, uint32_t u, uint32_t k
 // back_dist(i), s_bd[0] = 0.  The same answer as first_user_of for every k of
 // the class (the qualifying distances are the same), in one byte read.
 template <int B>
-__device__ __forceinline__ void first_user_bytes(uint8_t* s_fu8, uint8_t* s_bd)
+__device__ __forceinline__ void first_user_bytes(uint8_t* s_fu8, uint8_t* s_bd, const uint8_t* s_cls)
 {
     static_assert(B >= 3, "the nine sharing distances are distinct");
+    constexpr uint32_t KF = (uint32_t)(B * B + B);
     if (threadIdx.x < 16u)
         s_bd[threadIdx.x] = (uint8_t)(threadIdx.x >= 1u && threadIdx.x <= 9u ? back_dist<B>((int)threadIdx.x - 1) : 0u);
+    __syncthreads();   // s_cls, written by meshlet_tables before this call
     for (uint32_t x = threadIdx.x; x < 64u * 12u; x += blockDim.x) {
         const uint32_t nb = x / 12u, u = x % 12u;
-        uint32_t d[3] = { 0, 0, 0 }, uo[3] = { 0, 0, 0 };
+        // each earlier sharing cell: the class of its distance (the distance's
+        // index + 1, as the distances are distinct and sorted) and the edge's
+        // owner index there
+        uint32_t ci[3] = { 0, 0, 0 }, uo[3] = { 0, 0, 0 };
         uint32_t n = 0;
         for_sharing_cells<B>(u, nb, [&](int delta, uint32_t o) {
             if (delta < 0) {
-                d[n] = (uint32_t)(-delta);
+                const uint32_t d = (uint32_t)(-delta);
+                ci[n] = d >= KF ? kBackClasses - 1u : (uint32_t)s_cls[d];
                 uo[n] = o;
                 ++n;
             }
         });
+        // a sharing cell is in a meshlet that started k back iff its
+        // distance's class is at most k's; the first user is the farthest
         for (uint32_t cls = 0; cls < kBackClasses; ++cls) {
-            const uint32_t k = back_class_rep<B>(cls);
-            uint32_t back = 0, u1 = u;
+            uint32_t idx = 0, u1 = u;
             for (int f = 0; f < 3; ++f)
-                if (d[f] != 0u && d[f] <= k && d[f] > back) {
-                    back = d[f];
+                if (ci[f] != 0u && ci[f] <= cls && ci[f] > idx) {
+                    idx = ci[f];
                     u1 = uo[f];
                 }
-            uint32_t idx = 0;
-            for (int i = 0; i < 9; ++i)
-                if (back != 0u && back_dist<B>(i) == back) idx = (uint32_t)i + 1u;
             s_fu8[x * kBackClasses + cls] = (uint8_t)(idx | (u1 << 4));
         }
     }
@@ -3251,7 +3255,7 @@ __global__ __launch_bounds__(64 * TriMeshCfg<B>::WPG) void k_tri_meshlets(Extrac
                     (((qx - B * sx) + B * ((qy - B * sy) + B * (qz - B * sz))) << 16);
     }
     meshlet_tables<B>(s_emu, s_newtab, s_cls);
-    if constexpr (FU8) first_user_bytes<B>(s_fu8, s_bd);
+    if constexpr (FU8) first_user_bytes<B>(s_fu8, s_bd, s_cls);
     else first_user_table<B>(s_fu);
     __syncthreads();
     const uint32_t lane = lane_id(), w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
