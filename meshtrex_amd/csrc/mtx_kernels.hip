@@ -2212,7 +2212,7 @@ __global__ __launch_bounds__((64 * VertCfg<B, NORMALS, T>::WPG))
 __attribute__((amdgpu_waves_per_eu(VertCfg<B, NORMALS, T>::MIN_WAVES, 8))) void k_extract_vertices(ExtractArgs a)
 {
     using C = VertCfg<B, NORMALS, T>;
-    constexpr int A = C::A, E = C::E, NTILE = C::NTILE, WPG = C::WPG;
+    constexpr int A = C::A, E = C::E, WPG = C::WPG;
     constexpr uint32_t RP = C::RP, NS = C::NS, NSEG = C::NSEG, ND = C::ND;
     constexpr uint32_t LIST = C::LIST;
     constexpr int CPL = EB<B>::CPL;
@@ -2226,7 +2226,6 @@ __attribute__((amdgpu_waves_per_eu(VertCfg<B, NORMALS, T>::MIN_WAVES, 8))) void 
     float4* s_vo = s_vo_all[w];
     const uint32_t na = active_items(a);
     const Entry* __restrict__ table = static_cast<const Entry*>(a.corner_table);
-    constexpr int LPL = (NTILE + 63) / 64;
     const WaveItems it = cost_items(a, na, WPG, a.blk_vbase, a.totals[0]);
     // the next block's count, bases, ID, batch unit and corner entries load
     // while this block's tile loads and its vertices are computed: one memory
@@ -2333,26 +2332,45 @@ __attribute__((amdgpu_waves_per_eu(VertCfg<B, NORMALS, T>::MIN_WAVES, 8))) void 
             }
             __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the segments are in LDS
         } else {
-            constexpr int NLD = LPL < 11 ? LPL : 11;
-#pragma unroll 1
-            for (int k0 = 0; k0 < LPL; k0 += NLD) {
-                T v[NLD];
+            // a block at a face of the volume: the same row segments, each row
+            // clamped into the volume (clamp-to-edge); a segment that lies
+            // inside its row loads straight to LDS, one that crosses x = 0 or
+            // x = X is filled voxel by voxel with x clamped
+            uint32_t fix = 0;   // bit k: this lane's segment k crosses a row end
 #pragma unroll
-                for (int k = 0; k < NLD; ++k) {
-                    const uint32_t j = min(lane + 64u * (k0 + k), (uint32_t)NTILE - 1u);
-                    int32_t x = (int32_t)(ox + j % E) - A, y = (int32_t)(oy + (j / E) % E) - A;
-                    int32_t z = (int32_t)(oz + j / (E * E)) - A;
-                    x = min(max(x, 0), (int32_t)X - 1);
-                    y = min(max(y, 0), (int32_t)Y - 1);
-                    z = min(max(z, 0), (int32_t)Z - 1);
-                    v[k] = vol[(uint32_t)x + (uint64_t)X * ((uint32_t)y + (uint64_t)Y * (uint32_t)z)];
-                }
-#pragma unroll
-                for (int k = 0; k < NLD; ++k) {
-                    const uint32_t j = lane + 64u * (k0 + k);
-                    if (j < (uint32_t)NTILE) s_val[j % E + RP * (j / E)] = v[k];
+            for (uint32_t k = 0; k < ND; ++k) {
+                const uint32_t m = min(lane + 64u * k, NSEG - 1u);
+                const uint32_t r = m / NS;
+                const uint64_t gy = (uint64_t)min(max((int32_t)(oy + r % (uint32_t)E) - A, 0), (int32_t)Y - 1);
+                const uint64_t gz = (uint64_t)min(max((int32_t)(oz + r / (uint32_t)E) - A, 0), (int32_t)Z - 1);
+                const int32_t gx0 = (int32_t)(ox + 4u * (m % NS)) - A;
+                if (gx0 >= 0 && gx0 + 4 <= (int32_t)X) {
+                    const T* src = vol + ((uint64_t)gx0 + (uint64_t)X * (gy + (uint64_t)Y * gz));
+                    if constexpr (C::SEGB == 16) lds_direct_load16(src, s_val + 256u * k);
+                    else lds_direct_load4(src, s_val + 256u * k);
+                } else {
+                    fix |= 1u << k;
                 }
             }
+            if (fix) {
+#pragma unroll
+                for (uint32_t k = 0; k < ND; ++k) {
+                    if (!((fix >> k) & 1u)) continue;
+                    const uint32_t m = min(lane + 64u * k, NSEG - 1u);
+                    const uint32_t r = m / NS;
+                    const uint64_t gy = (uint64_t)min(max((int32_t)(oy + r % (uint32_t)E) - A, 0), (int32_t)Y - 1);
+                    const uint64_t gz = (uint64_t)min(max((int32_t)(oz + r / (uint32_t)E) - A, 0), (int32_t)Z - 1);
+                    const int32_t gx0 = (int32_t)(ox + 4u * (m % NS)) - A;
+                    const T* row = vol + (uint64_t)X * (gy + (uint64_t)Y * gz);
+                    T v4[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) v4[j] = row[min(max(gx0 + j, 0), (int32_t)X - 1)];
+                    // the slot the segment's direct load would have filled
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) s_val[256u * k + 4u * lane + (uint32_t)j] = v4[j];
+                }
+            }
+            __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the direct segments are in LDS
         }
         for (uint32_t c0 = 0; c0 < bnv; c0 += LIST) {
 #pragma unroll
