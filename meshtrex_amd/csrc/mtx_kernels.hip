@@ -1873,14 +1873,22 @@ __global__ __launch_bounds__(256) void k_extract_count_b8(ExtractArgs a)
     uint16_t* __restrict__ table = static_cast<uint16_t*>(a.corner_table);
     const uint32_t ly = lane & 7u, lz = lane >> 3;
     const WaveItems it = wave_items(na, 4);
+    // the next block's ID (and batch unit) load while this block is processed:
+    // its tile loads then wait on one memory round trip, not two or three
+    uint32_t id_next = it.first < it.end ? a.ids[it.first] : 0u;
+    uint32_t unit_next = BATCH && it.first < it.end ? a.item_unit[it.first] : 0u;
     for (uint32_t i = it.first; i < it.end; i += it.stride) {
-        uint32_t unit;
-        const ItemVol V = item_vol<BATCH>(a, i, unit);
+        const uint32_t id = __builtin_amdgcn_readfirstlane(id_next);
+        const uint32_t unit = __builtin_amdgcn_readfirstlane(unit_next);
+        if (i + it.stride < it.end) {
+            id_next = a.ids[i + it.stride];
+            if constexpr (BATCH) unit_next = a.item_unit[i + it.stride];
+        }
+        const ItemVol V = item_vol_of<BATCH>(a, unit);
         const T* __restrict__ vol = static_cast<const T*>(V.data);
         const uint32_t X = V.X, Y = V.Y, Z = V.Z;
         const bool degen = X < 2 || Y < 2 || Z < 2;
         const bool off32 = (uint64_t)X * Y * E < 0x80000000ull;
-        const uint32_t id = a.ids[i];
         const uint32_t bx = id % V.gx, by = (id / V.gx) % V.gy;
         const uint32_t bz = id / (V.gx * V.gy);
         const uint32_t ox = bx * B, oy = by * B, oz = bz * B;
