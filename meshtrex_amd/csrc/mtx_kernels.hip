@@ -3635,7 +3635,7 @@ __global__ __launch_bounds__(256) void k_meshlet_vertices(const float4* __restri
 // multiple of 8 workgroups for the XCD split, never more waves than blocks).
 // Measured on MI355X: a single volume is fastest with whole rounds of the
 // resident workgroups -- 10 per CU for the vertex and triangle passes (5
-// resident each), 24 for the count pass (8 resident, the extra rounds even out
+// resident each at B = 8), 24 for the count pass (the extra rounds even out
 // the blocks' load times), 8 for the other block sizes' count pass; fewer
 // leave CUs idle while long blocks finish, more add per-workgroup set-up.  A
 // batch, whose blocks differ far more in work (dense and sparse surfaces of
@@ -3719,9 +3719,11 @@ static hipError_t launch_vertices_b(const ExtractArgs& a, hipStream_t s)
 
 // Vertex pass, then triangles (and meshlets).  The two depend only on the
 // count pass and the scan, so with a side stream the vertex pass (bound by
-// its tile fetch) runs there beside the triangle pass (bound by LDS) and `s`
-// joins it before anything that reads the vertices: measured at 1024^3,
-// 33 us off the extraction.
+// its tile fetch) runs there beside the triangle pass (bound by its vector
+// ALU) and `s` joins it before anything that reads the vertices.  They share
+// the CUs only as far as LDS allows (three triangle + meshlet workgroups
+// fill a CU's), so the overlap is partial: 20-30 us off the extraction at
+// 1024^3 on one GPU box.
 template <int B, bool BATCH>
 static hipError_t launch_emit_b(const ExtractArgs& a, hipStream_t s)
 {
