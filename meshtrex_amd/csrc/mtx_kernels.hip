@@ -3635,14 +3635,19 @@ __global__ __launch_bounds__(256) void k_meshlet_vertices(const float4* __restri
 // multiple of 8 workgroups for the XCD split, never more waves than blocks).
 // Measured on MI355X: a single volume is fastest with whole rounds of the
 // resident workgroups -- 10 per CU for the vertex and triangle passes (5
-// resident each at B = 8), 24 for the count pass (the extra rounds even out
-// the blocks' load times), 8 for the other block sizes' count pass; fewer
+// resident each at B = 8), for the B = 8 count pass 24, or on grids of 2^20
+// blocks and more 96 (48 with the meshlet cut: the extra rounds even out the
+// blocks' load times; measured on one GPU box against 24 at 1024^3: -16 and
+// -2.5 us), 8 for the other block sizes' count pass; fewer
 // leave CUs idle while long blocks finish, more add per-workgroup set-up.  A
 // batch, whose blocks differ far more in work (dense and sparse surfaces of
 // many units), is fastest with finer static shares, 32-128 per CU.
 constexpr uint32_t kWgPerCuSingle = 8;
 constexpr uint32_t kWgPerCuSingleVertices = 10;
 constexpr uint32_t kWgPerCuSingleCount = 24;
+constexpr uint32_t kWgPerCuLargeCount = 96;
+constexpr uint32_t kWgPerCuLargeCountCut = 48;
+constexpr uint64_t kLargeGridBlocks = 1ull << 20;
 constexpr uint32_t kWgPerCuSingleTriangles = 10;
 constexpr uint32_t kWgPerCuBatch = 64;
 constexpr uint32_t kWgPerCuMeshlet = 8;
@@ -3689,14 +3694,15 @@ template <int B, bool BATCH>
 static hipError_t launch_count_b(const ExtractArgs& a, hipStream_t s)
 {
     if (B == 8) {
+        const bool large = a.max_blocks >= kLargeGridBlocks;
         if (!BATCH && a.starts) {   // with the meshlet cut
-            if (a.vol.dtype == kF32)
-                return launch_persistent(k_extract_count_b8<float, false, true>, 256, a, s, kWgPerCuSingleCount);
-            return launch_persistent(k_extract_count_b8<uint8_t, false, true>, 256, a, s, kWgPerCuSingleCount);
+            const uint32_t g = large ? kWgPerCuLargeCountCut : kWgPerCuSingleCount;
+            if (a.vol.dtype == kF32) return launch_persistent(k_extract_count_b8<float, false, true>, 256, a, s, g);
+            return launch_persistent(k_extract_count_b8<uint8_t, false, true>, 256, a, s, g);
         }
-        if (a.vol.dtype == kF32)
-            return launch_persistent(k_extract_count_b8<float, BATCH, false>, 256, a, s, kWgPerCuSingleCount);
-        return launch_persistent(k_extract_count_b8<uint8_t, BATCH, false>, 256, a, s, kWgPerCuSingleCount);
+        const uint32_t g = large ? kWgPerCuLargeCount : kWgPerCuSingleCount;
+        if (a.vol.dtype == kF32) return launch_persistent(k_extract_count_b8<float, BATCH, false>, 256, a, s, g);
+        return launch_persistent(k_extract_count_b8<uint8_t, BATCH, false>, 256, a, s, g);
     }
     if (a.vol.dtype == kF32) return launch_persistent(k_extract_count<B, float, BATCH>, 256, a, s);
     return launch_persistent(k_extract_count<B, uint8_t, BATCH>, 256, a, s);
