@@ -3634,8 +3634,12 @@ __global__ __launch_bounds__(256) void k_meshlet_vertices(const float4* __restri
 // Persistent grids of the extraction kernels, in workgroups per CU (a
 // multiple of 8 workgroups for the XCD split, never more waves than blocks).
 // Measured on MI355X: a single volume is fastest with whole rounds of the
-// resident workgroups -- 10 per CU for the vertex and triangle passes (5
-// resident each at B = 8), for the B = 8 count pass 24, or on grids of 2^20
+// resident workgroups -- 10 per CU for the triangle pass, 64 for the vertex
+// pass (5 resident at B = 8: with many more workgroups than fit, the
+// dispatcher hands them to CUs as others finish, which balances the passes'
+// uneven blocks and fills the CUs the triangle + meshlet pass frees; same-box
+// sweep at 1024^3: 10 -> 64 per CU took 28 us off both extractions), for
+// the B = 8 count pass 24, or on grids of 2^20
 // blocks and more 96 (48 with the meshlet cut: the extra rounds even out the
 // blocks' load times; measured on one GPU box against 24 at 1024^3: -16 and
 // -2.5 us), 8 for the other block sizes' count pass; fewer
@@ -3643,7 +3647,7 @@ __global__ __launch_bounds__(256) void k_meshlet_vertices(const float4* __restri
 // batch, whose blocks differ far more in work (dense and sparse surfaces of
 // many units), is fastest with finer static shares, 32-128 per CU.
 constexpr uint32_t kWgPerCuSingle = 8;
-constexpr uint32_t kWgPerCuSingleVertices = 10;
+constexpr uint32_t kWgPerCuSingleVertices = 64;
 constexpr uint32_t kWgPerCuSingleCount = 24;
 constexpr uint32_t kWgPerCuLargeCount = 96;
 constexpr uint32_t kWgPerCuLargeCountCut = 48;
