@@ -3732,8 +3732,8 @@ static hipError_t launch_vertices_b(const ExtractArgs& a, hipStream_t s)
 // its tile fetch) runs there beside the triangle pass (bound by its vector
 // ALU) and `s` joins it before anything that reads the vertices.  They share
 // the CUs only as far as LDS allows (three triangle + meshlet workgroups
-// fill a CU's), so the overlap is partial: 20-30 us off the extraction at
-// 1024^3 on one GPU box.
+// fill a CU's), so the overlap is partial: 16 us off the extraction at
+// 1024^3 (A/B on one GPU box).
 template <int B, bool BATCH>
 static hipError_t launch_emit_b(const ExtractArgs& a, hipStream_t s)
 {
