@@ -3634,7 +3634,9 @@ __global__ __launch_bounds__(256) void k_meshlet_vertices(const float4* __restri
 // Persistent grids of the extraction kernels, in workgroups per CU (a
 // multiple of 8 workgroups for the XCD split, never more waves than blocks).
 // Measured on MI355X: a single volume is fastest with whole rounds of the
-// resident workgroups -- 10 per CU for the triangle pass, 64 for the vertex
+// resident workgroups -- one round for the triangle passes (each workgroup
+// sets up LDS tables; same-box sweep of the shared-mesh pass at 1024^3: one
+// round of 6 per CU 8 us faster than 10 per CU), 64 for the vertex
 // pass (5 resident at B = 8: with many more workgroups than fit, the
 // dispatcher hands them to CUs as others finish, which balances the passes'
 // uneven blocks and fills the CUs the triangle + meshlet pass frees; same-box
@@ -3652,7 +3654,6 @@ constexpr uint32_t kWgPerCuSingleCount = 24;
 constexpr uint32_t kWgPerCuLargeCount = 96;
 constexpr uint32_t kWgPerCuLargeCountCut = 48;
 constexpr uint64_t kLargeGridBlocks = 1ull << 20;
-constexpr uint32_t kWgPerCuSingleTriangles = 10;
 constexpr uint32_t kWgPerCuBatch = 64;
 constexpr uint32_t kWgPerCuMeshlet = 8;
 
@@ -3762,10 +3763,8 @@ static hipError_t launch_emit_b(const ExtractArgs& a, hipStream_t s)
     }
     if (!fused) {
         e = a.vcap < kPackedIndexLimit
-                ? launch_persistent(k_extract_triangles<B, BATCH, true>, 64 * TriCfg<B>::WPG, a, s,
-                                    kWgPerCuSingleTriangles)
-                : launch_persistent(k_extract_triangles<B, BATCH, false>, 64 * TriCfg<B>::WPG, a, s,
-                                    kWgPerCuSingleTriangles);
+                ? launch_persistent(k_extract_triangles<B, BATCH, true>, 64 * TriCfg<B>::WPG, a, s, 0)
+                : launch_persistent(k_extract_triangles<B, BATCH, false>, 64 * TriCfg<B>::WPG, a, s, 0);
         // meshlets (not in the batched form)
         if (e == hipSuccess && !BATCH && a.starts)
             e = launch_persistent(k_meshlet_emit<B>, 64 * EmitCfg<B>::WPG, a, s, kWgPerCuMeshlet);
